@@ -1,0 +1,132 @@
+/*
+ * ppesdg.h -- C ABI of the MI355X PP-ES nodal DG step for 2D ideal MHD.
+ *
+ * Plain pointers and sizes only. Every call returns an int status (PPES_OK or
+ * one of the codes below); none of them stops the process. Fortran host code
+ * binds these through ISO_C_BINDING (pp-es-dg-mhd_amd/fortran/ppesdg_capi.f90).
+ *
+ * Host arrays use the reference's Fortran layouts (column-major):
+ *   state     (8, N, N, 0:nx+1, 0:ny+1)   u_con / uRK_con(...,s)   DG_solver.f90:6-14
+ *   residual  (8, N, N, nx, ny)           ut_con(...,s)
+ *   averages  (8, 0:nx+1, 0:ny+1)         u_avg
+ * with N = p + 1 (px = py = p, 1 <= p <= 4). On the device the state is kept
+ * as structure-of-arrays planes; the library converts at the boundary.
+ *
+ * State slots: 0 = u_con (the committed solution), 1..rk_stages = uRK_con(s).
+ * A solver handle owns one HIP stream and its device memory; handles are not
+ * thread-safe.
+ */
+#ifndef PPESDG_H
+#define PPESDG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PPES_OK 0
+#define PPES_NEG_AVG 1      /* inadmissible cell average: retry with smaller dt (drive.f90:285-290) */
+#define PPES_DT_COLLAPSE 2  /* retries exhausted or dt_lim < 1e-14 (drive.f90:97-99) */
+#define PPES_PP_FAIL 3      /* PP limiter fallback exhausted (DG_solver.f90:582-586) */
+#define PPES_BAD_ARG 4
+#define PPES_HIP_ERROR 5
+#define PPES_OOM 6
+#define PPES_UNSUPPORTED 7
+
+/* boundary kinds, applied on the device */
+#define PPES_BC_PERIODIC 0  /* BC_periodic      problems.f90:312-348 */
+#define PPES_BC_EXTRAP 1    /* BC_YeeSjogreen   problems.f90:386-423 (trace extrapolation) */
+#define PPES_BC_FIXED 2     /* BC_rotor         problems.f90:597-654 (fixed conservative state) */
+#define PPES_BC_HOST 3      /* ghosts supplied by the caller through ppes_upload_state */
+
+/* built-in initial data (nodal point sampling as init_by_fun_pri, problems.f90:25-51) */
+#define PPES_PROB_OT2PI 0     /* Orszag-Tang on [0,2pi]^2: rho=g^2, p=g, u=(-sin y, sin x), B=(-sin y, sin 2x) */
+#define PPES_PROB_ROTOR 1     /* Rotor_init_pri     problems.f90:170-199 */
+#define PPES_PROB_BLAST 2     /* blast on [-0.5,0.5]^2, p=1000 for r<0.1, B=(100/sqrt(4pi))(1,1,0)/sqrt2 */
+#define PPES_PROB_OTUNIT 3    /* OrszagTang_init_pri problems.f90:155-168 */
+#define PPES_PROB_BLASTREF 4  /* Blast_init_pri     problems.f90:201-218 */
+#define PPES_PROB_OTPERT 5    /* OT2PI plus a smooth perturbation (pert table, amplitude) */
+
+typedef struct ppes_config {
+    int nx, ny;            /* elements */
+    int px, py;            /* polynomial degree; px == py required */
+    int rk_stages;         /* 1, 2, 3 or 5 (SSP-RK(1,1),(2,2),(3,3),(5,4)) */
+    double left, right, bottom, top;
+    double gamma;
+    int bc_kind;           /* PPES_BC_* */
+    double bc_state[8];    /* conservative state for PPES_BC_FIXED */
+} ppes_config;
+
+typedef struct ppes_solver ppes_solver;
+
+typedef struct ppes_run_stats {
+    double t;              /* final time */
+    double dt_last;        /* last dt */
+    double dt_lim;         /* dt_lim of the final state */
+    double tot_ent;        /* total entropy after the last step */
+    long long steps, retries, negcells, stages;
+    int status;
+    int pad;
+} ppes_run_stats;
+
+/* init_DG_solver (DG_solver.f90:47-107): allocate device state, build tables. */
+int ppes_create(const ppes_config *cfg, ppes_solver **out);
+/* finalize_DG_solver (DG_solver.f90:109-123) */
+int ppes_destroy(ppes_solver *s);
+/* set_gamma (MHD_PDE.f90:5-10), per solver */
+int ppes_set_gamma(ppes_solver *s, double gamma);
+
+/* geometry (DG_solver.f90:19-24,93-106): hx, hy, grid_x(nx+1), grid_y(ny+1),
+ * xx(N,nx), yy(N,ny); any pointer may be NULL */
+int ppes_geometry(const ppes_solver *s, double *hx, double *hy, double *grid_x, double *grid_y,
+                  double *xx, double *yy);
+
+/* host <-> device state in the reference layouts */
+int ppes_upload_state(ppes_solver *s, int slot, const double *u);
+int ppes_download_state(ppes_solver *s, int slot, double *u);
+int ppes_upload_residual(ppes_solver *s, int slot, const double *ut);
+int ppes_download_residual(ppes_solver *s, int slot, double *ut);
+int ppes_download_avg(ppes_solver *s, double *u_avg);
+
+/* Init_handle => init_by_fun_pri with a built-in problem (problems.f90:25-51).
+ * pert: nk rows of (kx, ky, phi, psi) for PPES_PROB_OTPERT, else NULL. */
+int ppes_init_problem(ppes_solver *s, int prob, const double *pert, int nk, double amp);
+
+/* BC_handle on a state slot (drive.f90:68,277,291) for the device BC kinds */
+int ppes_apply_bc(ppes_solver *s, int slot);
+
+/* post_process (DG_solver.f90:128-661) on a state slot. ent_offset must be 0
+ * (the reference's option is off in every shipped case). Returns
+ * PPES_NEG_AVG / PPES_PP_FAIL as well as setting the flags. */
+int ppes_post_process(ppes_solver *s, int slot, int apply_oedg, int apply_ldf, double oedg_dt,
+                      int ent_offset, int comput_ent, int *neg_avg, int *num_negcell, double *tot_ent);
+
+/* DG_semidiscretize (DG_solver.f90:663-892): ut_con(...,ut_slot) = L(u slot) */
+int ppes_semidiscretize(ppes_solver *s, int ut_slot, int u_slot, double *dt_lim);
+
+/* SSP-RK stage combine of stage `stage` into uRK(stage) (drive.f90:130-272) */
+int ppes_rk_combine(ppes_solver *s, int stage, double dt);
+
+/* solve_execute (drive.f90:32-343) from the state in slot 0: initial BC,
+ * post-process and residual, then steps until t_final (> 0) or nsteps.
+ * oedg_mode: 0 off, 1 on the final RK stage with oedg_dt = dt (drive.f90:280),
+ * 2 on every stage with oedg_dt = beta_s dt (drive.f90:283). */
+int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, int use_ldf,
+               int ent_offset, ppes_run_stats *stats);
+
+/* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
+ * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major). */
+int ppes_tables(int p, double *x, double *w, double *D, double *Pmn, double *Pnm);
+/* The LDF projection (LDF.f90:114-250 with the Piola maps of
+ * DG_solver.f90:237-264) as one dense nodal operator on (Bx - avg, By - avg):
+ * op(2N^2, 2N^2) column-major (host only). */
+int ppes_ldf_nodal_operator(int p, double hx, double hy, double *op);
+
+/* wait for the solver's stream */
+int ppes_sync(ppes_solver *s);
+const char *ppes_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -55,6 +55,8 @@ def lib():
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(Summary)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_log_avg_divisor.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_log_avg_divisor.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -83,6 +85,10 @@ def ldf_modal_operator(p):
     op = np.zeros((m, m), order="F")
     lib().orc_ldf_modal_operator(p, _dp(op))
     return op
+
+
+def log_avg_divisor(L, R):
+    return lib().orc_log_avg_divisor(float(L), float(R))
 
 
 def set_threads(n):

@@ -415,6 +415,8 @@ static inline double log_avg_divisor(double LL, double RR)
     return (0.5 * log(zeta)) / f;
 }
 
+double orc_log_avg_divisor(double LL, double RR) { return log_avg_divisor(LL, RR); }
+
 /* primitive node record: rho, vx, vy, vz, bx, by, bz, beta */
 typedef struct { double rho, vx, vy, vz, bx, by, bz, beta; } prim_t;
 
@@ -643,7 +645,8 @@ static void pri_blastref(double x, double y, double *up)
     up[7] = ((x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) < 0.01) ? 1000.0 : 0.1;
 }
 /* OT-2pi plus a smooth perturbation: drho = amp sum_k cos(k.x + phi_k)/n,
- * B += curl(A_z), A_z = amp sum_k sin(k.x + psi_k)/(|k| n) */
+ * B += curl(A_z), A_z = amp sum_k sin(k.x + psi_k)/(|k| n). The cosines are
+ * evaluated in the separable form cos(kx x)cos(ky y + phi) - sin(kx x)sin(ky y + phi). */
 static void pri_otpert(double x, double y, double g, const double *pert, int nk, double amp, double *up)
 {
     pri_ot2pi(x, y, g, up);
@@ -651,8 +654,9 @@ static void pri_otpert(double x, double y, double g, const double *pert, int nk,
     for (int q = 0; q < nk; ++q) {
         double kx = pert[4 * q], ky = pert[4 * q + 1], phi = pert[4 * q + 2], psi = pert[4 * q + 3];
         double kn = sqrt(kx * kx + ky * ky);
-        drho += cos(kx * x + ky * y + phi);
-        double c = cos(kx * x + ky * y + psi) / kn;
+        double cx = cos(kx * x), sx = sin(kx * x);
+        drho += cx * cos(ky * y + phi) - sx * sin(ky * y + phi);
+        double c = (cx * cos(ky * y + psi) - sx * sin(ky * y + psi)) / kn;
         dbx += ky * c;
         dby -= kx * c;
     }

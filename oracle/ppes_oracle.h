@@ -72,6 +72,9 @@ typedef struct orc_summary {
 int orc_run(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_mode, int use_ldf,
             orc_summary *out);
 
+/* log_avg_divisor (DG_solver.f90:1653-1681), exported for its closed-form test */
+double orc_log_avg_divisor(double LL, double RR);
+
 /* thread count used by the element loops (0 = OpenMP default) */
 void orc_set_threads(int n);
 

@@ -1,0 +1,207 @@
+// Point operators of the PP-ES nodal DG step for 2D ideal MHD (device side).
+//
+// Each function keeps the evaluation order of the reference expression it
+// implements (Fortran left-to-right association), and the library is built
+// with -ffp-contract=off, so that apart from last-bit differences of log/exp
+// the device results equal the CPU oracle's.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ppes {
+
+constexpr double PP_EPS = 1.0e-11;    // DG_solver.f90:17
+constexpr double OEDG_EPS = 1.0e-8;   // DG_solver.f90:148
+constexpr double DBL_HUGE = 1.79769313486231571e+308;
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+// fluid pressure of a conservative state (_macro_prs_, DG_solver.f90:125)
+__device__ __forceinline__ double prs_macro(double rho, double mx, double my, double mz, double bx,
+                                            double by, double bz, double e, double g)
+{
+    return (g - 1.0) * (e - 0.5 * ((mx * mx + my * my + mz * mz) / rho + (bx * bx + by * by + bz * bz)));
+}
+
+// mathematical entropy (_macro_ent_, DG_solver.f90:126)
+__device__ __forceinline__ double ent_macro(double rho, double prs, double g)
+{
+    return -((rho * (log(prs) - g * log(rho))) / (g - 1.0));
+}
+
+// divisor turning the arithmetic mean into the logarithmic mean
+// (log_avg_divisor, DG_solver.f90:1653-1681)
+__device__ __forceinline__ double log_avg_divisor(double LL, double RR)
+{
+    const double a0 = 1.0, a1 = 1.0 / 3.0, a2 = 1.0 / 5.0, a3 = 1.0 / 7.0;
+    double zeta = LL / RR;
+    double f = (zeta - 1.0) / (zeta + 1.0);
+    double u = f * f;
+    if (u < 1.4e-4) return a0 + u * (a1 + u * (a2 + u * a3));
+    return (0.5 * log(zeta)) / f;
+}
+
+// Node primitives used by the volume flux differencing:
+// rho, vx, vy, vz, bx, by, bz, beta = rho / (2 p) (DG_solver.f90:739-760).
+struct Prim {
+    double rho, vx, vy, vz, bx, by, bz, beta;
+};
+
+// Entropy-conservative two-point flux in x (dir 0) or y (dir 1)
+// (ec_flux_x / ec_flux_y, DG_solver.f90:1571-1650). Component 4 (x) or 5 (y)
+// is identically zero and is returned as 0.
+template <int DIR>
+__device__ __forceinline__ void ec_flux(const Prim &L, const Prim &R, double g, double f[8])
+{
+    double rho_bar = 0.5 * (L.rho + R.rho);
+    double vx_bar = 0.5 * (L.vx + R.vx);
+    double vy_bar = 0.5 * (L.vy + R.vy);
+    double vz_bar = 0.5 * (L.vz + R.vz);
+    double bx_bar = 0.5 * (L.bx + R.bx);
+    double by_bar = 0.5 * (L.by + R.by);
+    double bz_bar = 0.5 * (L.bz + R.bz);
+    double beta_bar = 0.5 * (L.beta + R.beta);
+    double v_sq_bar = 0.5 * ((L.vx * L.vx + L.vy * L.vy + L.vz * L.vz) + (R.vx * R.vx + R.vy * R.vy + R.vz * R.vz));
+    double b_sq_bar = 0.5 * ((L.bx * L.bx + L.by * L.by + L.bz * L.bz) + (R.bx * R.bx + R.by * R.by + R.bz * R.bz));
+    double betavx_bar = 0.5 * (L.beta * L.vx + R.beta * R.vx);
+    double betavy_bar = 0.5 * (L.beta * L.vy + R.beta * R.vy);
+    double betavz_bar = 0.5 * (L.beta * L.vz + R.beta * R.vz);
+    double rho_hat = rho_bar / log_avg_divisor(L.rho, R.rho);
+    double beta_hat = beta_bar / log_avg_divisor(L.beta, R.beta);
+    if (DIR == 0) {
+        f[0] = rho_hat * vx_bar;
+        f[1] = vx_bar * f[0] - bx_bar * bx_bar + 0.5 * (b_sq_bar + rho_bar / beta_bar);
+        f[2] = vy_bar * f[0] - bx_bar * by_bar + 0.0;
+        f[3] = vz_bar * f[0] - bx_bar * bz_bar + 0.0;
+        f[4] = 0.0;
+        f[5] = (betavx_bar * by_bar - betavy_bar * bx_bar) / beta_bar;
+        f[6] = (betavx_bar * bz_bar - betavz_bar * bx_bar) / beta_bar;
+        f[7] = 0.5 * (1.0 / ((g - 1.0) * beta_hat) - v_sq_bar) * f[0] + (vx_bar * f[1] + vy_bar * f[2] + vz_bar * f[3]) +
+               (bx_bar * f[4] + by_bar * f[5] + bz_bar * f[6]) - 0.5 * vx_bar * b_sq_bar +
+               (vx_bar * bx_bar + vy_bar * by_bar + vz_bar * bz_bar) * bx_bar;
+    } else {
+        f[0] = rho_hat * vy_bar;
+        f[1] = vx_bar * f[0] - by_bar * bx_bar + 0.0;
+        f[2] = vy_bar * f[0] - by_bar * by_bar + 0.5 * (b_sq_bar + rho_bar / beta_bar);
+        f[3] = vz_bar * f[0] - by_bar * bz_bar + 0.0;
+        f[4] = (betavy_bar * bx_bar - betavx_bar * by_bar) / beta_bar;
+        f[5] = 0.0;
+        f[6] = (betavy_bar * bz_bar - betavz_bar * by_bar) / beta_bar;
+        f[7] = 0.5 * (1.0 / ((g - 1.0) * beta_hat) - v_sq_bar) * f[0] + (vx_bar * f[1] + vy_bar * f[2] + vz_bar * f[3]) +
+               (bx_bar * f[4] + by_bar * f[5] + bz_bar * f[6]) - 0.5 * vy_bar * b_sq_bar +
+               (vx_bar * bx_bar + vy_bar * by_bar + vz_bar * bz_bar) * by_bar;
+    }
+}
+
+// Positivity-preserving, entropy-stable HLL flux in x with the Godunov-Powell
+// normal-field average (ESPP_HLL_flux_x, DG_solver.f90:894-1071). Output:
+// nflux[8], nBn, and the signal speeds sl <= 0 <= sr.
+__device__ __forceinline__ void espp_hll_flux_x(const double wl[8], const double wr[8], double g, double nflux[8],
+                                                double &nBn, double &sl_out, double &sr_out)
+{
+    double rho_l = wl[0], mx_l = wl[1], my_l = wl[2], mz_l = wl[3], bx_l = wl[4], by_l = wl[5], bz_l = wl[6], e_l = wl[7];
+    double rho_r = wr[0], mx_r = wr[1], my_r = wr[2], mz_r = wr[3], bx_r = wr[4], by_r = wr[5], bz_r = wr[6], e_r = wr[7];
+    double inv_rho_l = 1.0 / rho_l, inv_rho_r = 1.0 / rho_r;
+    double vx_l = inv_rho_l * mx_l, vx_r = inv_rho_r * mx_r;
+    double vy_l = inv_rho_l * my_l, vy_r = inv_rho_r * my_r;
+    double vz_l = inv_rho_l * mz_l, vz_r = inv_rho_r * mz_r;
+    double e_knt_l = 0.5 * inv_rho_l * (mx_l * mx_l + my_l * my_l + mz_l * mz_l);
+    double e_knt_r = 0.5 * inv_rho_r * (mx_r * mx_r + my_r * my_r + mz_r * mz_r);
+    double ex_l = by_l * vz_l - bz_l * vy_l, ex_r = by_r * vz_r - bz_r * vy_r;
+    double ey_l = bz_l * vx_l - bx_l * vz_l, ey_r = bz_r * vx_r - bx_r * vz_r;
+    double ez_l = bx_l * vy_l - by_l * vx_l, ez_r = bx_r * vy_r - by_r * vx_r;
+    double b_sq_l = bx_l * bx_l + by_l * by_l + bz_l * bz_l;
+    double b_sq_r = bx_r * bx_r + by_r * by_r + bz_r * bz_r;
+    double p_mag_l = 0.5 * b_sq_l, p_mag_r = 0.5 * b_sq_r;
+    double p_fld_l = dmax((g - 1.0) * (e_l - (e_knt_l + p_mag_l)), PP_EPS);
+    double p_fld_r = dmax((g - 1.0) * (e_r - (e_knt_r + p_mag_r)), PP_EPS);
+    double p_tot_l = p_fld_l + p_mag_l, p_tot_r = p_fld_r + p_mag_r;
+    double b_dot_u_l = bx_l * vx_l + by_l * vy_l + bz_l * vz_l;
+    double b_dot_u_r = bx_r * vx_r + by_r * vy_r + bz_r * vz_r;
+    double snc_sq_l = g * p_fld_l * inv_rho_l, snc_sq_r = g * p_fld_r * inv_rho_r;
+    double alfven_sq_l = b_sq_l * inv_rho_l, alfven_sq_r = b_sq_r * inv_rho_r;
+    double sum_sq_l = snc_sq_l + alfven_sq_l, sum_sq_r = snc_sq_r + alfven_sq_r;
+    double cfx_l = sqrt(0.5 * (sum_sq_l + sqrt(dmax(sum_sq_l * sum_sq_l - 4.0 * snc_sq_l * inv_rho_l * (bx_l * bx_l), 0.0))));
+    double cfx_r = sqrt(0.5 * (sum_sq_r + sqrt(dmax(sum_sq_r * sum_sq_r - 4.0 * snc_sq_r * inv_rho_r * (bx_r * bx_r), 0.0))));
+    double inv_theta_l = rho_l / p_fld_l, inv_theta_r = rho_r / p_fld_r;
+    double phi_l = inv_theta_l * b_dot_u_l, phi_r = inv_theta_r * b_dot_u_r;
+    double sp_ent_l = log(p_fld_l) - g * log(rho_l);
+    double sp_ent_r = log(p_fld_r) - g * log(rho_r);
+    double ent_fun_l = -((rho_l * sp_ent_l) / (g - 1.0));
+    double ent_fun_r = -((rho_r * sp_ent_r) / (g - 1.0));
+    double ent_flux_l = ent_fun_l * vx_l, ent_flux_r = ent_fun_r * vx_r;
+    double fl[8], fr[8], vl[8], vr[8];
+    fl[0] = mx_l; fl[1] = vx_l * mx_l - bx_l * bx_l + p_tot_l; fl[2] = vx_l * my_l - by_l * bx_l;
+    fl[3] = vx_l * mz_l - bz_l * bx_l; fl[4] = 0.0; fl[5] = -ez_l; fl[6] = ey_l;
+    fl[7] = vx_l * (e_l + p_tot_l) - bx_l * b_dot_u_l;
+    fr[0] = mx_r; fr[1] = vx_r * mx_r - bx_r * bx_r + p_tot_r; fr[2] = vx_r * my_r - by_r * bx_r;
+    fr[3] = vx_r * mz_r - bz_r * bx_r; fr[4] = 0.0; fr[5] = -ez_r; fr[6] = ey_r;
+    fr[7] = vx_r * (e_r + p_tot_r) - bx_r * b_dot_u_r;
+    vl[0] = (g - sp_ent_l) / (g - 1.0) - e_knt_l / p_fld_l;
+    vl[1] = mx_l / p_fld_l; vl[2] = my_l / p_fld_l; vl[3] = mz_l / p_fld_l;
+    vl[4] = inv_theta_l * bx_l; vl[5] = inv_theta_l * by_l; vl[6] = inv_theta_l * bz_l; vl[7] = -inv_theta_l;
+    vr[0] = (g - sp_ent_r) / (g - 1.0) - e_knt_r / p_fld_r;
+    vr[1] = mx_r / p_fld_r; vr[2] = my_r / p_fld_r; vr[3] = mz_r / p_fld_r;
+    vr[4] = inv_theta_r * bx_r; vr[5] = inv_theta_r * by_r; vr[6] = inv_theta_r * bz_r; vr[7] = -inv_theta_r;
+    double sqrt_rho_l = sqrt(rho_l), sqrt_rho_r = sqrt(rho_r);
+    double sum_sqrt_rho = sqrt_rho_l + sqrt_rho_r;
+    double vx_roe = (vx_l * sqrt_rho_l + vx_r * sqrt_rho_r) / sum_sqrt_rho;
+    double db_aux = sqrt((bx_l - bx_r) * (bx_l - bx_r) + (by_l - by_r) * (by_l - by_r) + (bz_l - bz_r) * (bz_l - bz_r)) / sum_sqrt_rho;
+    double sl_pp = dmin(vx_l, vx_roe) - cfx_l - db_aux;
+    double sr_pp = dmax(vx_r, vx_roe) + cfx_r + db_aux;
+    double dot_a = 0.0, dot_bl = 0.0, dot_br = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dot_a += (vr[i] - vl[i]) * (wr[i] - wl[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dot_bl += vl[i] * (fr[i] - fl[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dot_br += vr[i] * (fr[i] - fl[i]);
+    double a_form = dmax(dot_a, 0.0) + 1e-8;
+    double bl_form = (ent_flux_r - ent_flux_l) - dot_bl - phi_l * (bx_r - bx_l);
+    double br_form = (ent_flux_r - ent_flux_l) - dot_br - phi_r * (bx_r - bx_l);
+    double al_form = dmax(bl_form / a_form, 0.0);
+    double ar_form = dmax(br_form / a_form, 0.0);
+    double aa_form = sqrt(al_form * ar_form);
+    double sl_es = -(ar_form + aa_form);
+    double sr_es = +(al_form + aa_form);
+    double sl = dmin(dmin(sl_es, sl_pp), 0.0);
+    double sr = dmax(dmax(sr_es, sr_pp), 0.0);
+    double dslr = sr - sl;
+    double wei_l = sr / dslr;
+    double wei_r = -sl / dslr;
+    double c = (sl * sr) / dslr;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) nflux[i] = wei_l * fl[i] + wei_r * fr[i] + c * (wr[i] - wl[i]);
+    nBn = wei_l * bx_l + wei_r * bx_r;
+    sl_out = sl;
+    sr_out = sr;
+}
+
+// x <-> y component swap (swap_xy, DG_solver.f90:1556-1568)
+__device__ __forceinline__ void swap_xy(const double a[8], double b[8])
+{
+    b[0] = a[0]; b[1] = a[2]; b[2] = a[1]; b[3] = a[3];
+    b[4] = a[5]; b[5] = a[4]; b[6] = a[6]; b[7] = a[7];
+}
+
+// fast magnetosonic spectral radius in x and y at one node
+// (DG_solver.f90:749-770 and :456-486)
+__device__ __forceinline__ void node_speeds(double rho, double mx, double my, double mz, double bx, double by,
+                                            double bz, double e_tot, double g, double &sx, double &sy)
+{
+    double inv_rho = 1.0 / rho;
+    double vx = inv_rho * mx, vy = inv_rho * my;
+    double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
+    double b_sq = bx * bx + by * by + bz * bz;
+    double p_mag = 0.5 * b_sq;
+    double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
+    double snc_sq = g * p_fld * inv_rho;
+    double alfven_sq = b_sq * inv_rho;
+    double sum_sq = snc_sq + alfven_sq;
+    double sum_sqsq = sum_sq * sum_sq;
+    double coef = 4.0 * snc_sq * inv_rho;
+    sx = fabs(vx) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sqsq - coef * (bx * bx), 0.0)), 0.0));
+    sy = fabs(vy) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sqsq - coef * (by * by), 0.0)), 0.0));
+}
+
+}  // namespace ppes

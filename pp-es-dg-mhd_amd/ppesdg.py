@@ -1,0 +1,209 @@
+"""ctypes binding of libppesdg.so, the MI355X PP-ES nodal DG step for 2D MHD.
+
+The class ``Solver`` mirrors the reference's module API (DG_solver.f90,
+drive.f90, problems.f90): ``post_process``, ``semidiscretize``,
+``rk_combine``, ``apply_bc``, ``solve`` (= solve_execute). Host arrays use the
+reference's Fortran layouts: state (8, N, N, nx+2, ny+2), residual
+(8, N, N, nx, ny), averages (8, nx+2, ny+2).
+
+There is no CPU fallback: if the HIP library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libppesdg.so")
+
+OK, NEG_AVG, DT_COLLAPSE, PP_FAIL, BAD_ARG, HIP_ERROR, OOM, UNSUPPORTED = range(8)
+BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3}
+PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5}
+
+_lib = None
+
+
+class PPESError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: status {code}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("px", ctypes.c_int), ("py", ctypes.c_int),
+                ("rk_stages", ctypes.c_int), ("left", ctypes.c_double), ("right", ctypes.c_double),
+                ("bottom", ctypes.c_double), ("top", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("bc_kind", ctypes.c_int), ("bc_state", ctypes.c_double * 8)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("dt_last", ctypes.c_double), ("dt_lim", ctypes.c_double),
+                ("tot_ent", ctypes.c_double), ("steps", ctypes.c_longlong), ("retries", ctypes.c_longlong),
+                ("negcells", ctypes.c_longlong), ("stages", ctypes.c_longlong), ("status", ctypes.c_int),
+                ("pad", ctypes.c_int)]
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `make -C pp-es-dg-mhd_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        vp = ctypes.c_void_p
+        sig = {
+            "ppes_create": [ctypes.POINTER(Config), ctypes.POINTER(vp)],
+            "ppes_destroy": [vp],
+            "ppes_set_gamma": [vp, ctypes.c_double],
+            "ppes_geometry": [vp, dp, dp, dp, dp, dp, dp],
+            "ppes_upload_state": [vp, ctypes.c_int, dp],
+            "ppes_download_state": [vp, ctypes.c_int, dp],
+            "ppes_upload_residual": [vp, ctypes.c_int, dp],
+            "ppes_download_residual": [vp, ctypes.c_int, dp],
+            "ppes_download_avg": [vp, dp],
+            "ppes_init_problem": [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double],
+            "ppes_apply_bc": [vp, ctypes.c_int],
+            "ppes_post_process": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.c_int, ip, ip, dp],
+            "ppes_semidiscretize": [vp, ctypes.c_int, ctypes.c_int, dp],
+            "ppes_rk_combine": [vp, ctypes.c_int, ctypes.c_double],
+            "ppes_solve": [vp, ctypes.c_double, ctypes.c_longlong, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, ctypes.POINTER(RunStats)],
+            "ppes_tables": [ctypes.c_int, dp, dp, dp, dp, dp],
+            "ppes_ldf_nodal_operator": [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp],
+            "ppes_sync": [vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.ppes_status_string.argtypes = [ctypes.c_int]
+        L.ppes_status_string.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, what, allow=(OK,)):
+    if rc not in allow:
+        raise PPESError(rc, f"{what} ({lib().ppes_status_string(rc).decode()})")
+    return rc
+
+
+def tables(p):
+    """gauss_lobatto / gauss_lobatto_matrix tables of degree p (host only)."""
+    n = p + 1
+    x, w = np.zeros(n), np.zeros(n)
+    D, Pmn, Pnm = (np.zeros((n, n), order="F") for _ in range(3))
+    _check(lib().ppes_tables(p, _dp(x), _dp(w), _dp(D), _dp(Pmn), _dp(Pnm)), "ppes_tables")
+    return dict(x=x, w=w, D=D, Pmn=Pmn, Pnm=Pnm)
+
+
+def ldf_nodal_operator(p, hx, hy):
+    m = 2 * (p + 1) ** 2
+    op = np.zeros((m, m), order="F")
+    _check(lib().ppes_ldf_nodal_operator(p, hx, hy, _dp(op)), "ppes_ldf_nodal_operator")
+    return op
+
+
+class Solver:
+    """One device-resident instance (init_DG_solver ... finalize_DG_solver)."""
+
+    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
+        cfg = Config()
+        cfg.nx, cfg.ny, cfg.px, cfg.py, cfg.rk_stages = nx, ny, p, p, rk
+        cfg.left, cfg.right, cfg.bottom, cfg.top = [float(v) for v in domain]
+        cfg.gamma = float(gamma)
+        cfg.bc_kind = BC[bc]
+        if bc_state is not None:
+            for v in range(8):
+                cfg.bc_state[v] = float(bc_state[v])
+        h = ctypes.c_void_p()
+        _check(lib().ppes_create(ctypes.byref(cfg), ctypes.byref(h)), "ppes_create")
+        self.h = h
+        self.nx, self.ny, self.p, self.N, self.rk = nx, ny, p, p + 1, rk
+        N = self.N
+        self.shape_u = (8, N, N, nx + 2, ny + 2)
+        self.shape_ut = (8, N, N, nx, ny)
+        self.shape_avg = (8, nx + 2, ny + 2)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ppes_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def geometry(self):
+        hx, hy = ctypes.c_double(), ctypes.c_double()
+        gx, gy = np.zeros(self.nx + 1), np.zeros(self.ny + 1)
+        xx, yy = np.zeros((self.N, self.nx), order="F"), np.zeros((self.N, self.ny), order="F")
+        _check(lib().ppes_geometry(self.h, ctypes.byref(hx), ctypes.byref(hy), _dp(gx), _dp(gy), _dp(xx), _dp(yy)),
+               "ppes_geometry")
+        return dict(hx=hx.value, hy=hy.value, grid_x=gx, grid_y=gy, xx=xx, yy=yy)
+
+    def upload_state(self, slot, u):
+        a = np.asfortranarray(u, dtype=np.float64)
+        assert a.shape == self.shape_u
+        _check(lib().ppes_upload_state(self.h, slot, _dp(a)), "ppes_upload_state")
+
+    def download_state(self, slot=0):
+        a = np.zeros(self.shape_u, order="F")
+        _check(lib().ppes_download_state(self.h, slot, _dp(a)), "ppes_download_state")
+        return a
+
+    def upload_residual(self, slot, ut):
+        a = np.asfortranarray(ut, dtype=np.float64)
+        assert a.shape == self.shape_ut
+        _check(lib().ppes_upload_residual(self.h, slot, _dp(a)), "ppes_upload_residual")
+
+    def download_residual(self, slot):
+        a = np.zeros(self.shape_ut, order="F")
+        _check(lib().ppes_download_residual(self.h, slot, _dp(a)), "ppes_download_residual")
+        return a
+
+    def download_avg(self):
+        a = np.zeros(self.shape_avg, order="F")
+        _check(lib().ppes_download_avg(self.h, _dp(a)), "ppes_download_avg")
+        return a
+
+    def init_problem(self, prob, pert=None, amp=0.0):
+        if pert is None:
+            _check(lib().ppes_init_problem(self.h, PROB[prob], None, 0, 0.0), "ppes_init_problem")
+        else:
+            pt = np.ascontiguousarray(pert, dtype=np.float64)
+            _check(lib().ppes_init_problem(self.h, PROB[prob], _dp(pt), len(pt), float(amp)), "ppes_init_problem")
+
+    def apply_bc(self, slot):
+        _check(lib().ppes_apply_bc(self.h, slot), "ppes_apply_bc")
+
+    def post_process(self, slot, apply_oedg, apply_ldf, oedg_dt, comput_ent=True):
+        neg, nneg, ent = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = lib().ppes_post_process(self.h, slot, int(apply_oedg), int(apply_ldf), float(oedg_dt), 0,
+                                     int(comput_ent), ctypes.byref(neg), ctypes.byref(nneg), ctypes.byref(ent))
+        _check(rc, "ppes_post_process", allow=(OK, NEG_AVG, PP_FAIL))
+        return dict(rc=rc, neg_avg=bool(neg.value), num_negcell=nneg.value, tot_ent=ent.value)
+
+    def semidiscretize(self, ut_slot, u_slot):
+        dtl = ctypes.c_double()
+        _check(lib().ppes_semidiscretize(self.h, ut_slot, u_slot, ctypes.byref(dtl)), "ppes_semidiscretize")
+        return dtl.value
+
+    def rk_combine(self, stage, dt):
+        _check(lib().ppes_rk_combine(self.h, stage, float(dt)), "ppes_rk_combine")
+
+    def solve(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1, use_ldf=True):
+        st = RunStats()
+        rc = lib().ppes_solve(self.h, float(t_final), int(nsteps), float(cfl), int(oedg_mode), int(use_ldf), 0,
+                              ctypes.byref(st))
+        _check(rc, "ppes_solve", allow=(OK, DT_COLLAPSE, PP_FAIL, NEG_AVG))
+        return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
+
+    def sync(self):
+        _check(lib().ppes_sync(self.h), "ppes_sync")

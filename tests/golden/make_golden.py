@@ -1,0 +1,126 @@
+"""Write the golden fixtures from the reference solver (run in the build
+container only: it needs /root/reference compiled by `make -C oracle ref`).
+
+Every fixture is produced by oracle/_ref/ref_harness, which drives the
+reference's own modules through their public API (see oracle/ref_harness.f90).
+The fixtures are data only: inputs, outputs and the harness arguments.
+
+    python tests/golden/make_golden.py [--long]
+
+--long also runs the full-size Orszag-Tang case (P2, 128x128, t = 0.5),
+which takes a few minutes on one core.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def load_dump(d):
+    out = {}
+    for line in open(os.path.join(d, "manifest.txt")):
+        f = line.split()
+        if not f:
+            continue
+        shp = tuple(int(x) for x in f[2:])
+        out[f[0]] = np.fromfile(os.path.join(d, f[0] + ".bin"), dtype="<f8").reshape(shp, order="F")
+    return out
+
+
+def run(args, keep=None):
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [HARNESS] + [f"{k}={v}" for k, v in args.items()] + [f"out={d}"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"{cmd} failed:\n{res.stdout}\n{res.stderr}")
+        data = load_dump(d)
+        data["_stdout"] = res.stdout
+    if keep is not None:
+        data = {k: v for k, v in data.items() if any(k == p or k.startswith(p) for p in keep) or k == "_stdout"}
+    return data
+
+
+def save(name, args, data):
+    arrays = {k: v for k, v in data.items() if not k.startswith("_")}
+    arrays["args_json"] = np.array(json.dumps(args))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    size = os.path.getsize(os.path.join(HERE, name + ".npz"))
+    print(f"{name}.npz  {size / 1024:.0f} KiB  ({len(arrays)} arrays)")
+
+
+STAGE_CASES = {
+    # name: harness arguments (first step dumped stage by stage, then the final state)
+    "stages_ot2pi_p2_ldf0": dict(mode="stages", prob="ot2pi", nx=8, ny=6, p=2, nsteps=3, oedg=1, ldf=0, cfl=0.2, dumpsteps=1),
+    "stages_ot2pi_p2": dict(mode="stages", prob="ot2pi", nx=8, ny=6, p=2, nsteps=3, oedg=1, ldf=1, cfl=0.2, dumpsteps=1),
+    "stages_rotor_p3_ldf0": dict(mode="stages", prob="rotor", nx=7, ny=6, p=3, nsteps=3, oedg=2, ldf=0, cfl=0.15, dumpsteps=1),
+    "stages_rotor_p3": dict(mode="stages", prob="rotor", nx=7, ny=6, p=3, nsteps=3, oedg=2, ldf=1, cfl=0.15, dumpsteps=1),
+    "stages_blast_p1": dict(mode="stages", prob="blast", nx=9, ny=8, p=1, nsteps=3, oedg=2, ldf=1, cfl=0.15, dumpsteps=1),
+    "stages_blast_p2_ldf0": dict(mode="stages", prob="blast", nx=8, ny=7, p=2, nsteps=3, oedg=2, ldf=0, cfl=0.15, dumpsteps=1),
+    "stages_blast_p4": dict(mode="stages", prob="blast", nx=5, ny=4, p=4, nsteps=2, oedg=2, ldf=1, cfl=0.15, dumpsteps=1),
+    "stages_otunit_p3": dict(mode="stages", prob="otunit", nx=6, ny=6, p=3, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
+}
+
+END_CASES = {
+    # reduced blast (config 3 shape: OEDG every stage, extrapolation BCs) to t = 0.002
+    "end_blast_p2_40": dict(mode="stages", prob="blast", nx=40, ny=40, p=2, nsteps=0, tfinal=0.002, oedg=2, ldf=1, cfl=0.15, dumpsteps=0),
+    # reference solve_execute itself (final-stage OEDG), rotor P2 32x32 to t = 0.01
+    "end_rotor_p2_32": dict(mode="solve", prob="rotor", nx=32, ny=32, p=2, tfinal=0.01, oedg=1, ldf=1, cfl=0.2),
+    # Orszag-Tang on [0,2pi]^2, P1 and P4 on small grids to t = 0.1
+    "end_ot2pi_p1_32": dict(mode="solve", prob="ot2pi", nx=32, ny=32, p=1, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
+    "end_ot2pi_p4_12": dict(mode="solve", prob="ot2pi", nx=12, ny=12, p=4, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
+}
+
+LONG_CASES = {
+    # the CPU-runnable configuration: OT 2pi, P2, 128x128, CFL 0.2, t = 0.5
+    "end_ot2pi_p2_128": dict(mode="solve", prob="ot2pi", nx=128, ny=128, p=2, tfinal=0.5, oedg=1, ldf=1, cfl=0.2),
+}
+
+
+def steps_from_stdout(txt):
+    for line in txt.splitlines():
+        if line.strip().startswith("# time steps"):
+            return int(line.split("=")[1])
+    return -1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--long", action="store_true")
+    ap.add_argument("--only", default="")
+    a = ap.parse_args()
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    # tables of gauss.f90 and the LDF projector of LDF.f90, p = 1..6
+    if not a.only or a.only == "tables":
+        save("tables", {"mode": "tables"}, run({"mode": "tables"}))
+    cases = dict(STAGE_CASES)
+    cases.update(END_CASES)
+    if a.long:
+        cases.update(LONG_CASES)
+    for name, args in cases.items():
+        if a.only and a.only != name:
+            continue
+        if name.startswith("end_"):
+            keep = ["u_final", "avg_final", "summary"]
+        else:
+            keep = ["u_init", "avg_init", "ent_init", "ut_init", "dtlim_init", "s1_", "u_final", "avg_final", "summary"]
+        data = run(args, keep)
+        if args["mode"] == "solve":
+            data["steps"] = np.array(steps_from_stdout(data["_stdout"]))
+        if name.startswith("end_"):
+            # interior density only keeps the fixture small
+            data["rho_final"] = data.pop("u_final")[0, :, :, 1:-1, 1:-1]
+            data["rho_avg_final"] = data.pop("avg_final")[0, 1:-1, 1:-1]
+        save(name, args, data)
+
+
+if __name__ == "__main__":
+    main()
