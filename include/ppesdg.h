@@ -76,6 +76,10 @@ int ppes_destroy(ppes_solver *s);
 /* set_gamma (MHD_PDE.f90:5-10), per solver */
 int ppes_set_gamma(ppes_solver *s, double gamma);
 
+/* choose the boundary kind after creation (the BC_handle binding of
+ * drive.f90:25); state = conservative ghost state for PPES_BC_FIXED */
+int ppes_set_bc(ppes_solver *s, int kind, const double *state);
+
 /* geometry (DG_solver.f90:19-24,93-106): hx, hy, grid_x(nx+1), grid_y(ny+1),
  * xx(N,nx), yy(N,ny); any pointer may be NULL */
 int ppes_geometry(const ppes_solver *s, double *hx, double *hy, double *grid_x, double *grid_y,
@@ -114,6 +118,24 @@ int ppes_rk_combine(ppes_solver *s, int stage, double dt);
 int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, int use_ldf,
                int ent_offset, ppes_run_stats *stats);
 
+/* The same loop in two parts: ppes_solve_begin runs the start-up
+ * (drive.f90:66-73); ppes_advance continues the step loop for nsteps steps
+ * (all steps if nsteps <= 0) or until t_final (> 0). Stats are cumulative. */
+int ppes_solve_begin(ppes_solver *s, int use_ldf);
+int ppes_advance(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, ppes_run_stats *stats);
+
+/* Per-kernel device time, measured with HIP events on the solver's stream
+ * while timing is on (no reference counterpart: measurement support). */
+#define PPES_K_RESIDUAL 0
+#define PPES_K_AVG_LDF 1
+#define PPES_K_OEDG 2
+#define PPES_K_LIMIT 3
+#define PPES_K_COMBINE 4
+#define PPES_K_BC 5
+#define PPES_NKERNELS 6
+int ppes_set_timing(ppes_solver *s, int on);
+int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *count);
+
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
  * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major). */
 int ppes_tables(int p, double *x, double *w, double *D, double *Pmn, double *Pnm);
@@ -121,6 +143,9 @@ int ppes_tables(int p, double *x, double *w, double *D, double *Pmn, double *Pnm
  * DG_solver.f90:237-264) as one dense nodal operator on (Bx - avg, By - avg):
  * op(2N^2, 2N^2) column-major (host only). */
 int ppes_ldf_nodal_operator(int p, double hx, double hy, double *op);
+/* LDF_projector%apply (LDF.f90:219-250) as the dense modal operator on the
+ * Legendre coefficients (Bx then By, x index fastest), 2N^2 x 2N^2 (host only) */
+int ppes_ldf_modal_operator(int p, double *op);
 
 /* wait for the solver's stream */
 int ppes_sync(ppes_solver *s);

@@ -335,10 +335,13 @@ __global__ void __launch_bounds__(BLOCK) k_pp_avg_ldf(KParams kp, double *__rest
         for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; scons[el][v][n] = q[v]; }
     }
     __syncthreads();
-    if (valid && n < 8) {
-        double a = 0.0;
-        for (int m = 0; m < NN; ++m) a = a + kp.wavg[m] * scons[el][n][m];
-        savg[el][n] = a;
+    // one lane per variable (a P1 element has only 4 lanes: stride over them)
+    if (valid) {
+        for (int v = n; v < 8; v += NN) {
+            double a = 0.0;
+            for (int m = 0; m < NN; ++m) a = a + kp.wavg[m] * scons[el][v][m];
+            savg[el][v] = a;
+        }
     }
     __syncthreads();
     double avg[8];
@@ -372,7 +375,8 @@ __global__ void __launch_bounds__(BLOCK) k_pp_avg_ldf(KParams kp, double *__rest
             u[5 * vs + base] = q[5];
         }
     }
-    if (valid && ok && n < 8) uavg[eg * 8 + n] = avg[n];
+    if (valid && ok)
+        for (int v = n; v < 8; v += NN) uavg[eg * 8 + v] = avg[v];
 
     if (partials == nullptr) return;
     // per-workgroup (sum, min, max) over interior elements, deterministic order

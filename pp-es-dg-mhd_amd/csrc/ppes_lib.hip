@@ -138,6 +138,21 @@ __global__ void k_reset(DevStatus *st, int mask)
 
 // ---------------------------------------------------------------- solver
 
+// host-side state of the drive loop between ppes_advance calls
+struct DriveState {
+    bool begun = false, flag = true;
+    int use_ldf = 1;
+    double t = 0.0, dt = 0.0, dt_lim = 0.0, tot_ent = 0.0;
+    double tRK[5] = {0, 0, 0, 0, 0};
+    long long steps = 0, retries = 0, negcells = 0, stages = 0, stages_retried = 0;
+};
+
+struct KTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+    double total_ms = 0.0;
+    long long count = 0;
+};
+
 struct ppes_solver {
     ppes_config cfg;
     int p, N, NN, S;
@@ -153,6 +168,53 @@ struct ppes_solver {
     DevStatus *h_st = nullptr;  // pinned mirror
     std::vector<double> qx, qw, D;
     std::vector<double> xx, yy;
+    DriveState drive;
+    bool timing = false;
+    KTimer timers[PPES_NKERNELS];
+};
+
+// Event-pair timer of one kernel class on the solver's stream
+static void harvest_timer(KTimer &k)
+{
+    for (auto &pr : k.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            k.total_ms += ms;
+            k.count++;
+        }
+        k.pool.push_back(pr);
+    }
+    k.pending.clear();
+}
+
+struct TimedLaunch {
+    ppes_solver *s;
+    int id;
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    TimedLaunch(ppes_solver *s_, int id_) : s(s_), id(id_)
+    {
+        if (!s->timing) return;
+        KTimer &k = s->timers[id];
+        if (k.pending.size() > 4096) {
+            (void)hipStreamSynchronize(s->stream);
+            harvest_timer(k);
+        }
+        if (k.pool.empty()) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            k.pool.push_back({a, b});
+        }
+        ev = k.pool.back();
+        k.pool.pop_back();
+        (void)hipEventRecord(ev.first, s->stream);
+    }
+    ~TimedLaunch()
+    {
+        if (!s->timing) return;
+        (void)hipEventRecord(ev.second, s->stream);
+        s->timers[id].pending.push_back(ev);
+    }
 };
 
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -222,6 +284,12 @@ static void fill_kparams(ppes_solver *s)
 
 static void free_all(ppes_solver *s)
 {
+    for (auto &k : s->timers) {
+        for (auto &pr : k.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+        for (auto &pr : k.pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+        k.pending.clear();
+        k.pool.clear();
+    }
     for (double *p : s->d_u) if (p) (void)hipFree(p);
     for (double *p : s->d_ut) if (p) (void)hipFree(p);
     double *bufs[] = {s->d_uavg, s->d_sigx, s->d_sigy, s->d_part, s->d_entpart, s->d_ldf, s->d_ghost0};
@@ -317,6 +385,23 @@ extern "C" int ppes_set_gamma(ppes_solver *s, double gamma)
     s->cfg.gamma = gamma;
     s->kp.g = gamma;
     return PPES_OK;
+}
+
+extern "C" int ppes_set_bc(ppes_solver *s, int kind, const double *state)
+{
+    if (!s || kind < PPES_BC_PERIODIC || kind > PPES_BC_HOST) return PPES_BAD_ARG;
+    if (kind == PPES_BC_FIXED && !state) return PPES_BAD_ARG;
+    s->cfg.bc_kind = kind;
+    for (int v = 0; v < 8; ++v) s->cfg.bc_state[v] = state ? state[v] : 0.0;
+    s->kp.bc_kind = kind;
+    for (int v = 0; v < 8; ++v) s->kp.bc_state[v] = s->cfg.bc_state[v];
+    return PPES_OK;
+}
+
+extern "C" int ppes_ldf_modal_operator(int p, double *op)
+{
+    if (!op) return PPES_BAD_ARG;
+    return host_ldf_modal(p, op) ? PPES_BAD_ARG : PPES_OK;
 }
 
 extern "C" int ppes_geometry(const ppes_solver *s, double *hx, double *hy, double *grid_x, double *grid_y,
@@ -482,6 +567,7 @@ static int launch_reset(ppes_solver *s, int mask)
 static int launch_bc(ppes_solver *s, double *u)
 {
     if (s->cfg.bc_kind == PPES_BC_HOST) return PPES_OK;
+    TimedLaunch tl(s, PPES_K_BC);
     const long long nthreads = (2LL * s->cfg.nx + 2LL * s->cfg.ny) * s->NN;
     DISPATCH_P(s->p, hipLaunchKernelGGL(k_bc<P>, dim3((unsigned)cdiv(nthreads, BLOCK)), dim3(BLOCK), 0, s->stream,
                                         s->kp, u, s->d_st));
@@ -501,6 +587,7 @@ static int launch_ghost_copy(ppes_solver *s, double *u, int dir)
 // residual of state `u` into `ut`; the caller resets dt_lim beforehand
 static int launch_residual(ppes_solver *s, const double *u, double *ut)
 {
+    TimedLaunch tl(s, PPES_K_RESIDUAL);
     DISPATCH_P(s->p, hipLaunchKernelGGL(k_residual<P>, dim3((unsigned)s->nblk_el), dim3(BLOCK), 0, s->stream, s->kp,
                                         u, ut, s->d_st));
     HIP_TRY(hipGetLastError());
@@ -510,11 +597,15 @@ static int launch_residual(ppes_solver *s, const double *u, double *ut)
 static int launch_post_process(ppes_solver *s, double *u, int apply_oedg, int apply_ldf, double oedg_dt,
                                int comput_ent)
 {
+    {
+    TimedLaunch tl(s, PPES_K_AVG_LDF);
     DISPATCH_P(s->p, hipLaunchKernelGGL(k_pp_avg_ldf<P>, dim3((unsigned)s->nblk_pp), dim3(BLOCK), 0, s->stream,
                                         s->kp, u, s->d_uavg, (const double *)s->d_ldf, apply_ldf,
                                         apply_oedg ? s->d_part : (double *)nullptr, s->d_st));
     HIP_TRY(hipGetLastError());
+    }
     if (apply_oedg) {
+        TimedLaunch tl(s, PPES_K_OEDG);
         hipLaunchKernelGGL(k_reduce_global, dim3(1), dim3(BLOCK), 0, s->stream, (const double *)s->d_part,
                            s->nblk_pp, s->d_st);
         HIP_TRY(hipGetLastError());
@@ -524,11 +615,14 @@ static int launch_post_process(ppes_solver *s, double *u, int apply_oedg, int ap
                                             (const DevStatus *)s->d_st));
         HIP_TRY(hipGetLastError());
     }
+    {
+    TimedLaunch tl(s, PPES_K_LIMIT);
     DISPATCH_P(s->p, hipLaunchKernelGGL(k_pp_limit<P>, dim3((unsigned)s->nblk_el), dim3(BLOCK), 0, s->stream, s->kp,
                                         u, (const double *)s->d_uavg, (const double *)s->d_sigx,
                                         (const double *)s->d_sigy, apply_oedg, oedg_dt, comput_ent, s->d_entpart,
                                         s->d_st));
     HIP_TRY(hipGetLastError());
+    }
     if (comput_ent) {
         hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(BLOCK), 0, s->stream, (const double *)s->d_entpart,
                            s->nblk_el, s->d_st);
@@ -568,6 +662,7 @@ static int launch_combine(ppes_solver *s, int st, double dt)
         }
     }
     const long long nthreads = (long long)s->cfg.nx * s->cfg.ny * s->NN * 8;
+    TimedLaunch tl(s, PPES_K_COMBINE);
     DISPATCH_P(s->p, hipLaunchKernelGGL(k_combine<P>, dim3((unsigned)cdiv(nthreads, BLOCK)), dim3(BLOCK), 0, s->stream,
                                         s->kp, a, (const DevStatus *)s->d_st));
     HIP_TRY(hipGetLastError());
@@ -650,21 +745,15 @@ extern "C" int ppes_rk_combine(ppes_solver *s, int stage, double dt)
     return PPES_OK;
 }
 
-// The drive loop (drive.f90:57-320), one host synchronisation per RK step.
-extern "C" int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, int use_ldf,
-                          int ent_offset, ppes_run_stats *stats)
+// ------------------------------------------------------------ drive loop
+
+// Begin a solve (drive.f90:66-73): BC, initial post-process, BC, the
+// committed solution's ghost layer saved, residual of u^n into ut slot 1.
+extern "C" int ppes_solve_begin(ppes_solver *s, int use_ldf)
 {
-    if (!s || !(cfl > 0) || oedg_mode < 0 || oedg_mode > 2) return PPES_BAD_ARG;
-    if (ent_offset) return PPES_UNSUPPORTED;
+    if (!s) return PPES_BAD_ARG;
     if (s->cfg.bc_kind == PPES_BC_HOST) return PPES_UNSUPPORTED;
-    ppes_run_stats loc;
-    memset(&loc, 0, sizeof loc);
-    ppes_run_stats *out = stats ? stats : &loc;
-    memset(out, 0, sizeof *out);
-    const int S = s->S;
-    const double cfl_rk = (S <= 3) ? 1.0 : 1.508;
     int rc;
-    // initial BC, post-process, BC, residual (drive.f90:66-73)
     if ((rc = launch_reset(s, 1 | 2 | 4))) return rc;
     if ((rc = launch_bc(s, s->d_u[0]))) return rc;
     if ((rc = launch_post_process(s, s->d_u[0], 0, use_ldf, 0.0, 1))) return rc;
@@ -675,52 +764,72 @@ extern "C" int ppes_solve(ppes_solver *s, double t_final, long long nsteps, doub
     if ((rc = launch_ghost_copy(s, s->d_u[0], 0))) return rc;
     if ((rc = launch_residual(s, s->d_u[0], s->d_ut[1]))) return rc;
     if ((rc = status_pull(s))) return rc;
-    if (s->h_st->neg_avg) { out->status = PPES_NEG_AVG; return PPES_NEG_AVG; }
-    double dt_lim = bits_to_double(s->h_st->dtlim_bits);
-    double t = 0.0, dt = 0.0, tot_ent = s->h_st->tot_ent;
-    double tRK[5] = {0, 0, 0, 0, 0};
-    bool flag = true;
-    long long steps = 0, retries = 0, negcells = 0, stages = 0;
+    DriveState &d = s->drive;
+    d = DriveState();
+    d.begun = true;
+    d.use_ldf = use_ldf;
+    if (s->h_st->pp_fail) return PPES_PP_FAIL;
+    if (s->h_st->neg_avg) return PPES_NEG_AVG;  // 'Initial value has negative cell average!' (drive.f90:70)
+    d.dt_lim = bits_to_double(s->h_st->dtlim_bits);
+    d.tot_ent = s->h_st->tot_ent;
+    return PPES_OK;
+}
+
+// Continue the drive loop (drive.f90:83-320) for up to nsteps steps (all
+// steps when nsteps <= 0) or until t_final (> 0); one host synchronisation
+// per RK step. Statistics are cumulative since ppes_solve_begin.
+extern "C" int ppes_advance(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode,
+                            ppes_run_stats *out)
+{
+    if (!s || !(cfl > 0) || oedg_mode < 0 || oedg_mode > 2) return PPES_BAD_ARG;
+    DriveState &d = s->drive;
+    if (!d.begun) return PPES_BAD_ARG;
+    const int S = s->S;
+    const double cfl_rk = (S <= 3) ? 1.0 : 1.508;
+    int rc;
+    long long done_here = 0;
     int status = PPES_OK;
-    while (flag && (t_final > 0.0 || steps < nsteps)) {
-        steps++;
+    while (d.flag && (t_final > 0.0 || done_here < nsteps)) {
+        if (t_final > 0.0 && nsteps > 0 && done_here >= nsteps) break;
+        d.steps++;
+        done_here++;
         int try_num = 0;
         bool try_again = true;
         while (try_again) {
             try_again = false;
             try_num++;
-            if (try_num > 10 || dt_lim < 1e-14) { status = PPES_DT_COLLAPSE; goto done; }
+            if (try_num > 10 || d.dt_lim < 1e-14) { status = PPES_DT_COLLAPSE; goto done; }
             if ((rc = launch_reset(s, 1 | 2 | 4))) return rc;
             if (try_num > 1) {
-                retries++;
+                d.retries++;
                 if ((rc = launch_ghost_copy(s, s->d_u[0], 1))) return rc;
                 if ((rc = launch_residual(s, s->d_u[0], s->d_ut[1]))) return rc;
                 if ((rc = status_pull(s))) return rc;
-                dt_lim = bits_to_double(s->h_st->dtlim_bits);
+                d.dt_lim = bits_to_double(s->h_st->dtlim_bits);
                 if ((rc = launch_reset(s, 1 | 2 | 4))) return rc;
             }
             if (try_num <= 1) {
-                dt = cfl_rk * cfl * dt_lim;
+                d.dt = cfl_rk * cfl * d.dt_lim;
             } else {
                 double base = sqrt(1.25) - 0.5, r = 1.0;
                 for (int e = try_num - 1; e > 0; e >>= 1) {
                     if (e & 1) r *= base;
                     base *= base;
                 }
-                dt = r * dt;
+                d.dt = r * d.dt;
             }
             if (t_final > 0.0) {
-                if (t + dt >= t_final) { dt = t_final - t; flag = false; }
-                else flag = true;
+                if (d.t + d.dt >= t_final) { d.dt = t_final - d.t; d.flag = false; }
+                else d.flag = true;
             }
             for (int st = 1; st <= S; ++st) {
                 double beta;
-                rk_time(S, st, t, tRK, dt, &tRK[st - 1], &beta);
+                rk_time(S, st, d.t, d.tRK, d.dt, &d.tRK[st - 1], &beta);
                 const bool rk_final = (st == S);
-                if ((rc = launch_combine(s, st, dt))) return rc;
+                if ((rc = launch_combine(s, st, d.dt))) return rc;
                 if ((rc = launch_bc(s, s->d_u[st]))) return rc;
-                if (oedg_mode == 2) rc = launch_post_process(s, s->d_u[st], 1, use_ldf, beta * dt, rk_final);
-                else rc = launch_post_process(s, s->d_u[st], oedg_mode == 1 && rk_final, use_ldf, dt, rk_final);
+                if (oedg_mode == 2) rc = launch_post_process(s, s->d_u[st], 1, d.use_ldf, beta * d.dt, rk_final);
+                else rc = launch_post_process(s, s->d_u[st], oedg_mode == 1 && rk_final, d.use_ldf, d.dt, rk_final);
                 if (rc) return rc;
                 if ((rc = launch_bc(s, s->d_u[st]))) return rc;
                 if (rk_final && (rc = launch_reset(s, 1))) return rc;
@@ -729,28 +838,69 @@ extern "C" int ppes_solve(ppes_solver *s, double t_final, long long nsteps, doub
             if ((rc = status_pull(s))) return rc;
             if (s->h_st->neg_avg) {
                 try_again = true;
+                d.stages_retried += S;
                 continue;
             }
             if (s->h_st->pp_fail) { status = PPES_PP_FAIL; goto done; }
-            negcells += s->h_st->num_negcell;
-            stages += S;
-            tot_ent = s->h_st->tot_ent;
-            dt_lim = bits_to_double(s->h_st->dtlim_bits);
+            d.negcells += s->h_st->num_negcell;
+            d.stages += S;
+            d.tot_ent = s->h_st->tot_ent;
+            d.dt_lim = bits_to_double(s->h_st->dtlim_bits);
         }
-        // commit (drive.f90:304-312): swap the final stage into slot 0
+        // commit (drive.f90:304-312): the final stage becomes slot 0
         std::swap(s->d_u[0], s->d_u[S]);
-        t = tRK[S - 1];
+        d.t = d.tRK[S - 1];
     }
 done:
-    out->t = t;
-    out->dt_last = dt;
-    out->dt_lim = dt_lim;
-    out->tot_ent = tot_ent;
-    out->steps = steps;
-    out->retries = retries;
-    out->negcells = negcells;
-    out->stages = stages;
-    out->status = status;
+    if (out) {
+        memset(out, 0, sizeof *out);
+        out->t = d.t;
+        out->dt_last = d.dt;
+        out->dt_lim = d.dt_lim;
+        out->tot_ent = d.tot_ent;
+        out->steps = d.steps;
+        out->retries = d.retries;
+        out->negcells = d.negcells;
+        out->stages = d.stages;
+        out->status = status;
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return status;
+}
+
+extern "C" int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, int use_ldf,
+                          int ent_offset, ppes_run_stats *stats)
+{
+    if (!s || !(cfl > 0) || oedg_mode < 0 || oedg_mode > 2) return PPES_BAD_ARG;
+    if (ent_offset) return PPES_UNSUPPORTED;
+    if (stats) memset(stats, 0, sizeof *stats);
+    int rc = ppes_solve_begin(s, use_ldf);
+    if (rc) {
+        if (stats) stats->status = rc;
+        return rc;
+    }
+    return ppes_advance(s, t_final, nsteps, cfl, oedg_mode, stats);
+}
+
+extern "C" int ppes_set_timing(ppes_solver *s, int on)
+{
+    if (!s) return PPES_BAD_ARG;
+    (void)hipStreamSynchronize(s->stream);
+    s->timing = on != 0;
+    for (int k = 0; k < PPES_NKERNELS; ++k) {
+        harvest_timer(s->timers[k]);
+        s->timers[k].total_ms = 0.0;
+        s->timers[k].count = 0;
+    }
+    return PPES_OK;
+}
+
+extern "C" int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *count)
+{
+    if (!s || kernel < 0 || kernel >= PPES_NKERNELS) return PPES_BAD_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    harvest_timer(s->timers[kernel]);
+    if (total_ms) *total_ms = s->timers[kernel].total_ms;
+    if (count) *count = s->timers[kernel].count;
+    return PPES_OK;
 }

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libppesdg.so")
 
 OK, NEG_AVG, DT_COLLAPSE, PP_FAIL, BAD_ARG, HIP_ERROR, OOM, UNSUPPORTED = range(8)
 BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3}
+KERNELS = {"residual": 0, "avg_ldf": 1, "oedg": 2, "limit": 3, "combine": 4, "bc": 5}
 PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5}
 
 _lib = None
@@ -74,6 +75,11 @@ def lib():
             "ppes_tables": [ctypes.c_int, dp, dp, dp, dp, dp],
             "ppes_ldf_nodal_operator": [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp],
             "ppes_sync": [vp],
+            "ppes_solve_begin": [vp, ctypes.c_int],
+            "ppes_advance": [vp, ctypes.c_double, ctypes.c_longlong, ctypes.c_double, ctypes.c_int,
+                             ctypes.POINTER(RunStats)],
+            "ppes_set_timing": [vp, ctypes.c_int],
+            "ppes_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -204,6 +210,24 @@ class Solver:
                               ctypes.byref(st))
         _check(rc, "ppes_solve", allow=(OK, DT_COLLAPSE, PP_FAIL, NEG_AVG))
         return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
+
+    def solve_begin(self, use_ldf=True):
+        _check(lib().ppes_solve_begin(self.h, int(use_ldf)), "ppes_solve_begin")
+
+    def advance(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1):
+        st = RunStats()
+        rc = lib().ppes_advance(self.h, float(t_final), int(nsteps), float(cfl), int(oedg_mode), ctypes.byref(st))
+        _check(rc, "ppes_advance", allow=(OK, DT_COLLAPSE, PP_FAIL))
+        return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
+
+    def set_timing(self, on):
+        _check(lib().ppes_set_timing(self.h, int(on)), "ppes_set_timing")
+
+    def kernel_time(self, kernel):
+        """(total ms, launches) of one kernel class while timing is on."""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().ppes_kernel_time(self.h, KERNELS[kernel], ctypes.byref(ms), ctypes.byref(n)), "ppes_kernel_time")
+        return ms.value, n.value
 
     def sync(self):
         _check(lib().ppes_sync(self.h), "ppes_sync")

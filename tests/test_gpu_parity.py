@@ -144,7 +144,7 @@ def test_solve_matches_oracle(prob, oedg_mode, cfl, p):
     assert s["negcells"] == s_ref["negcells"]
     assert err <= 1e-10
     assert rel_l1(got[0], ref[0]) <= 1e-10
-    assert abs(s["t"] - s_ref["t"]) <= 1e-14 * s_ref["t"]
+    assert abs(s["t"] - s_ref["t"]) <= 1e-12 * s_ref["t"]  # dt follows dt_lim, which carries the LDF round-off
 
 
 def test_solve_to_final_time_matches_oracle():
@@ -155,7 +155,9 @@ def test_solve_to_final_time_matches_oracle():
     s_ref = o.run(0.05, 0, 0.2, 1, True)
     s = g.solve(0.05, 0, 0.2, 1, True)
     assert s["steps"] == s_ref["steps"]
-    assert s["t"] == s_ref["t"] == 0.05
+    # t advances through the RK pseudo-time (drive.f90:207,312), which can end
+    # an ulp away from t_final, as in the reference
+    assert s["t"] == s_ref["t"] and abs(s["t"] - 0.05) <= 1e-15
     assert rel_l1(interior(g.download_state(0))[0], interior(o.u)[0]) <= 1e-10
 
 
