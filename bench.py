@@ -32,12 +32,14 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic (compulsory) HBM bytes per GLL node for one launch of each
 # kernel class: 8 FP64 variables = 64 B per node per state stream.
 #   residual: read u, write ut (neighbour traces and ghosts not counted)
-#   avg_ldf:  read u, write Bx and By
-#   limit:    read u, write u
-#   combine:  read u^n, ut (+ u^(s-1) in stages 2, 3), write u^(s): mean over RK3
-#   oedg:     read u once
-ALG_BYTES_PER_NODE = {"residual": 128.0, "avg_ldf": 80.0, "limit": 128.0,
-                      "combine": (192.0 + 256.0 + 256.0) / 3.0, "oedg": 64.0, "bc": 0.0}
+#   stage:    read u^n, ut (+ u^(s-1) in stages 2, 3), write u^(s): mean over
+#             the three SSP-RK3 stages (cell averages, LDF, limiter fused)
+#   oedg:     read u once (face indicators)
+#   limit:    read u, write u (OEDG damping + limiter)
+ALG_BYTES_PER_NODE = {"residual": 128.0, "stage": (192.0 + 256.0 + 256.0) / 3.0, "oedg": 64.0,
+                      "limit": 128.0, "reduce": 0.0, "bc": 0.0}
+# whole path, SSP-RK3 with OEDG on the final stage: 640 B per node-step
+STEP_BYTES_PER_NODE_STAGE = 640.0 / 3.0
 
 
 def perturbation_table(seed=0, kmax=4):
@@ -85,7 +87,19 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (reference for HBM byte counters)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        n = 1 << 27  # 2^27 doubles = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(instances shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
@@ -149,6 +163,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": bytes_per_launch},
+        "step_roofline": {"alg_bytes_per_node_stage": STEP_BYTES_PER_NODE_STAGE,
+                          "achieved_GBs": STEP_BYTES_PER_NODE_STAGE * value / 1e9,
+                          "frac_hbm": STEP_BYTES_PER_NODE_STAGE * value / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in kt.items()},
         "checks": checks,
     }

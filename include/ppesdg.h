@@ -124,17 +124,60 @@ int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int
 int ppes_solve_begin(ppes_solver *s, int use_ldf);
 int ppes_advance(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, ppes_run_stats *stats);
 
-/* Per-kernel device time, measured with HIP events on the solver's stream
- * while timing is on (no reference counterpart: measurement support). */
+/* Per-kernel-class device time, measured with HIP events on the solver's
+ * stream while timing is on (no reference counterpart: measurement support).
+ *   RESIDUAL  DG_semidiscretize
+ *   STAGE     RK combine + cell averages + LDF + positivity limiter + entropy
+ *             (post_process without OEDG, fused with the drive loop's combine)
+ *   OEDG      OEDG face jump indicators
+ *   LIMIT     OEDG damping + positivity limiter (stages that apply OEDG)
+ *   REDUCE    global mean / variation reductions of the OEDG normalisation
+ *   BC        ghost strips */
 #define PPES_K_RESIDUAL 0
-#define PPES_K_AVG_LDF 1
+#define PPES_K_STAGE 1
 #define PPES_K_OEDG 2
 #define PPES_K_LIMIT 3
-#define PPES_K_COMBINE 4
+#define PPES_K_REDUCE 4
 #define PPES_K_BC 5
 #define PPES_NKERNELS 6
 int ppes_set_timing(ppes_solver *s, int on);
 int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *count);
+
+/* ---- Batched instances (no reference counterpart: the reference runs one
+ * problem per process, drive.f90:32-343). A batch holds independent
+ * instances of one RK method, each with its own mesh, degree (P1-P4), gamma,
+ * BC, dt and end time. One batch step advances every active instance by one
+ * RK step; kernels of one degree cover all of that degree's instances in one
+ * launch, and the step can be replayed from a captured HIP graph. An instance
+ * run inside a batch gives the same result as the same instance run alone. */
+typedef struct ppes_batch ppes_batch;
+/* instance states */
+#define PPES_INST_IDLE 0
+#define PPES_INST_ACTIVE 1
+#define PPES_INST_DONE 2
+#define PPES_INST_FAILED 3
+int ppes_batch_create(int rk_stages, ppes_batch **out);
+int ppes_batch_destroy(ppes_batch *b);
+/* allocate an instance (cfg->rk_stages must equal the batch's); *id is its handle */
+int ppes_batch_add(ppes_batch *b, const ppes_config *cfg, int *id);
+int ppes_batch_remove(ppes_batch *b, int id);
+int ppes_batch_init_problem(ppes_batch *b, int id, int prob, const double *pert, int nk, double amp);
+int ppes_batch_upload_state(ppes_batch *b, int id, int slot, const double *u);
+int ppes_batch_download_state(ppes_batch *b, int id, int slot, double *u);
+int ppes_batch_download_avg(ppes_batch *b, int id, double *u_avg);
+/* start-up of one instance (drive.f90:66-73) and activation: it runs until
+ * t_final (> 0) or max_steps (> 0) steps */
+int ppes_batch_start(ppes_batch *b, int id, double t_final, long long max_steps, double cfl, int oedg_mode,
+                     int use_ldf);
+/* advance every active instance by up to nsteps batch steps (until none is
+ * active if nsteps <= 0); *n_active = instances still active afterwards */
+int ppes_batch_step(ppes_batch *b, long long nsteps, int use_graph, long long *done_steps, int *n_active);
+int ppes_batch_status(ppes_batch *b, int id, int *state, ppes_run_stats *stats);
+/* device bytes held by an instance / needed by a configuration */
+int ppes_batch_memory(ppes_batch *b, int id, long long *bytes);
+long long ppes_instance_bytes(const ppes_config *cfg);
+int ppes_batch_set_timing(ppes_batch *b, int on);
+int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long long *count);
 
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
  * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major). */

@@ -1,0 +1,583 @@
+// Multi-instance engine (see ppes_engine.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "ppes_engine.hpp"
+#include "ppes_kernels.hpp"
+#include "ppes_tables.hpp"
+
+namespace ppes {
+
+#define ENG_TRY(x)                                                                                          \
+    do {                                                                                                    \
+        hipError_t e_ = (x);                                                                                \
+        if (e_ != hipSuccess) {                                                                             \
+            fprintf(stderr, "ppesdg: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return PPES_HIP_ERROR;                                                                          \
+        }                                                                                                   \
+    } while (0)
+
+#define ENG_DISPATCH(p, CALL)                     \
+    switch (p) {                                  \
+    case 1: { constexpr int P = 1; CALL; } break; \
+    case 2: { constexpr int P = 2; CALL; } break; \
+    case 3: { constexpr int P = 3; CALL; } break; \
+    case 4: { constexpr int P = 4; CALL; } break; \
+    default: return PPES_UNSUPPORTED;             \
+    }
+
+static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+template <int P>
+static long long epb() { return Shape<P>::EPB; }
+
+static long long epb_of(int p)
+{
+    switch (p) {
+    case 1: return epb<1>();
+    case 2: return epb<2>();
+    case 3: return epb<3>();
+    default: return epb<4>();
+    }
+}
+
+// ------------------------------------------------------------------ timers
+
+struct Timed {
+    Engine *e;
+    int id;
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    Timed(Engine *e_, int id_) : e(e_), id(id_)
+    {
+        if (!e->timing) return;
+        KTimer &k = e->timers[id];
+        if (k.pending.size() > 4096) {
+            (void)hipStreamSynchronize(e->stream);
+            e->harvest(id);
+        }
+        if (k.pool.empty()) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            k.pool.push_back({a, b});
+        }
+        ev = k.pool.back();
+        k.pool.pop_back();
+        (void)hipEventRecord(ev.first, e->stream);
+    }
+    ~Timed()
+    {
+        if (!e->timing) return;
+        (void)hipEventRecord(ev.second, e->stream);
+        e->timers[id].pending.push_back(ev);
+    }
+};
+
+void Engine::harvest(int id)
+{
+    KTimer &k = timers[id];
+    for (auto &pr : k.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            k.total_ms += ms;
+            k.count++;
+        }
+        k.pool.push_back(pr);
+    }
+    k.pending.clear();
+}
+
+// ------------------------------------------------------------------ set-up
+
+Engine::Engine(int rk_stages) : S(rk_stages) {}
+
+Engine::~Engine()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &k : timers) {
+        for (auto &pr : k.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+        for (auto &pr : k.pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    }
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &g : groups) free_group(g);
+    if (d_all) (void)hipFree(d_all);
+    for (size_t i = 0; i < hosts.size(); ++i)
+        if (hosts[i]) remove((int)i);
+    if (d_recs) (void)hipFree(d_recs);
+    if (h_recs) (void)hipHostFree(h_recs);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Engine::init()
+{
+    if (S != 1 && S != 2 && S != 3 && S != 5) return PPES_BAD_ARG;
+    ENG_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    return ensure_capacity(4);
+}
+
+int Engine::ensure_capacity(int n)
+{
+    if (n <= cap) return PPES_OK;
+    int nc = std::max(n, 2 * cap);
+    Inst *nh = nullptr, *nd = nullptr;
+    ENG_TRY(hipHostMalloc((void **)&nh, sizeof(Inst) * nc, hipHostMallocDefault));
+    ENG_TRY(hipMalloc((void **)&nd, sizeof(Inst) * nc));
+    memset(nh, 0, sizeof(Inst) * nc);
+    if (cap) {
+        ENG_TRY(hipStreamSynchronize(stream));
+        ENG_TRY(hipMemcpy(nh, d_recs, sizeof(Inst) * cap, hipMemcpyDeviceToHost));
+        ENG_TRY(hipFree(d_recs));
+        ENG_TRY(hipHostFree(h_recs));
+    }
+    ENG_TRY(hipMemcpy(nd, nh, sizeof(Inst) * nc, hipMemcpyHostToDevice));
+    h_recs = nh;
+    d_recs = nd;
+    cap = nc;
+    invalidate_plan();
+    return PPES_OK;
+}
+
+int Engine::push()
+{
+    ENG_TRY(hipMemcpyAsync(d_recs, h_recs, sizeof(Inst) * cap, hipMemcpyHostToDevice, stream));
+    ENG_TRY(hipStreamSynchronize(stream));
+    return PPES_OK;
+}
+
+int Engine::pull()
+{
+    ENG_TRY(hipMemcpyAsync(h_recs, d_recs, sizeof(Inst) * cap, hipMemcpyDeviceToHost, stream));
+    ENG_TRY(hipStreamSynchronize(stream));
+    return PPES_OK;
+}
+
+int Engine::sync()
+{
+    ENG_TRY(hipStreamSynchronize(stream));
+    return PPES_OK;
+}
+
+int Engine::add(const ppes_config &cfg, int *id)
+{
+    if (cfg.nx < 1 || cfg.ny < 1 || cfg.px != cfg.py || cfg.px < 1 || cfg.px > 4) return PPES_BAD_ARG;
+    if (cfg.rk_stages != S) return PPES_BAD_ARG;
+    if (!(cfg.right > cfg.left) || !(cfg.top > cfg.bottom) || !(cfg.gamma > 1.0)) return PPES_BAD_ARG;
+    if (cfg.bc_kind < PPES_BC_PERIODIC || cfg.bc_kind > PPES_BC_HOST) return PPES_BAD_ARG;
+    int slot = -1;
+    for (size_t i = 0; i < hosts.size(); ++i)
+        if (!hosts[i]) { slot = (int)i; break; }
+    if (slot < 0) {
+        slot = (int)hosts.size();
+        hosts.push_back(nullptr);
+    }
+    int rc = ensure_capacity(slot + 1);
+    if (rc) return rc;
+    rc = pull();  // keep device-side pointer swaps of other instances
+    if (rc) return rc;
+    HostInst *h = new HostInst();
+    h->cfg = cfg;
+    h->p = cfg.px;
+    h->N = h->p + 1;
+    h->NN = h->N * h->N;
+    h->S = S;
+    const int N = h->N, NN = h->NN, nx = cfg.nx, ny = cfg.ny;
+    h->qx.resize(N);
+    h->qw.resize(N);
+    h->D.resize(NN);
+    host_tables(h->p, h->qx.data(), h->qw.data(), h->D.data(), nullptr, nullptr);
+    // geometry (DG_solver.f90:93-106)
+    h->hx = (cfg.right - cfg.left) / nx;
+    h->hy = (cfg.top - cfg.bottom) / ny;
+    h->xx.resize((size_t)N * nx);
+    h->yy.resize((size_t)N * ny);
+    for (int i = 1; i <= nx; ++i)
+        for (int k = 0; k < N; ++k) h->xx[k + (size_t)N * (i - 1)] = (0.5 * h->hx) * h->qx[k] + (cfg.left + (i - 0.5) * h->hx);
+    for (int j = 1; j <= ny; ++j)
+        for (int k = 0; k < N; ++k) h->yy[k + (size_t)N * (j - 1)] = (0.5 * h->hy) * h->qx[k] + (cfg.bottom + (j - 0.5) * h->hy);
+
+    Inst &r = h_recs[slot];
+    memset(&r, 0, sizeof r);
+    r.nx = nx;
+    r.ny = ny;
+    r.p = h->p;
+    r.S = S;
+    r.vs = (long long)(nx + 2) * (ny + 2) * NN;
+    r.vsi = (long long)nx * ny * NN;
+    r.hx = h->hx;
+    r.hy = h->hy;
+    r.g = cfg.gamma;
+    for (int i = 0; i < N; ++i) r.qw[i] = h->qw[i];
+    for (int i = 0; i < NN; ++i) r.D[i] = h->D[i];
+    for (int l = 0; l < N; ++l)
+        for (int k = 0; k < N; ++k) {
+            r.wavg[k + N * l] = (0.25 * h->qw[k]) * h->qw[l];
+            r.went[k + N * l] = (((0.25 * h->hx) * h->hy) * h->qw[k]) * h->qw[l];
+        }
+    r.bc_kind = cfg.bc_kind;
+    for (int v = 0; v < 8; ++v) r.bc_state[v] = cfg.bc_state[v];
+    const long long E = epb_of(h->p);
+    r.nblk_el = cdiv((long long)nx * ny, E);
+    r.nblk_slot = cdiv((long long)(nx + 2) * (ny + 2), E);
+    r.ctl.state = ST_IDLE;
+    r.ctl.dtlim_bits = 0x7FF0000000000000ULL;
+
+    auto alloc = [&](double **p, size_t n) -> bool {
+        if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) { *p = nullptr; return false; }
+        h->allocs.push_back(*p);
+        h->bytes += std::max<size_t>(n, 1) * sizeof(double);
+        return true;
+    };
+    bool ok = true;
+    const size_t nstate = 8 * (size_t)r.vs, nres = 8 * (size_t)r.vsi;
+    for (int k = 0; k <= S && ok; ++k) ok = alloc(&r.u[k], nstate);
+    for (int k = 1; k <= S && ok; ++k) ok = alloc(&r.ut[k], nres);
+    ok = ok && alloc(&r.uavg, 8 * (size_t)(nx + 2) * (ny + 2));
+    ok = ok && alloc(&r.sigx, 8 * (size_t)(nx + 1) * ny);
+    ok = ok && alloc(&r.sigy, 8 * (size_t)nx * (ny + 1));
+    ok = ok && alloc(&r.ldf, (size_t)4 * NN * NN);
+    ok = ok && alloc(&r.part, 24 * (size_t)r.nblk_slot);
+    ok = ok && alloc(&r.part2, 24 * (size_t)cdiv(r.nblk_slot, BLOCK));
+    ok = ok && alloc(&r.entpart, (size_t)std::max(r.nblk_el, r.nblk_slot));
+    ok = ok && alloc(&r.ghost0, 8 * (size_t)(2 * nx + 2 * ny) * NN);
+    hosts[slot] = h;
+    if (!ok) {
+        remove(slot);
+        return PPES_OOM;
+    }
+    std::vector<double> op((size_t)4 * NN * NN);
+    host_ldf_nodal(h->p, h->hx, h->hy, op.data());
+    ENG_TRY(hipMemcpyAsync(r.ldf, op.data(), op.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    for (int k = 0; k <= S; ++k) ENG_TRY(hipMemsetAsync(r.u[k], 0, nstate * sizeof(double), stream));
+    ENG_TRY(hipMemsetAsync(r.uavg, 0, 8 * (size_t)(nx + 2) * (ny + 2) * sizeof(double), stream));
+    rc = push();
+    if (rc) return rc;
+    invalidate_plan();
+    *id = slot;
+    return PPES_OK;
+}
+
+int Engine::remove(int id)
+{
+    if (!valid(id)) return PPES_BAD_ARG;
+    if (stream) (void)hipStreamSynchronize(stream);
+    HostInst *h = hosts[id];
+    for (double *p : h->allocs) (void)hipFree(p);
+    delete h;
+    hosts[id] = nullptr;
+    if (h_recs && id < cap) {
+        memset(&h_recs[id], 0, sizeof(Inst));
+        if (d_recs) (void)hipMemcpy(&d_recs[id], &h_recs[id], sizeof(Inst), hipMemcpyHostToDevice);
+    }
+    invalidate_plan();
+    return PPES_OK;
+}
+
+int Engine::active_count() const
+{
+    int n = 0;
+    for (size_t i = 0; i < hosts.size(); ++i)
+        if (hosts[i] && h_recs[i].ctl.state == ST_ACTIVE) ++n;
+    return n;
+}
+
+// ------------------------------------------------------------------ plans
+
+void Engine::free_group(Group &g)
+{
+    Blk *m[] = {g.d_el, g.d_slot, g.d_face, g.d_bc, g.d_red1, g.d_red2};
+    for (Blk *p : m) if (p) (void)hipFree(p);
+    if (g.d_list) (void)hipFree(g.d_list);
+    g = Group();
+}
+
+int Engine::build_group(Group &g, const std::vector<int> &members, int p)
+{
+    g.p = p;
+    g.members = members;
+    std::vector<Blk> el, sl, fc, bc, r1, r2;
+    for (int id : members) {
+        const Inst &r = h_recs[id];
+        const int NN = (p + 1) * (p + 1);
+        for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
+        for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
+        const long long nf = ((long long)(r.nx + 1) * r.ny + (long long)r.nx * (r.ny + 1)) * 8;
+        for (long long b = 0; b < cdiv(nf, BLOCK); ++b) fc.push_back({id, (int)b});
+        const long long nb = (2LL * r.nx + 2LL * r.ny) * NN;
+        for (long long b = 0; b < cdiv(nb, BLOCK); ++b) bc.push_back({id, (int)b});
+        for (long long b = 0; b < cdiv(r.nblk_slot, BLOCK); ++b) r1.push_back({id, (int)b});
+        r2.push_back({id, 0});
+    }
+    auto up = [&](Blk **d, const std::vector<Blk> &h) -> int {
+        ENG_TRY(hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(Blk)));
+        if (!h.empty()) ENG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(Blk), hipMemcpyHostToDevice));
+        return PPES_OK;
+    };
+    int rc;
+    if ((rc = up(&g.d_el, el)) || (rc = up(&g.d_slot, sl)) || (rc = up(&g.d_face, fc)) || (rc = up(&g.d_bc, bc)) ||
+        (rc = up(&g.d_red1, r1)) || (rc = up(&g.d_red2, r2)))
+        return rc;
+    ENG_TRY(hipMalloc((void **)&g.d_list, std::max<size_t>(members.size(), 1) * sizeof(int)));
+    ENG_TRY(hipMemcpy(g.d_list, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice));
+    g.n_el = (long long)el.size();
+    g.n_slot = (long long)sl.size();
+    g.n_face = (long long)fc.size();
+    g.n_bc = (long long)bc.size();
+    g.n_red1 = (long long)r1.size();
+    return PPES_OK;
+}
+
+int Engine::build_plan()
+{
+    if (plan_ok) return PPES_OK;
+    for (auto &g : groups) free_group(g);
+    groups.clear();
+    if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
+    std::vector<int> all;
+    for (int s = 0; s <= MAXS; ++s) any_oedg_stage[s] = false;
+    for (int p = 1; p <= 4; ++p) {
+        std::vector<int> mem;
+        for (size_t i = 0; i < hosts.size(); ++i)
+            if (hosts[i] && hosts[i]->p == p && h_recs[i].ctl.state == ST_ACTIVE) mem.push_back((int)i);
+        if (mem.empty()) continue;
+        Group g;
+        int rc = build_group(g, mem, p);
+        if (rc) return rc;
+        groups.push_back(g);
+        for (int id : mem) {
+            all.push_back(id);
+            const int mode = h_recs[id].ctl.oedg_mode;
+            for (int st = 1; st <= S; ++st)
+                if (mode == 2 || (mode == 1 && st == S)) any_oedg_stage[st] = true;
+        }
+    }
+    n_all = (int)all.size();
+    ENG_TRY(hipMalloc((void **)&d_all, std::max<size_t>(all.size(), 1) * sizeof(int)));
+    if (!all.empty()) ENG_TRY(hipMemcpy(d_all, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+    plan_ok = true;
+    graph_ok = false;
+    return PPES_OK;
+}
+
+// ------------------------------------------------------------------ launches
+
+#define LAUNCH(kernel, grid, ...)                                                                  \
+    do {                                                                                           \
+        if ((grid) > 0) {                                                                          \
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(grid)), dim3(BLOCK), 0, stream, __VA_ARGS__); \
+            ENG_TRY(hipGetLastError());                                                            \
+        }                                                                                          \
+    } while (0)
+
+#define LAUNCH1(kernel, n, ...)                                                                            \
+    do {                                                                                                   \
+        if ((n) > 0) {                                                                                     \
+            hipLaunchKernelGGL(kernel, dim3((unsigned)cdiv((n), 64)), dim3(64), 0, stream, __VA_ARGS__);   \
+            ENG_TRY(hipGetLastError());                                                                    \
+        }                                                                                                  \
+    } while (0)
+
+// One RK step of every instance of the plan (drive.f90:83-320). With
+// all_kernels the retry kernels are queued unconditionally (they exit at once
+// for instances that are not retrying), as a captured graph or a group of
+// steps queued without host synchronisation needs; otherwise the host mirror
+// decides.
+int Engine::launch_step(bool all_kernels)
+{
+    int rc = build_plan();
+    if (rc) return rc;
+    if (n_all == 0) return PPES_OK;
+    LAUNCH1(k_step_begin, n_all, d_recs, (const int *)d_all, n_all);
+    // a retried step recomputes L(u^n) from the saved ghost layer (drive.f90:104)
+    bool any_retry = all_kernels;
+    if (!all_kernels)
+        for (const auto &g : groups)
+            for (int id : g.members) any_retry = any_retry || h_recs[id].ctl.retry_residual;
+    if (any_retry) {
+        for (auto &g : groups) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 1));
+        LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 1);
+        for (auto &g : groups) {
+            Timed tm(this, KT_RESIDUAL);
+            ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 1));
+        }
+    }
+    for (int st = 1; st <= S; ++st) {
+        for (auto &g : groups) {
+            Timed tm(this, KT_STAGE);
+            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, st, (int)F_COMBINE));
+        }
+        if (any_oedg_stage[st]) {
+            for (auto &g : groups) {
+                {
+                    Timed tm(this, KT_REDUCE);
+                    LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, st, 0, 0);
+                    LAUNCH(k_reduce_global, (long long)g.members.size(), d_recs, (const Blk *)g.d_red2, st, 0, 1);
+                }
+                {
+                    Timed tm(this, KT_OEDG);
+                    ENG_DISPATCH(g.p, LAUNCH(k_oedg_faces<P>, g.n_face, d_recs, (const Blk *)g.d_face, st, 0));
+                }
+                {
+                    Timed tm(this, KT_LIMIT);
+                    ENG_DISPATCH(g.p, LAUNCH(k_oedg_limit<P>, g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                }
+            }
+        }
+        if (st == S)
+            for (auto &g : groups) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, st, 0);
+        for (auto &g : groups) {
+            Timed tm(this, KT_BC);
+            ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, st));
+        }
+        LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 0);
+        for (auto &g : groups) {
+            Timed tm(this, KT_RESIDUAL);
+            ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, st, (st % S) + 1, 0));
+        }
+    }
+    LAUNCH1(k_step_end, n_all, d_recs, (const int *)d_all, n_all);
+    return PPES_OK;
+}
+
+// capture one step of the current plan as a hipGraph
+int Engine::capture()
+{
+    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    const bool was_timing = timing;
+    timing = false;  // no events inside a captured graph
+    ENG_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(true);
+    hipError_t ec = hipStreamEndCapture(stream, &graph);
+    timing = was_timing;
+    if (rc) return rc;
+    ENG_TRY(ec);
+    ENG_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    graph_ok = true;
+    return PPES_OK;
+}
+
+// Advance every active instance by up to nsteps steps (until none is active
+// when nsteps <= 0). Steps are queued `sync_every` at a time between host
+// synchronisations: an instance that finishes inside a group makes the
+// remaining steps of the group no-ops for it (its kernels exit at once),
+// after which the plan is rebuilt without it.
+int Engine::run_steps(long long nsteps, bool use_graph, long long *done_steps)
+{
+    int rc = build_plan();
+    if (rc) return rc;
+    long long k = 0;
+    while (nsteps <= 0 || k < nsteps) {
+        if (active_count() == 0) break;
+        if (use_graph && !graph_ok && (rc = capture())) return rc;
+        long long chunk = std::max(1, sync_every);
+        if (nsteps > 0) chunk = std::min(chunk, nsteps - k);
+        for (long long c = 0; c < chunk; ++c) {
+            if (use_graph) {
+                ENG_TRY(hipGraphLaunch(graph_exec, stream));
+            } else if ((rc = launch_step(chunk > 1))) {
+                return rc;
+            }
+        }
+        if ((rc = pull())) return rc;
+        k += chunk;
+        // membership changed (an instance finished or failed): rebuild the
+        // plan so that finished instances no longer occupy workgroups
+        bool changed = false;
+        for (const auto &g : groups)
+            for (int id : g.members)
+                if (h_recs[id].ctl.state != ST_ACTIVE) changed = true;
+        if (changed) {
+            invalidate_plan();
+            if ((rc = build_plan())) return rc;
+        }
+    }
+    if (done_steps) *done_steps = k;
+    return PPES_OK;
+}
+
+// start-up of the given (active) instances: BC, post-process without OEDG,
+// BC, saved ghost layer, residual of u^n (drive.f90:66-73)
+int Engine::launch_startup(const std::vector<int> &ids)
+{
+    std::vector<int> by_p[5];
+    for (int id : ids) by_p[hosts[id]->p].push_back(id);
+    int *d_ids = nullptr;
+    ENG_TRY(hipMalloc((void **)&d_ids, std::max<size_t>(ids.size(), 1) * sizeof(int)));
+    ENG_TRY(hipMemcpy(d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int n = (int)ids.size();
+    int rc = PPES_OK;
+    std::vector<Group> tmp;
+    for (int p = 1; p <= 4 && rc == PPES_OK; ++p) {
+        if (by_p[p].empty()) continue;
+        Group g;
+        rc = build_group(g, by_p[p], p);
+        tmp.push_back(g);
+    }
+    if (rc == PPES_OK) {
+        LAUNCH1(k_reset_flags, n, d_recs, (const int *)d_ids, n, 1 | 2 | 4);
+        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        for (auto &g : tmp)
+            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, 0,
+                                     (int)(F_GHOST_MEM | F_FORCE_NO_OEDG | F_ENT)));
+        for (auto &g : tmp) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, 0, (int)(F_FORCE_NO_OEDG | F_ENT));
+        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
+        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 0));
+    }
+    (void)hipStreamSynchronize(stream);
+    for (auto &g : tmp) free_group(g);
+    (void)hipFree(d_ids);
+    return rc;
+}
+
+// single reference-API calls on one instance
+int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
+{
+    std::vector<int> ids{id};
+    Group g;
+    int rc = build_group(g, ids, hosts[id]->p);
+    if (rc) { free_group(g); return rc; }
+    int *d_ids = g.d_list;
+    switch (what) {
+    case API_RESET:
+        LAUNCH1(k_reset_flags, 1, d_recs, (const int *)d_ids, 1, flags);
+        break;
+    case API_BC:
+        ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, slot));
+        break;
+    case API_GHOST_SAVE:
+        ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        break;
+    case API_COMBINE:
+        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
+        break;
+    case API_RESID:
+        LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
+        ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, slot, ut_slot, 0));
+        break;
+    case API_POST: {
+        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, slot, flags));
+        if (flags & F_FORCE_OEDG) {
+            LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, slot, flags, 0);
+            LAUNCH(k_reduce_global, 1, d_recs, (const Blk *)g.d_red2, slot, flags, 1);
+            ENG_DISPATCH(g.p, LAUNCH(k_oedg_faces<P>, g.n_face, d_recs, (const Blk *)g.d_face, slot, flags));
+            ENG_DISPATCH(g.p, LAUNCH(k_oedg_limit<P>, g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
+        }
+        if (flags & F_ENT) LAUNCH(k_reduce_ent, 1, d_recs, (const int *)d_ids, slot, flags);
+        break;
+    }
+    default:
+        rc = PPES_BAD_ARG;
+    }
+    (void)hipStreamSynchronize(stream);
+    free_group(g);
+    return rc;
+}
+
+}  // namespace ppes

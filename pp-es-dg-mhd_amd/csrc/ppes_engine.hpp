@@ -1,0 +1,99 @@
+// Multi-instance engine: device records, launch plans and the step sequence.
+//
+// An Engine owns a set of instances (independent simulations, each with its
+// own size, degree, gamma, BC, time step and end time) on one HIP stream.
+// Kernels of one polynomial degree run over all of that degree's instances in
+// one launch, through block maps built here. A step of every active instance
+// is one fixed launch sequence whose data dependence goes only through the
+// device records, so it can be captured once as a hipGraph and replayed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/ppesdg.h"
+#include "ppes_types.hpp"
+
+namespace ppes {
+
+struct HostInst {
+    ppes_config cfg;
+    int p = 0, N = 0, NN = 0, S = 0;
+    double hx = 0, hy = 0;
+    std::vector<double> qx, qw, D, xx, yy;
+    std::vector<double *> allocs;
+    size_t bytes = 0;
+};
+
+struct Group {
+    int p = 0;
+    std::vector<int> members;
+    Blk *d_el = nullptr, *d_slot = nullptr, *d_face = nullptr, *d_bc = nullptr, *d_red1 = nullptr, *d_red2 = nullptr;
+    int *d_list = nullptr;
+    long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0;
+};
+
+enum KTimerId { KT_RESIDUAL = 0, KT_STAGE = 1, KT_OEDG = 2, KT_LIMIT = 3, KT_REDUCE = 4, KT_BC = 5, KT_COUNT = 6 };
+
+struct KTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+    double total_ms = 0.0;
+    long long count = 0;
+};
+
+class Engine {
+public:
+    explicit Engine(int rk_stages);
+    ~Engine();
+    int init();                                   // stream and record storage
+    int add(const ppes_config &cfg, int *id);     // allocate an instance (inactive)
+    int remove(int id);
+    bool valid(int id) const { return id >= 0 && id < (int)hosts.size() && hosts[id] != nullptr; }
+    HostInst &host(int id) { return *hosts[id]; }
+    Inst &rec(int id) { return h_recs[id]; }      // host mirror of the record
+
+    int push();                                   // host mirror -> device
+    int pull();                                   // device -> host mirror (synchronises)
+    int sync();
+
+    // launch helpers over an explicit instance list (plans are built on demand)
+    int launch_startup(const std::vector<int> &ids);             // drive.f90:66-73
+    int launch_step(bool all_kernels);                           // one RK step of every active instance
+    int run_steps(long long nsteps, bool use_graph, long long *done_steps);
+    int sync_every = 1;                                          // steps queued per host synchronisation
+    int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
+
+    int active_count() const;
+    void invalidate_plan() { plan_ok = false; graph_ok = false; }
+
+    // per-kernel-class timers (HIP events on the engine's stream)
+    bool timing = false;
+    KTimer timers[KT_COUNT];
+    void harvest(int k);
+
+    hipStream_t stream = nullptr;
+    int S;
+    std::vector<HostInst *> hosts;
+    Inst *h_recs = nullptr;  // pinned, capacity entries
+    Inst *d_recs = nullptr;
+    int cap = 0;
+
+private:
+    int ensure_capacity(int n);
+    int build_plan();
+    int build_group(Group &g, const std::vector<int> &members, int p);
+    int capture();
+    void free_group(Group &g);
+    int *d_all = nullptr;
+    int n_all = 0;
+    std::vector<Group> groups;
+    bool plan_ok = false;
+    bool any_oedg_stage[MAXS + 1] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_ok = false;
+};
+
+enum ApiCall { API_BC = 0, API_POST = 1, API_RESID = 2, API_COMBINE = 3, API_GHOST_SAVE = 4, API_RESET = 5 };
+
+}  // namespace ppes

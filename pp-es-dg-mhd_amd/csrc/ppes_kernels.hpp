@@ -1,155 +1,209 @@
-// Device kernels of the PP-ES nodal DG step (gfx950, FP64).
-//
-// Data layout in HBM (structure of arrays):
-//   state  u[v][ey][ex][n]     v = 0..7, ex = 0..nx+1, ey = 0..ny+1 (one
-//                               ghost layer), n = k + N*l the GLL node
-//   resid  ut[v][ey-1][ex-1][n] interior elements only
-//   avg    uavg[ey][ex][v]      cell averages (one 64-byte record per element)
-// A 64-lane wavefront reading variable v of consecutive elements touches one
-// contiguous run of doubles.
-//
-// Element-group mapping: a 256-thread workgroup holds EPB = 256 / N^2
-// elements, each owned by N^2 consecutive lanes (lane = GLL node).
+// Device kernels of the PP-ES nodal DG step (gfx950, FP64) over the
+// per-instance records of ppes_types.hpp. Included by one translation unit
+// (the engine), which launches every kernel here.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "ppes_physics.hpp"
+#include "ppes_types.hpp"
 
 namespace ppes {
 
-constexpr int BLOCK = 256;
-constexpr int MAXN = 5;  // p <= 4 on the device
-
-struct KParams {
-    int nx, ny;            // interior elements
-    long long vs;          // variable stride of a state array: (nx+2)(ny+2)N^2
-    long long vsi;         // variable stride of a residual array: nx ny N^2
-    double hx, hy, g;      // element sizes and gamma
-    double qw[MAXN];       // GLL weights
-    double D[MAXN * MAXN]; // nodal differentiation matrix, D[k + N r] = l_r'(x_k)
-    double wavg[MAXN * MAXN];  // (0.25 w_k) w_l, cell-average weights
-    double went[MAXN * MAXN];  // (((0.25 hx) hy) w_k) w_l, entropy weights
-    int bc_kind;
-    double bc_state[8];
-};
-
-// device-side status block of one solver
-struct DevStatus {
-    unsigned long long dtlim_bits;  // min over elements, bits of a positive double
-    int neg_avg;                    // a cell average became inadmissible
-    int pp_fail;                    // PP fallback exhausted (DG_solver.f90:582-586)
-    int num_negcell;                // PP-limited cells in this post-process
-    int pad;
-    double glob[16];                // u_avg_global[8], u_vari_global[8]
-    double tot_ent;
-};
-
-enum BcKind { BC_PERIODIC = 0, BC_EXTRAP = 1, BC_FIXED = 2 };
-
-template <int P>
-struct Shape {
-    static constexpr int N = P + 1;
-    static constexpr int NN = N * N;
-    static constexpr int EPB = BLOCK / NN;                 // elements per workgroup
-    static constexpr int H = N * (N - 1) / 2;              // pairs per line
-    static constexpr int PAIRS = NN * P;                   // x- plus y-pairs per element
-    static constexpr int FACE = 4 * N;                     // face nodes per element
-    static constexpr int BUF = (PAIRS * 8 > FACE * 12) ? PAIRS * 8 : FACE * 12;  // doubles per element
-};
-
-__device__ __forceinline__ long long gidx(const KParams &kp, int ex, int ey) { return (long long)ey * (kp.nx + 2) + ex; }
+// RK stage combine of one value (drive.f90:130-272), in the reference's
+// written form for each method and stage. gs/gt index the state / residual.
+__device__ __forceinline__ double rk_combine_value(const Inst &I, int st, int v, long long gs, long long gt)
+{
+    const double dt = I.ctl.dt;
+    const long long os = v * I.vs + gs, ot = v * I.vsi + gt;
+    const double un = I.u[0][os], t = I.ut[st][ot];
+    switch (I.S) {
+    case 1:
+        return un + dt * t;
+    case 2:
+        if (st == 1) return un + dt * t;
+        return 0.5 * un + 0.5 * (I.u[1][os] + dt * t);
+    case 3:
+        if (st == 1) return un + dt * t;
+        if (st == 2) return 0.75 * un + 0.25 * (I.u[1][os] + dt * t);
+        return (1.0 / 3.0) * un + (2.0 / 3.0) * (I.u[2][os] + dt * t);
+    default:
+        switch (st) {
+        case 1: return un + (0.391752226571890 * dt) * t;
+        case 2: return 0.444370493651235 * un + 0.555629506348765 * I.u[1][os] + (0.368410593050371 * dt) * t;
+        case 3: return 0.620101851488403 * un + 0.379898148511597 * I.u[2][os] + (0.251891774271694 * dt) * t;
+        case 4: return 0.178079954393132 * un + 0.821920045606868 * I.u[3][os] + (0.544974750228521 * dt) * t;
+        default:
+            return 0.517231671970585 * I.u[2][os] + 0.096059710526147 * I.u[3][os] + (0.063692468666290 * dt) * I.ut[4][ot] +
+                   0.386708617503269 * I.u[4][os] + (0.226007483236906 * dt) * t;
+        }
+    }
+}
 
 // --------------------------------------------------------------------------
-// DG_semidiscretize (DG_solver.f90:663-892): face ES-HLL fluxes, EC flux
-// differencing with the Godunov-Powell source, surface terms and dt_lim.
+// DG_semidiscretize (DG_solver.f90:663-892): EC flux differencing with the
+// Godunov-Powell source, ES-HLL face fluxes, surface terms and dt_lim.
+// mode 0: every live instance, ut[ut_slot] = L(u[u_slot]);
+// mode 1: only instances retrying a step (ut[1] = L(u^n), drive.f90:104).
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_residual(KParams kp, const double *__restrict__ u,
-                                                    double *__restrict__ ut, DevStatus *st)
+__global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
     using S = Shape<P>;
-    constexpr int N = S::N, NN = S::NN, EPB = S::EPB;
+    constexpr int N = S::N, NN = S::NN, EPB = S::EPB, REC = S::REC;
     __shared__ Prim sprim[EPB][NN];
     __shared__ double sbuf[EPB][S::BUF];
     __shared__ double sred[BLOCK][2];
 
-    if (st->neg_avg) return;
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I)) return;
+    if (mode == 1 && !I.ctl.retry_residual) return;
+    const double *__restrict__ u = I.u[u_slot];
+    double *__restrict__ ut = I.ut[ut_slot];
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
     const int k = n % N, l = n / N;
-    const long long eg = (long long)blockIdx.x * EPB + el;
-    const bool valid = (el < EPB) && (eg < (long long)kp.nx * kp.ny);
-    const int i = valid ? (int)(eg % kp.nx) + 1 : 1;
-    const int j = valid ? (int)(eg / kp.nx) + 1 : 1;
-    const double g = kp.g, hx = kp.hx, hy = kp.hy;
-    const long long vs = kp.vs;
+    const long long eg = (long long)bm.local * EPB + el;
+    const bool valid = (el < EPB) && (eg < (long long)I.nx * I.ny);
+    const int i = valid ? (int)(eg % I.nx) + 1 : 1;
+    const int j = valid ? (int)(eg / I.nx) + 1 : 1;
+    const double g = I.g, hx = I.hx, hy = I.hy;
+    const long long vs = I.vs;
+    const long long base = gidx(I, i, j) * NN + n;
 
-    double F[8], G[8], GP[8];
+    // ---- node primitives (DG_solver.f90:734-801)
+    double q[8];
     double msx = 0.0, msy = 0.0;
-    double bxn = 0.0, byn = 0.0;
     if (valid) {
-        const long long base = gidx(kp, i, j) * NN + n;
-        double q[8];
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = u[v * vs + base];
-        double rho = q[0], mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
-        double inv_rho = 1.0 / rho;
-        double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
-        double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
-        double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
-        double b_sq = bx * bx + by * by + bz * bz;
-        double p_mag = 0.5 * b_sq;
-        double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
-        double beta = 0.5 * rho / p_fld;
-        double p_tot = p_fld + p_mag;
-        double b_dot_u = bx * vx + by * vy + bz * vz;
-        double snc_sq = g * p_fld * inv_rho;
-        double alfven_sq = b_sq * inv_rho;
-        double sum_sq = snc_sq + alfven_sq;
-        double coef = 4.0 * snc_sq * inv_rho;
+        const double rho = q[0], mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
+        const double inv_rho = 1.0 / rho;
+        const double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
+        const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
+        const double b_sq = bx * bx + by * by + bz * bz;
+        const double p_mag = 0.5 * b_sq;
+        const double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
+        const double snc_sq = g * p_fld * inv_rho;
+        const double alfven_sq = b_sq * inv_rho;
+        const double sum_sq = snc_sq + alfven_sq;
+        const double coef = 4.0 * snc_sq * inv_rho;
         msx = fabs(vx) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (bx * bx), 0.0)), 0.0));
         msy = fabs(vy) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (by * by), 0.0)), 0.0));
         Prim pr;
-        pr.rho = rho; pr.vx = vx; pr.vy = vy; pr.vz = vz; pr.bx = bx; pr.by = by; pr.bz = bz; pr.beta = beta;
+        pr.rho = rho; pr.vx = vx; pr.vy = vy; pr.vz = vz; pr.bx = bx; pr.by = by; pr.bz = bz;
+        pr.beta = 0.5 * rho / p_fld;
         sprim[el][n] = pr;
-        bxn = bx; byn = by;
-        F[0] = mx;
-        F[1] = vx * mx - bx * bx + p_tot;
-        F[2] = vx * my - by * bx + 0.0;
-        F[3] = vx * mz - bz * bx + 0.0;
-        F[4] = 0.0;
-        F[5] = -ez;
-        F[6] = ey;
-        F[7] = vx * (e_tot + p_tot) - bx * b_dot_u;
-        G[0] = my;
-        G[1] = vy * mx - bx * by + 0.0;
-        G[2] = vy * my - by * by + p_tot;
-        G[3] = vy * mz - bz * by + 0.0;
-        G[4] = ez;
-        G[5] = 0.0;
-        G[6] = -ex;
-        G[7] = vy * (e_tot + p_tot) - by * b_dot_u;
-        GP[0] = 0.0; GP[1] = bx; GP[2] = by; GP[3] = bz;
-        GP[4] = vx; GP[5] = vy; GP[6] = vz; GP[7] = b_dot_u;
     }
+    __syncthreads();
 
-    // ---- face fluxes of the element's four faces (both neighbours compute a
-    // shared face from identical inputs, so they agree bit for bit)
+    // ---- EC pair fluxes, each pair evaluated once (P rounds, all lanes busy);
+    // the component that is identically zero (4 in x, 5 in y) is not stored
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+            const int qp = n + c * NN;
+            double ec[8];
+            double *o = &sbuf[el][qp * REC];
+            if (qp < NN * P / 2) {
+                const int lr = qp / S::H, a = qp - lr * S::H;
+                int kk = 1;
+                while ((kk + 1) * kk / 2 <= a) ++kk;
+                const int rr = a - kk * (kk - 1) / 2;
+                ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
+                o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[5]; o[5] = ec[6]; o[6] = ec[7];
+            } else {
+                const int qq = qp - NN * P / 2;
+                const int kc = qq / S::H, a = qq - kc * S::H;
+                int ll = 1;
+                while ((ll + 1) * ll / 2 <= a) ++ll;
+                const int ss = a - ll * (ll - 1) / 2;
+                ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
+                o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[4]; o[5] = ec[6]; o[6] = ec[7];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- volume terms in the reference's accumulation order (DG_solver.f90:828-851)
+    double acc[8];
+    if (valid) {
+        const double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
+        double divB;
+        {
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int r = 0; r < N; ++r) a += I.D[k + N * r] * sprim[el][r + N * l].bx;
+#pragma unroll
+            for (int s = 0; s < N; ++s) b += sprim[el][k + N * s].by * I.D[l + N * s];
+            divB = (2.0 / hx) * a + (2.0 / hy) * b;
+        }
+        {
+            const Prim &pr = sprim[el][n];
+            const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
+            const double vx = pr.vx, vy = pr.vy, vz = pr.vz;
+            const double inv_rho = 1.0 / q[0];
+            const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
+            const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
+            const double b_sq = bx * bx + by * by + bz * bz;
+            const double p_mag = 0.5 * b_sq;
+            const double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
+            const double p_tot = p_fld + p_mag;
+            const double b_dot_u = bx * vx + by * vy + bz * vz;
+            const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
+                                 0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
+            const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
+                                 ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
+            const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
+            const double dxx = cx * I.D[k + N * k], dyy = cy * I.D[l + N * l];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) acc[v] = 0.0 - dxx * F[v] - dyy * G[v] - divB * GP[v];
+        }
+        // x-pairs (k, r), r < k
+        for (int r = 0; r < k; ++r) {
+            const double *o = &sbuf[el][(l * S::H + k * (k - 1) / 2 + r) * REC];
+            const double a = cx * I.D[k + N * r];
+            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
+            acc[5] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+        }
+        // y-pairs ((k,l), (k,s)), s < l
+        for (int s = 0; s < l; ++s) {
+            const double *o = &sbuf[el][(NN * P / 2 + k * S::H + l * (l - 1) / 2 + s) * REC];
+            const double a = cy * I.D[l + N * s];
+            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
+            acc[4] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+        }
+        // x-pairs (k', k), k' > k
+        for (int kk = k + 1; kk < N; ++kk) {
+            const double *o = &sbuf[el][(l * S::H + kk * (kk - 1) / 2 + k) * REC];
+            const double a = cx * I.D[k + N * kk];
+            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
+            acc[5] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+        }
+        // y-pairs ((k,l'), (k,l)), l' > l
+        for (int ll = l + 1; ll < N; ++ll) {
+            const double *o = &sbuf[el][(NN * P / 2 + k * S::H + ll * (ll - 1) / 2 + l) * REC];
+            const double a = cy * I.D[l + N * ll];
+            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
+            acc[4] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+        }
+    }
+    __syncthreads();  // pair records consumed; the buffer now holds face data
+
+    // ---- face fluxes of the element's four faces; both neighbours compute a
+    // shared face from identical inputs, so they agree bit for bit
     if (valid) {
         for (int f = n; f < S::FACE; f += NN) {
             const int side = f / N, a = f - side * N;
             double wl[8], wr[8], nf[8], nB, sl, sr;
             if (side < 2) {
-                // x-face: left (side 0) between (i-1,j) and (i,j); right (side 1) between (i,j) and (i+1,j)
-                const long long bl = gidx(kp, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
-                const long long br = gidx(kp, side == 0 ? i : i + 1, j) * NN + 0 + N * a;
+                const long long bl = gidx(I, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
+                const long long br = gidx(I, side == 0 ? i : i + 1, j) * NN + 0 + N * a;
 #pragma unroll
                 for (int v = 0; v < 8; ++v) { wl[v] = u[v * vs + bl]; wr[v] = u[v * vs + br]; }
                 espp_hll_flux_x(wl, wr, g, nf, nB, sl, sr);
             } else {
-                // y-face: bottom (side 2) between (i,j-1) and (i,j); top (side 3) between (i,j) and (i,j+1)
-                const long long bb = gidx(kp, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
-                const long long bt = gidx(kp, i, side == 2 ? j : j + 1) * NN + a + N * 0;
+                const long long bb = gidx(I, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
+                const long long bt = gidx(I, i, side == 2 ? j : j + 1) * NN + a + N * 0;
                 double t1[8], t2[8], tf[8];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) { t1[v] = u[v * vs + bb]; t2[v] = u[v * vs + bt]; }
@@ -158,7 +212,7 @@ __global__ void __launch_bounds__(BLOCK) k_residual(KParams kp, const double *__
                 espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
                 swap_xy(tf, nf);
             }
-            double *o = &sbuf[el][f * 12];
+            double *o = &sbuf[el][f * 11];
 #pragma unroll
             for (int v = 0; v < 8; ++v) o[v] = nf[v];
             o[8] = nB; o[9] = sl; o[10] = sr;
@@ -166,180 +220,248 @@ __global__ void __launch_bounds__(BLOCK) k_residual(KParams kp, const double *__
     }
     __syncthreads();
 
-    // ---- surface-term values and wave speeds for dt_lim
-    double Sx[8], Sy[8];
+    // ---- surface terms, x-faces then y-faces (DG_solver.f90:872-882)
     double ax = msx, ay = msy;
-    if (valid) {
+    if (valid && (k == 0 || k == N - 1 || l == 0 || l == N - 1)) {
+        const Prim &pr = sprim[el][n];
+        const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
+        const double vx = pr.vx, vy = pr.vy, vz = pr.vz;
+        const double inv_rho = 1.0 / q[0];
+        const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
+        const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
+        const double b_sq = bx * bx + by * by + bz * bz;
+        const double p_mag = 0.5 * b_sq;
+        const double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
+        const double p_tot = p_fld + p_mag;
+        const double b_dot_u = bx * vx + by * vy + bz * vz;
+        const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
         if (k == 0 || k == N - 1) {
-            const double *o = &sbuf[el][((k == 0) ? 0 : 1) * N * 12 + l * 12];
-            const double c = (k == 0) ? (2.0 / hx) / kp.qw[0] : (2.0 / hx) / kp.qw[N - 1];
+            const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
+                                 0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
+            const double *o = &sbuf[el][((k == 0) ? 0 : 1) * N * 11 + l * 11];
+            const double c = (k == 0) ? (2.0 / hx) / I.qw[0] : (2.0 / hx) / I.qw[N - 1];
+            if (k == 0) {
 #pragma unroll
-            for (int v = 0; v < 8; ++v) Sx[v] = c * (F[v] - o[v] + GP[v] * (bxn - o[8]));
-            if (k == 0) ax = dmax(ax, o[10]);       // sr at the left face
-            if (k == N - 1) ax = dmax(ax, -o[9]);   // -sl at the right face
+                for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (F[v] - o[v] + GP[v] * (bx - o[8]));
+                ax = dmax(ax, o[10]);  // sr at the left face
+            } else {
+#pragma unroll
+                for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (F[v] - o[v] + GP[v] * (bx - o[8]));
+                ax = dmax(ax, -o[9]);  // -sl at the right face
+            }
         }
         if (l == 0 || l == N - 1) {
-            const double *o = &sbuf[el][((l == 0) ? 2 : 3) * N * 12 + k * 12];
-            const double c = (l == 0) ? (2.0 / hy) / kp.qw[0] : (2.0 / hy) / kp.qw[N - 1];
+            const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
+                                 ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
+            const double *o = &sbuf[el][((l == 0) ? 2 : 3) * N * 11 + k * 11];
+            const double c = (l == 0) ? (2.0 / hy) / I.qw[0] : (2.0 / hy) / I.qw[N - 1];
+            if (l == 0) {
 #pragma unroll
-            for (int v = 0; v < 8; ++v) Sy[v] = c * (G[v] - o[v] + GP[v] * (byn - o[8]));
-            if (l == 0) ay = dmax(ay, o[10]);
-            if (l == N - 1) ay = dmax(ay, -o[9]);
+                for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (G[v] - o[v] + GP[v] * (by - o[8]));
+                ay = dmax(ay, o[10]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (G[v] - o[v] + GP[v] * (by - o[8]));
+                ay = dmax(ay, -o[9]);
+            }
         }
     }
+    if (valid) {
+        const long long ob = ((long long)(j - 1) * I.nx + (i - 1)) * NN + n;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) ut[v * I.vsi + ob] = acc[v];
+    }
+    // ---- element dt_lim (DG_solver.f90:806); max is exact in any order
     sred[t][0] = ax;
     sred[t][1] = ay;
     __syncthreads();
-
-    // ---- element dt_lim (DG_solver.f90:806): max is exact in any order
     if (valid && n == 0) {
         double mx_ = 0.0, my_ = 0.0;
         for (int m = 0; m < NN; ++m) { mx_ = dmax(mx_, sred[el * NN + m][0]); my_ = dmax(my_, sred[el * NN + m][1]); }
-        double dtl = 1.0 / ((2 * P + 1) * mx_ / hx + (2 * P + 1) * my_ / hy);
-        atomicMin(&st->dtlim_bits, (unsigned long long)__double_as_longlong(dtl));
+        const double dtl = 1.0 / ((2 * P + 1) * mx_ / hx + (2 * P + 1) * my_ / hy);
+        atomicMin(&I.ctl.dtlim_bits, (unsigned long long)__double_as_longlong(dtl));
     }
-
-    // ---- EC pair fluxes, each pair evaluated once (P rounds, all lanes busy)
-    if (valid) {
-#pragma unroll
-        for (int c = 0; c < P; ++c) {
-            const int q = n + c * NN;
-            double ec[8];
-            if (q < NN * P / 2) {
-                // x-pair in row lr between nodes (kk, lr) and (rr, lr), rr < kk
-                const int lr = q / S::H, a = q - lr * S::H;
-                int kk = 1;
-                while ((kk + 1) * kk / 2 <= a) ++kk;
-                const int rr = a - kk * (kk - 1) / 2;
-                ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
-            } else {
-                // y-pair in column kc between nodes (kc, ll) and (kc, ss), ss < ll
-                const int qq = q - NN * P / 2;
-                const int kc = qq / S::H, a = qq - kc * S::H;
-                int ll = 1;
-                while ((ll + 1) * ll / 2 <= a) ++ll;
-                const int ss = a - ll * (ll - 1) / 2;
-                ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
-            }
-            double *o = &sbuf[el][q * 8];
-#pragma unroll
-            for (int v = 0; v < 8; ++v) o[v] = ec[v];
-        }
-    }
-    // the face data in sbuf has been consumed before the barrier above; the
-    // pair region overlaps it, so wait for all pair writes
-    __syncthreads();
-
-    if (!valid) return;
-    // ---- gather in the reference's accumulation order (DG_solver.f90:830-882)
-    const double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
-    double divB;
-    {
-        double a = 0.0, b = 0.0;
-#pragma unroll
-        for (int r = 0; r < N; ++r) a += kp.D[k + N * r] * sprim[el][r + N * l].bx;
-#pragma unroll
-        for (int s = 0; s < N; ++s) b += sprim[el][k + N * s].by * kp.D[l + N * s];
-        divB = (2.0 / hx) * a + (2.0 / hy) * b;
-    }
-    double acc[8];
-    {
-        const double dxx = cx * kp.D[k + N * k], dyy = cy * kp.D[l + N * l];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = 0.0 - dxx * F[v] - dyy * G[v] - divB * GP[v];
-    }
-    // x-pairs (k, r), r < k
-    for (int r = 0; r < k; ++r) {
-        const double *o = &sbuf[el][(l * S::H + k * (k - 1) / 2 + r) * 8];
-        const double a = cx * kp.D[k + N * r];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] - a * o[v];
-    }
-    // y-pairs ((k,l), (k,s)), s < l
-    for (int s = 0; s < l; ++s) {
-        const double *o = &sbuf[el][(NN * P / 2 + k * S::H + l * (l - 1) / 2 + s) * 8];
-        const double a = cy * kp.D[l + N * s];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] - a * o[v];
-    }
-    // x-pairs (k', k), k' > k
-    for (int kk = k + 1; kk < N; ++kk) {
-        const double *o = &sbuf[el][(l * S::H + kk * (kk - 1) / 2 + k) * 8];
-        const double a = cx * kp.D[k + N * kk];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] - a * o[v];
-    }
-    // y-pairs ((k,l'), (k,l)), l' > l
-    for (int ll = l + 1; ll < N; ++ll) {
-        const double *o = &sbuf[el][(NN * P / 2 + k * S::H + ll * (ll - 1) / 2 + l) * 8];
-        const double a = cy * kp.D[l + N * ll];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] - a * o[v];
-    }
-    // surface terms: x-faces, then y-faces
-    if (k == 0) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] - Sx[v];
-    } else if (k == N - 1) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] + Sx[v];
-    }
-    if (l == 0) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] - Sy[v];
-    } else if (l == N - 1) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) acc[v] = acc[v] + Sy[v];
-    }
-    const long long ob = ((long long)(j - 1) * kp.nx + (i - 1)) * NN + n;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) ut[v * kp.vsi + ob] = acc[v];
 }
 
 // --------------------------------------------------------------------------
-// post_process, first loop (DG_solver.f90:161-278): cell averages of the
-// interior and the four ghost strips, the inadmissible-average flag, and the
-// LDF projection of (Bx, By) as one dense 2N^2 x 2N^2 nodal operator
-// (column-major, ldf[c * 2N^2 + r]). When `partials` is non-null each
-// workgroup also writes, over its interior elements, sum_e (hx hy) avg_e and
-// the node-wise min and max of every variable after LDF (24 doubles).
+// Zhang-Shu positivity scaling of one element toward its mean
+// (DG_solver.f90:528-603). Called by every thread of the workgroup (it
+// synchronises); lanes with valid = false pass neutral values.
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_pp_avg_ldf(KParams kp, double *__restrict__ u, double *__restrict__ uavg,
-                                                      const double *__restrict__ ldf, int apply_ldf,
-                                                      double *__restrict__ partials, DevStatus *st)
+__device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8], bool valid, double g,
+                                                 double (*sred)[2], int t, int el, bool &neg_node, bool &fail)
+{
+    constexpr int NN = (P + 1) * (P + 1);
+    // density, once
+    sred[t][0] = valid ? q[0] : DBL_HUGE;
+    __syncthreads();
+    double den_min = DBL_HUGE;
+    if (valid)
+        for (int m = 0; m < NN; ++m) den_min = dmin(den_min, sred[el * NN + m][0]);
+    __syncthreads();
+    neg_node = false;
+    fail = false;
+    if (valid && den_min < PP_EPS) {
+        neg_node = true;
+        const double theta = dmax(dmin((avg[0] - PP_EPS) / (avg[0] - den_min), 1.0), 0.0);
+        q[0] = avg[0] + theta * (q[0] - avg[0]);
+        q[0] = dmax(q[0], PP_EPS);
+    }
+    // pressure: one check, then at most two shrinks by 0.6; element-uniform
+    // predicates, and every round reaches the barriers
+    const double prs_avg = valid ? prs_macro(avg[0], avg[1], avg[2], avg[3], avg[4], avg[5], avg[6], avg[7], g) : 0.0;
+    double theta_p = 0.0;
+    bool active = false;
+    for (int round = 0; round < 4; ++round) {
+        sred[t][0] = valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE;
+        __syncthreads();
+        double prs_min = DBL_HUGE;
+        if (valid)
+            for (int m = 0; m < NN; ++m) prs_min = dmin(prs_min, sred[el * NN + m][0]);
+        __syncthreads();
+        if (!valid) continue;
+        if (round == 0) {
+            if (prs_min < PP_EPS) {
+                neg_node = true;
+                active = true;
+                theta_p = dmax(dmin((prs_avg - PP_EPS) / (prs_avg - prs_min), 1.0), 0.0);
+#pragma unroll
+                for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta_p * (q[v] - avg[v]);
+            }
+        } else if (active) {
+            if (prs_min < 0.99 * PP_EPS) {
+                if (round == 3) {
+                    fail = true;
+                } else {
+                    theta_p = 0.6 * theta_p;
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta_p * (q[v] - avg[v]);
+                }
+            } else {
+                active = false;
+            }
+        }
+    }
+}
+
+// Cell entropy of one element summed in the reference's node order, then the
+// workgroup's sum over its counted elements in element order; thread 0
+// receives the workgroup partial (DG_solver.f90:635-649).
+template <int P>
+__device__ __forceinline__ double entropy_partial(const double q[8], bool valid, bool counted, const Inst &I,
+                                                  double (*sred)[2], double *sent, int *scnt, int t, int el, int n)
+{
+    constexpr int NN = (P + 1) * (P + 1), EPB = BLOCK / NN;
+    const double g = I.g;
+    sred[t][0] = valid ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
+                       : 0.0;
+    __syncthreads();
+    if (el < EPB && n == 0) {
+        double ce = 0.0;
+        if (valid)
+            for (int m = 0; m < NN; ++m) ce = ce + sred[el * NN + m][0];
+        sent[el] = ce;
+        scnt[el] = counted ? 1 : 0;
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (t == 0)
+        for (int e = 0; e < EPB; ++e)
+            if (scnt[e]) s = s + sent[e];
+    return s;
+}
+
+// --------------------------------------------------------------------------
+// Stage post-process over the interior and the four ghost strips:
+// [RK combine] -> cell averages and the inadmissible-average flag -> LDF
+// projection -> either (PP path) the positivity limiter, entropy and the
+// final state, or (OEDG path) the post-LDF state plus per-workgroup
+// (sum, min, max) records for the global OEDG normalisation
+// (DG_solver.f90:161-278,428-656; drive.f90:130-272).
+// Ghost strips are synthesised from the BC of the combined source elements,
+// exactly what BC_handle would copy (problems.f90:312-348,386-423,597-654),
+// or read from memory (F_GHOST_MEM).
+template <int P>
+__global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map, int st, int flags)
 {
     using S = Shape<P>;
     constexpr int N = S::N, NN = S::NN, EPB = S::EPB;
+    constexpr int M = 2 * NN;
     __shared__ double scons[EPB][8][NN];
     __shared__ double savg[EPB][8];
-    __shared__ double sdb[EPB][2 * NN];
+    __shared__ double sdb[EPB][M];
+    __shared__ double sldf[M * M];
     __shared__ int sok[EPB];
-    __shared__ double sred[BLOCK][8];
+    __shared__ int scnt[EPB];
+    __shared__ double sred[BLOCK][2];
+    __shared__ double sent[EPB];
 
-    if (st->neg_avg) return;
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I)) return;
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
-    const long long nslots = (long long)(kp.nx + 2) * (kp.ny + 2);
-    const long long eg = (long long)blockIdx.x * EPB + el;
-    const int ex = (int)(eg % (kp.nx + 2)), ey = (int)(eg / (kp.nx + 2));
+    const int k = n % N, l = n / N;
+    const long long nslots = (long long)(I.nx + 2) * (I.ny + 2);
+    const long long eg = (long long)bm.local * EPB + el;
+    const int ex = (int)(eg % (I.nx + 2)), ey = (int)(eg / (I.nx + 2));
     const bool inrange = (el < EPB) && (eg < nslots);
-    const bool strip = (ex >= 1 && ex <= kp.nx) || (ey >= 1 && ey <= kp.ny);  // not a corner
-    const bool valid = inrange && strip;
-    const bool interior = valid && ex >= 1 && ex <= kp.nx && ey >= 1 && ey <= kp.ny;
-    const long long vs = kp.vs;
+    const bool xin = ex >= 1 && ex <= I.nx, yin = ey >= 1 && ey <= I.ny;
+    const bool valid = inrange && (xin || yin);  // corners are never used
+    const bool interior = valid && xin && yin;
+    const long long vs = I.vs;
     const long long base = eg * NN + n;
+    double *__restrict__ out = I.u[st];
+    const bool use_ldf = I.ctl.use_ldf != 0;
 
+    if (use_ldf && !(flags & F_COMBINE_ONLY)) {
+        for (int c = t; c < M * M; c += BLOCK) sldf[c] = I.ldf[c];
+    }
+
+    // ---- the stage state at this node
     double q[8];
     if (valid) {
+        if (!(flags & F_COMBINE) || (!interior && (flags & F_GHOST_MEM))) {
 #pragma unroll
-        for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; scons[el][v][n] = q[v]; }
+            for (int v = 0; v < 8; ++v) q[v] = out[v * vs + base];
+        } else if (interior) {
+            const long long gt = ((long long)(ey - 1) * I.nx + (ex - 1)) * NN + n;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(I, st, v, base, gt);
+        } else if (I.bc_kind == BC_FIXED) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) q[v] = I.bc_state[v];
+        } else {
+            // source element and node of this ghost node
+            int sx, sy, sn;
+            const bool per = I.bc_kind == BC_PERIODIC;
+            if (ex == 0) { sx = per ? I.nx : 1; sy = ey; sn = per ? n : 0 + N * l; }
+            else if (ex == I.nx + 1) { sx = per ? 1 : I.nx; sy = ey; sn = per ? n : (N - 1) + N * l; }
+            else if (ey == 0) { sx = ex; sy = per ? I.ny : 1; sn = per ? n : k + N * 0; }
+            else { sx = ex; sy = per ? 1 : I.ny; sn = per ? n : k + N * (N - 1); }
+            const long long gs = gidx(I, sx, sy) * NN + sn;
+            const long long gt = ((long long)(sy - 1) * I.nx + (sx - 1)) * NN + sn;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(I, st, v, gs, gt);
+        }
+        if (flags & F_COMBINE_ONLY) {
+            if (interior)
+#pragma unroll
+                for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) scons[el][v][n] = q[v];
+        }
     }
+    if (flags & F_COMBINE_ONLY) return;
     __syncthreads();
-    // one lane per variable (a P1 element has only 4 lanes: stride over them)
+
+    // ---- cell average in the reference's order (DG_solver.f90:177-184), one
+    // lane per variable (striding when an element has fewer than 8 lanes)
     if (valid) {
         for (int v = n; v < 8; v += NN) {
             double a = 0.0;
-            for (int m = 0; m < NN; ++m) a = a + kp.wavg[m] * scons[el][v][m];
+            for (int m = 0; m < NN; ++m) a = a + I.wavg[m] * scons[el][v][m];
             savg[el][v] = a;
         }
     }
@@ -349,83 +471,116 @@ __global__ void __launch_bounds__(BLOCK) k_pp_avg_ldf(KParams kp, double *__rest
     if (valid) {
 #pragma unroll
         for (int v = 0; v < 8; ++v) avg[v] = savg[el][v];
-        const double den_avg = avg[0];
-        const double prs_avg = prs_macro(avg[0], avg[1], avg[2], avg[3], avg[4], avg[5], avg[6], avg[7], kp.g);
-        ok = !(den_avg < PP_EPS || prs_avg < PP_EPS);
-        if (!ok && n == 0) atomicOr(&st->neg_avg, 1);
+        const double prs_avg = prs_macro(avg[0], avg[1], avg[2], avg[3], avg[4], avg[5], avg[6], avg[7], I.g);
+        ok = !(avg[0] < PP_EPS || prs_avg < PP_EPS);
+        if (!ok && n == 0) atomicOr(&I.ctl.neg_avg, 1);
     }
     if (el < EPB && n == 0) sok[el] = (valid && ok) ? 1 : 0;
-    if (apply_ldf) {
+
+    // ---- LDF projection of (Bx, By) as one dense nodal operator (LDS-staged)
+    if (use_ldf) {
         if (valid && ok) {
             sdb[el][n] = q[4] - avg[4];
             sdb[el][NN + n] = q[5] - avg[5];
         }
         __syncthreads();
         if (valid && ok) {
-            constexpr int M = 2 * NN;
             double bxs = 0.0, bys = 0.0;
+#pragma unroll 4
             for (int c = 0; c < M; ++c) {
                 const double d = sdb[el][c];
-                bxs = bxs + ldf[c * M + n] * d;
-                bys = bys + ldf[c * M + NN + n] * d;
+                bxs = bxs + sldf[c * M + n] * d;
+                bys = bys + sldf[c * M + NN + n] * d;
             }
             q[4] = bxs + avg[4];
             q[5] = bys + avg[5];
-            u[4 * vs + base] = q[4];
-            u[5 * vs + base] = q[5];
         }
     }
     if (valid && ok)
-        for (int v = n; v < 8; v += NN) uavg[eg * 8 + v] = avg[v];
+        for (int v = n; v < 8; v += NN) I.uavg[eg * 8 + v] = avg[v];
 
-    if (partials == nullptr) return;
-    // per-workgroup (sum, min, max) over interior elements, deterministic order
-    const bool contrib = interior && ok;
+    if (stage_oedg(I, st, flags)) {
+        // post-LDF state (ghost strips too: the OEDG face jumps read them)
+        if (valid && ok)
 #pragma unroll
-    for (int v = 0; v < 8; ++v) sred[t][v] = contrib ? q[v] : 0.0;
-    __syncthreads();
-    if (t < 8) {
-        const int v = t;
-        double s = 0.0, mn = DBL_HUGE, mxv = -DBL_HUGE;
-        for (int e = 0; e < EPB; ++e) {
-            const long long egg = (long long)blockIdx.x * EPB + e;
-            if (egg >= nslots || !sok[e]) continue;
-            const int exx = (int)(egg % (kp.nx + 2)), eyy = (int)(egg / (kp.nx + 2));
-            if (!(exx >= 1 && exx <= kp.nx && eyy >= 1 && eyy <= kp.ny)) continue;
-            s = s + (kp.hx * kp.hy) * savg[e][v];
-            for (int m = 0; m < NN; ++m) {
-                const double x = sred[e * NN + m][v];
-                mn = dmin(mn, x);
-                mxv = dmax(mxv, x);
+            for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
+        // per-workgroup (sum, min, max) over interior elements, fixed order;
+        // scons is reused for the post-LDF node values
+        __syncthreads();
+        if (valid)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) scons[el][v][n] = q[v];
+        __syncthreads();
+        if (t < 8) {
+            const int v = t;
+            double s = 0.0, mn = DBL_HUGE, mxv = -DBL_HUGE;
+            for (int e = 0; e < EPB; ++e) {
+                const long long egg = (long long)bm.local * EPB + e;
+                if (egg >= nslots || !sok[e]) continue;
+                const int exx = (int)(egg % (I.nx + 2)), eyy = (int)(egg / (I.nx + 2));
+                if (!(exx >= 1 && exx <= I.nx && eyy >= 1 && eyy <= I.ny)) continue;
+                s = s + (I.hx * I.hy) * savg[e][v];
+                for (int m = 0; m < NN; ++m) {
+                    const double x = scons[e][v][m];
+                    mn = dmin(mn, x);
+                    mxv = dmax(mxv, x);
+                }
             }
+            I.part[(long long)bm.local * 24 + v] = s;
+            I.part[(long long)bm.local * 24 + 8 + v] = mn;
+            I.part[(long long)bm.local * 24 + 16 + v] = mxv;
         }
-        partials[(long long)blockIdx.x * 24 + v] = s;
-        partials[(long long)blockIdx.x * 24 + 8 + v] = mn;
-        partials[(long long)blockIdx.x * 24 + 16 + v] = mxv;
+        return;
     }
+
+    // ---- PP path: positivity limiter of interior elements, entropy, final state
+    bool neg_node = false, fail = false;
+    const bool lim = interior && ok;
+    pp_limit_element<P>(q, avg, lim, I.g, sred, t, el, neg_node, fail);
+    if (lim && n == 0) {
+        if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
+        if (fail) atomicOr(&I.ctl.pp_fail, 1);
+    }
+    if (stage_ent(I, st, flags)) {
+        const double s = entropy_partial<P>(q, lim, lim, I, sred, sent, scnt, t, el, n);
+        if (t == 0) I.entpart[bm.local] = s;
+    }
+    if (valid && ok)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
 }
 
-// Reduce the per-workgroup partials into u_avg_global and u_vari_global
-// (DG_solver.f90:228,282-294): max |u - c| = max(max u - c, c - min u) exactly.
-__global__ void __launch_bounds__(BLOCK) k_reduce_global(const double *__restrict__ partials, long long nblk,
-                                                         DevStatus *st)
+// Two-level deterministic reduction of the per-workgroup (sum, min, max)
+// records into u_avg_global and u_vari_global (DG_solver.f90:228,282-294):
+// level 1 (final = 0): workgroup (inst, chunk) reduces records
+// [256 chunk, 256 chunk + 256) by a fixed tree; level 2 (final = 1): one
+// workgroup per instance reduces its level-1 records.
+// max|u - c| = max(max u - c, c - min u) exactly.
+__global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk *map, int st, int flags, int final)
 {
     __shared__ double ss[BLOCK][8], smn[BLOCK][8], smx[BLOCK][8];
-    if (st->neg_avg) return;
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const int t = threadIdx.x;
+    const long long nrec1 = (I.nblk_slot + BLOCK - 1) / BLOCK;
     double s[8], mn[8], mx[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) { s[v] = 0.0; mn[v] = DBL_HUGE; mx[v] = -DBL_HUGE; }
-    // contiguous chunk per thread, summed in order
-    const long long per = (nblk + BLOCK - 1) / BLOCK;
-    const long long b0 = t * per, b1 = (b0 + per < nblk) ? b0 + per : nblk;
-    for (long long b = b0; b < b1; ++b)
+    if (final) {
+        for (long long r = t; r < nrec1; r += BLOCK)
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            s[v] = s[v] + partials[b * 24 + v];
-            mn[v] = dmin(mn[v], partials[b * 24 + 8 + v]);
-            mx[v] = dmax(mx[v], partials[b * 24 + 16 + v]);
-        }
+            for (int v = 0; v < 8; ++v) {
+                s[v] = s[v] + I.part2[r * 24 + v];
+                mn[v] = dmin(mn[v], I.part2[r * 24 + 8 + v]);
+                mx[v] = dmax(mx[v], I.part2[r * 24 + 16 + v]);
+            }
+    } else {
+        const long long r = (long long)bm.local * BLOCK + t;
+        if (r < I.nblk_slot)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) { s[v] = I.part[r * 24 + v]; mn[v] = I.part[r * 24 + 8 + v]; mx[v] = I.part[r * 24 + 16 + v]; }
+    }
 #pragma unroll
     for (int v = 0; v < 8; ++v) { ss[t][v] = s[v]; smn[t][v] = mn[v]; smx[t][v] = mx[v]; }
     __syncthreads();
@@ -440,51 +595,60 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_global(const double *__restric
         __syncthreads();
     }
     if (t < 8) {
-        const double c = ss[0][t];
-        st->glob[t] = c;
-        st->glob[8 + t] = dmax(dmax(smx[0][t] - c, c - smn[0][t]), 0.0);
+        if (final) {
+            const double c = ss[0][t];
+            I.ctl.glob[t] = c;
+            I.ctl.glob[8 + t] = dmax(dmax(smx[0][t] - c, c - smn[0][t]), 0.0);
+        } else {
+            I.part2[(long long)bm.local * 24 + t] = ss[0][t];
+            I.part2[(long long)bm.local * 24 + 8 + t] = smn[0][t];
+            I.part2[(long long)bm.local * 24 + 16 + t] = smx[0][t];
+        }
     }
 }
 
-// OEDG jump indicators (DG_solver.f90:295-422), one thread per (face, variable).
-// x-faces i = 1..nx+1 (between (i-1,j) and (i,j)), j = 1..ny, then y-faces.
+// OEDG jump indicators (DG_solver.f90:295-422), one thread per (face,
+// variable); x-faces i = 1..nx+1 between (i-1,j) and (i,j), then y-faces.
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_oedg_faces(KParams kp, const double *__restrict__ u,
-                                                      double *__restrict__ sigx, double *__restrict__ sigy,
-                                                      const DevStatus *st)
+__global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
     constexpr int N = P + 1, NN = N * N;
-    if (st->neg_avg) return;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nxf = (long long)(kp.nx + 1) * kp.ny * 8, nyf = (long long)kp.nx * (kp.ny + 1) * 8;
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
+    const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
+    const long long nxf = (long long)(I.nx + 1) * I.ny * 8, nyf = (long long)I.nx * (I.ny + 1) * 8;
     if (tid >= nxf + nyf) return;
-    const double hx = kp.hx, hy = kp.hy;
-    const int px = P, py = P, mxp = P;
-    const int den0 = 2 * (2 * P - 1 > 1 ? 2 * P - 1 : 1);
+    const double *__restrict__ u = I.u[st];
+    const double hx = I.hx, hy = I.hy;
+    constexpr int px = P, py = P, mxp = P;
+    constexpr int den0 = 2 * (2 * P - 1 > 1 ? 2 * P - 1 : 1);
     double A[NN], B[NN], tmp[NN], jump[N], jt[N];
     double D2x[NN], D2y[NN];
 #pragma unroll
-    for (int m = 0; m < NN; ++m) { D2x[m] = (2.0 / hx) * kp.D[m]; D2y[m] = (2.0 / hy) * kp.D[m]; }
+    for (int m = 0; m < NN; ++m) { D2x[m] = (2.0 / hx) * I.D[m]; D2y[m] = (2.0 / hy) * I.D[m]; }
     if (tid < nxf) {
         const int v = (int)(tid % 8);
         const long long f = tid / 8;
-        const int i = (int)(f % (kp.nx + 1)) + 1, j = (int)(f / (kp.nx + 1)) + 1;
-        const long long bl = gidx(kp, i - 1, j) * NN, br = gidx(kp, i, j) * NN;
+        const int i = (int)(f % (I.nx + 1)) + 1, j = (int)(f / (I.nx + 1)) + 1;
+        const long long bl = gidx(I, i - 1, j) * NN, br = gidx(I, i, j) * NN;
 #pragma unroll
-        for (int m = 0; m < NN; ++m) { A[m] = u[v * kp.vs + bl + m]; B[m] = u[v * kp.vs + br + m]; }
+        for (int m = 0; m < NN; ++m) { A[m] = u[v * I.vs + bl + m]; B[m] = u[v * I.vs + br + m]; }
         double total = 0.0;
         int fac_kk = 1;
         double hx_pow_kk = 1.0;
+#pragma unroll
         for (int kk = 0; kk <= px; ++kk) {
 #pragma unroll
             for (int l = 0; l < N; ++l) jump[l] = B[0 + N * l] - A[(N - 1) + N * l];
             int fac_ll = 1;
             double hx_pow = hx_pow_kk;
+#pragma unroll
             for (int lq = 0; lq <= mxp - kk; ++lq) {
                 const int mm = kk + lq;
                 double jl1 = 0.0;
 #pragma unroll
-                for (int l = 0; l < N; ++l) jl1 += (0.5 * kp.qw[l]) * fabs(jump[l]);
+                for (int l = 0; l < N; ++l) jl1 += (0.5 * I.qw[l]) * fabs(jump[l]);
                 const double coef = ((double)(2 * mm + 1) * hx_pow) / (double)(den0 * fac_kk * fac_ll);
                 total = total + coef * jl1;
                 if (lq < py) {
@@ -528,28 +692,30 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(KParams kp, const double *
                 hx_pow_kk = hx_pow_kk * hx;
             }
         }
-        sigx[f * 8 + v] = total;
+        I.sigx[f * 8 + v] = total;
     } else {
         const long long tt = tid - nxf;
         const int v = (int)(tt % 8);
         const long long f = tt / 8;
-        const int i = (int)(f % kp.nx) + 1, j = (int)(f / kp.nx) + 1;
-        const long long bb = gidx(kp, i, j - 1) * NN, bt = gidx(kp, i, j) * NN;
+        const int i = (int)(f % I.nx) + 1, j = (int)(f / I.nx) + 1;
+        const long long bb = gidx(I, i, j - 1) * NN, bt = gidx(I, i, j) * NN;
 #pragma unroll
-        for (int m = 0; m < NN; ++m) { A[m] = u[v * kp.vs + bb + m]; B[m] = u[v * kp.vs + bt + m]; }
+        for (int m = 0; m < NN; ++m) { A[m] = u[v * I.vs + bb + m]; B[m] = u[v * I.vs + bt + m]; }
         double total = 0.0;
         int fac_ll = 1;
         double hy_pow_ll = 1.0;
+#pragma unroll
         for (int lq = 0; lq <= py; ++lq) {
 #pragma unroll
             for (int k = 0; k < N; ++k) jump[k] = B[k + N * 0] - A[k + N * (N - 1)];
             int fac_kk = 1;
             double hy_pow = hy_pow_ll;
+#pragma unroll
             for (int kk = 0; kk <= mxp - lq; ++kk) {
                 const int mm = kk + lq;
                 double jl1 = 0.0;
 #pragma unroll
-                for (int k = 0; k < N; ++k) jl1 += (0.5 * kp.qw[k]) * fabs(jump[k]);
+                for (int k = 0; k < N; ++k) jl1 += (0.5 * I.qw[k]) * fabs(jump[k]);
                 const double coef = ((double)(2 * mm + 1) * hy_pow) / (double)(den0 * fac_kk * fac_ll);
                 total = total + coef * jl1;
                 if (kk < px) {
@@ -593,42 +759,42 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(KParams kp, const double *
                 hy_pow_ll = hy_pow_ll * hy;
             }
         }
-        sigy[f * 8 + v] = total;
+        I.sigy[f * 8 + v] = total;
     }
 }
 
-// post_process, limiter loop (DG_solver.f90:428-656): OEDG damping, the
-// Zhang-Shu positivity scaling toward the cell mean, and the cell entropy.
-// Each workgroup writes the entropy sum of its elements to ent_partials.
+// OEDG path, limiter loop (DG_solver.f90:428-656): OEDG damping toward the
+// cell mean, the positivity limiter, the cell entropy, the final state.
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_pp_limit(KParams kp, double *__restrict__ u, const double *__restrict__ uavg,
-                                                    const double *__restrict__ sigx, const double *__restrict__ sigy,
-                                                    int apply_oedg, double oedg_dt, int comput_ent,
-                                                    double *__restrict__ ent_partials, DevStatus *st)
+__global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *map, int st, int flags)
 {
     using S = Shape<P>;
-    constexpr int N = S::N, NN = S::NN, EPB = S::EPB;
+    constexpr int NN = S::NN, EPB = S::EPB;
     __shared__ double sred[BLOCK][2];
     __shared__ double sent[EPB];
+    __shared__ int scnt[EPB];
 
-    if (st->neg_avg) return;
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
-    const long long eg = (long long)blockIdx.x * EPB + el;
-    const bool valid = (el < EPB) && (eg < (long long)kp.nx * kp.ny);
-    const int i = valid ? (int)(eg % kp.nx) + 1 : 1;
-    const int j = valid ? (int)(eg / kp.nx) + 1 : 1;
-    const double g = kp.g;
-    const long long vs = kp.vs;
-    const long long ge = gidx(kp, i, j);
+    const long long eg = (long long)bm.local * EPB + el;
+    const bool valid = (el < EPB) && (eg < (long long)I.nx * I.ny);
+    const int i = valid ? (int)(eg % I.nx) + 1 : 1;
+    const int j = valid ? (int)(eg / I.nx) + 1 : 1;
+    const double g = I.g;
+    const long long vs = I.vs;
+    const long long ge = gidx(I, i, j);
     const long long base = ge * NN + n;
+    double *__restrict__ u = I.u[st];
 
     double q[8], avg[8];
     if (valid) {
 #pragma unroll
-        for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; avg[v] = uavg[ge * 8 + v]; }
+        for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; avg[v] = I.uavg[ge * 8 + v]; }
     }
-    if (apply_oedg) {
+    {
         double sx = 0.0, sy = 0.0;
         if (valid) node_speeds(dmax(q[0], PP_EPS), q[1], q[2], q[3], q[4], q[5], q[6], q[7], g, sx, sy);
         sred[t][0] = sx;
@@ -637,98 +803,36 @@ __global__ void __launch_bounds__(BLOCK) k_pp_limit(KParams kp, double *__restri
         if (valid) {
             double mex = 0.0, mey = 0.0;
             for (int m = 0; m < NN; ++m) { mex = dmax(mex, sred[el * NN + m][0]); mey = dmax(mey, sred[el * NN + m][1]); }
-            const double *sxL = sigx + 8 * ((long long)(i - 1) + (long long)(kp.nx + 1) * (j - 1));
-            const double *sxR = sigx + 8 * ((long long)i + (long long)(kp.nx + 1) * (j - 1));
-            const double *syB = sigy + 8 * ((long long)(i - 1) + (long long)kp.nx * (j - 1));
-            const double *syT = sigy + 8 * ((long long)(i - 1) + (long long)kp.nx * j);
+            const double *sxL = I.sigx + 8 * ((long long)(i - 1) + (long long)(I.nx + 1) * (j - 1));
+            const double *sxR = I.sigx + 8 * ((long long)i + (long long)(I.nx + 1) * (j - 1));
+            const double *syB = I.sigy + 8 * ((long long)(i - 1) + (long long)I.nx * (j - 1));
+            const double *syT = I.sigy + 8 * ((long long)(i - 1) + (long long)I.nx * j);
             double dm = 0.0;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
-                const double var = st->glob[8 + v];
+                const double var = I.ctl.glob[8 + v];
                 double delta;
                 if (var > OEDG_EPS)
-                    delta = ((mex / kp.hx) * (sxL[v] + sxR[v]) + (mey / kp.hy) * (syB[v] + syT[v])) / var;
+                    delta = ((mex / I.hx) * (sxL[v] + sxR[v]) + (mey / I.hy) * (syB[v] + syT[v])) / var;
                 else
                     delta = 0.0;
                 dm = (v == 0) ? delta : dmax(dm, delta);
             }
-            const double theta = exp(-oedg_dt * dm);
+            const double theta = exp(-stage_oedg_dt(I, st) * dm);
 #pragma unroll
             for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta * (q[v] - avg[v]);
         }
         __syncthreads();
     }
-    // density (once)
-    sred[t][0] = valid ? q[0] : DBL_HUGE;
-    __syncthreads();
-    bool neg_node = false;
-    double den_min = DBL_HUGE;
-    if (valid)
-        for (int m = 0; m < NN; ++m) den_min = dmin(den_min, sred[el * NN + m][0]);
-    __syncthreads();
-    if (valid && den_min < PP_EPS) {
-        neg_node = true;
-        const double theta = dmax(dmin((avg[0] - PP_EPS) / (avg[0] - den_min), 1.0), 0.0);
-        q[0] = avg[0] + theta * (q[0] - avg[0]);
-        q[0] = dmax(q[0], PP_EPS);
-    }
-    // pressure: first check, then at most two shrinks by 0.6 (element-uniform
-    // predicates; every round reaches the barriers)
-    double prs_avg = 0.0, theta_p = 0.0;
-    if (valid) prs_avg = prs_macro(avg[0], avg[1], avg[2], avg[3], avg[4], avg[5], avg[6], avg[7], g);
-    bool active = false, fail = false;
-    for (int round = 0; round < 4; ++round) {
-        sred[t][0] = valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE;
-        __syncthreads();
-        double prs_min = DBL_HUGE;
-        if (valid)
-            for (int m = 0; m < NN; ++m) prs_min = dmin(prs_min, sred[el * NN + m][0]);
-        __syncthreads();
-        if (valid) {
-            if (round == 0) {
-                if (prs_min < PP_EPS) {
-                    neg_node = true;
-                    active = true;
-                    theta_p = dmax(dmin((prs_avg - PP_EPS) / (prs_avg - prs_min), 1.0), 0.0);
-#pragma unroll
-                    for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta_p * (q[v] - avg[v]);
-                }
-            } else if (active) {
-                if (prs_min < 0.99 * PP_EPS) {
-                    if (round == 3) {
-                        fail = true;
-                    } else {
-                        theta_p = 0.6 * theta_p;
-#pragma unroll
-                        for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta_p * (q[v] - avg[v]);
-                    }
-                } else {
-                    active = false;
-                }
-            }
-        }
-    }
+    bool neg_node = false, fail = false;
+    pp_limit_element<P>(q, avg, valid, g, sred, t, el, neg_node, fail);
     if (valid && n == 0) {
-        if (neg_node) atomicAdd(&st->num_negcell, 1);
-        if (fail) atomicOr(&st->pp_fail, 1);
+        if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
+        if (fail) atomicOr(&I.ctl.pp_fail, 1);
     }
-    if (comput_ent) {
-        double e = 0.0;
-        if (valid) e = kp.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g);
-        sred[t][0] = e;
-        __syncthreads();
-        if (valid && n == 0) {
-            double ce = 0.0;
-            for (int m = 0; m < NN; ++m) ce = ce + sred[el * NN + m][0];
-            sent[el] = ce;
-        }
-        __syncthreads();
-        if (t == 0) {
-            double s = 0.0;
-            for (int e2 = 0; e2 < EPB; ++e2)
-                if ((long long)blockIdx.x * EPB + e2 < (long long)kp.nx * kp.ny) s = s + sent[e2];
-            ent_partials[blockIdx.x] = s;
-        }
+    if (stage_ent(I, st, flags)) {
+        const double s = entropy_partial<P>(q, valid, valid, I, sred, sent, scnt, t, el, n);
+        if (t == 0) I.entpart[bm.local] = s;
     }
     if (valid) {
 #pragma unroll
@@ -736,140 +840,211 @@ __global__ void __launch_bounds__(BLOCK) k_pp_limit(KParams kp, double *__restri
     }
 }
 
-// Sum of the per-workgroup entropy partials in a fixed order.
-__global__ void __launch_bounds__(BLOCK) k_reduce_sum(const double *__restrict__ parts, long long n, DevStatus *st)
+// total entropy of each instance: one workgroup per instance, contiguous
+// chunks summed in order, then a fixed tree
+__global__ void __launch_bounds__(BLOCK) k_reduce_ent(Inst *insts, const int *list, int st, int flags)
 {
     __shared__ double ss[BLOCK];
-    if (st->neg_avg) return;
+    Inst &I = insts[list[blockIdx.x]];
+    if (!inst_live(I) || !stage_ent(I, st, flags)) return;
+    const long long n = stage_oedg(I, st, flags) ? I.nblk_el : I.nblk_slot;
     const int t = threadIdx.x;
     const long long per = (n + BLOCK - 1) / BLOCK;
     const long long b0 = t * per, b1 = (b0 + per < n) ? b0 + per : n;
     double s = 0.0;
-    for (long long b = b0; b < b1; ++b) s = s + parts[b];
+    for (long long b = b0; b < b1; ++b) s = s + I.entpart[b];
     ss[t] = s;
     __syncthreads();
     for (int w = BLOCK / 2; w > 0; w >>= 1) {
         if (t < w) ss[t] = ss[t] + ss[t + w];
         __syncthreads();
     }
-    if (t == 0) st->tot_ent = ss[0];
+    if (t == 0) I.ctl.tot_ent = ss[0];
 }
 
-// Ghost strips (BC_periodic / BC_YeeSjogreen / BC_rotor-type fixed state,
-// problems.f90:312-348,386-423,597-654). One thread per (ghost element, node).
+// Ghost strips of state slot `slot` (BC_periodic / BC_YeeSjogreen /
+// BC_rotor-type fixed state, problems.f90:312-348,386-423,597-654); one
+// thread per (ghost element, node)
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_bc(KParams kp, double *__restrict__ u, const DevStatus *st)
+__global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int slot)
 {
     constexpr int N = P + 1, NN = N * N;
-    if (st && st->neg_avg) return;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nstrip = 2LL * kp.ny + 2LL * kp.nx;
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I) || I.bc_kind == BC_HOST) return;
+    const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
+    const long long nstrip = 2LL * I.ny + 2LL * I.nx;
     if (tid >= nstrip * NN) return;
+    double *__restrict__ u = I.u[slot];
     const int n = (int)(tid % NN);
     const long long s = tid / NN;
     const int k = n % N, l = n / N;
+    const bool per = I.bc_kind == BC_PERIODIC;
     int gx, gy, sx, sy, sn;
-    if (s < kp.ny) {                         // left ghost (0, j)
+    if (s < I.ny) {
         const int j = (int)s + 1;
-        gx = 0; gy = j;
-        if (kp.bc_kind == BC_PERIODIC) { sx = kp.nx; sy = j; sn = n; }
-        else { sx = 1; sy = j; sn = 0 + N * l; }
-    } else if (s < 2LL * kp.ny) {            // right ghost (nx+1, j)
-        const int j = (int)(s - kp.ny) + 1;
-        gx = kp.nx + 1; gy = j;
-        if (kp.bc_kind == BC_PERIODIC) { sx = 1; sy = j; sn = n; }
-        else { sx = kp.nx; sy = j; sn = (N - 1) + N * l; }
-    } else if (s < 2LL * kp.ny + kp.nx) {    // bottom ghost (i, 0)
-        const int i = (int)(s - 2LL * kp.ny) + 1;
-        gx = i; gy = 0;
-        if (kp.bc_kind == BC_PERIODIC) { sx = i; sy = kp.ny; sn = n; }
-        else { sx = i; sy = 1; sn = k + N * 0; }
-    } else {                                 // top ghost (i, ny+1)
-        const int i = (int)(s - 2LL * kp.ny - kp.nx) + 1;
-        gx = i; gy = kp.ny + 1;
-        if (kp.bc_kind == BC_PERIODIC) { sx = i; sy = 1; sn = n; }
-        else { sx = i; sy = kp.ny; sn = k + N * (N - 1); }
+        gx = 0; gy = j; sx = per ? I.nx : 1; sy = j; sn = per ? n : 0 + N * l;
+    } else if (s < 2LL * I.ny) {
+        const int j = (int)(s - I.ny) + 1;
+        gx = I.nx + 1; gy = j; sx = per ? 1 : I.nx; sy = j; sn = per ? n : (N - 1) + N * l;
+    } else if (s < 2LL * I.ny + I.nx) {
+        const int i = (int)(s - 2LL * I.ny) + 1;
+        gx = i; gy = 0; sx = i; sy = per ? I.ny : 1; sn = per ? n : k + N * 0;
+    } else {
+        const int i = (int)(s - 2LL * I.ny - I.nx) + 1;
+        gx = i; gy = I.ny + 1; sx = i; sy = per ? 1 : I.ny; sn = per ? n : k + N * (N - 1);
     }
-    const long long dst = gidx(kp, gx, gy) * NN + n;
-    const long long src = gidx(kp, sx, sy) * NN + sn;
+    const long long dst = gidx(I, gx, gy) * NN + n;
+    const long long src = gidx(I, sx, sy) * NN + sn;
 #pragma unroll
-    for (int v = 0; v < 8; ++v) u[v * kp.vs + dst] = (kp.bc_kind == BC_FIXED) ? kp.bc_state[v] : u[v * kp.vs + src];
+    for (int v = 0; v < 8; ++v) u[v * I.vs + dst] = (I.bc_kind == BC_FIXED) ? I.bc_state[v] : u[v * I.vs + src];
 }
 
-// Copy the ghost strips of a state into / out of a compact strip buffer
-// (dir 0: state -> strips, dir 1: strips -> state).
+// Copy the ghost strips of u^n into (dir 0) or out of (dir 1) the saved
+// strips; dir 1 runs only for instances retrying a step. The committed
+// solution's ghost layer of the reference is never refreshed after the
+// start-up (drive.f90:304-311 copies the interior only), and a retried
+// residual (drive.f90:104) reads it.
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_ghost_copy(KParams kp, double *__restrict__ u, double *__restrict__ strips, int dir)
+__global__ void __launch_bounds__(BLOCK) k_ghost_copy(Inst *insts, const Blk *map, int dir)
 {
-    constexpr int N = P + 1, NN = N * N;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nstrip = 2LL * kp.ny + 2LL * kp.nx;
+    constexpr int NN = (P + 1) * (P + 1);
+    const Blk bm = map[blockIdx.x];
+    Inst &I = insts[bm.inst];
+    if (!inst_live(I)) return;
+    if (dir == 1 && !I.ctl.retry_residual) return;
+    const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
+    const long long nstrip = 2LL * I.ny + 2LL * I.nx;
     if (tid >= nstrip * NN) return;
     const int n = (int)(tid % NN);
     const long long s = tid / NN;
     int gx, gy;
-    if (s < kp.ny) { gx = 0; gy = (int)s + 1; }
-    else if (s < 2LL * kp.ny) { gx = kp.nx + 1; gy = (int)(s - kp.ny) + 1; }
-    else if (s < 2LL * kp.ny + kp.nx) { gx = (int)(s - 2LL * kp.ny) + 1; gy = 0; }
-    else { gx = (int)(s - 2LL * kp.ny - kp.nx) + 1; gy = kp.ny + 1; }
-    const long long g = gidx(kp, gx, gy) * NN + n;
+    if (s < I.ny) { gx = 0; gy = (int)s + 1; }
+    else if (s < 2LL * I.ny) { gx = I.nx + 1; gy = (int)(s - I.ny) + 1; }
+    else if (s < 2LL * I.ny + I.nx) { gx = (int)(s - 2LL * I.ny) + 1; gy = 0; }
+    else { gx = (int)(s - 2LL * I.ny - I.nx) + 1; gy = I.ny + 1; }
+    const long long gg = gidx(I, gx, gy) * NN + n;
+    double *__restrict__ u = I.u[0];
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
         const long long b = (long long)v * nstrip * NN + tid;
-        if (dir == 0) strips[b] = u[v * kp.vs + g];
-        else u[v * kp.vs + g] = strips[b];
+        if (dir == 0) I.ghost0[b] = u[v * I.vs + gg];
+        else u[v * I.vs + gg] = I.ghost0[b];
     }
 }
 
-// SSP-RK stage combine over interior nodes (drive.f90:130-272). Coefficients
-// come as (a_n, a_1..a_4 on stages, b_t on dt*ut_s, b_t4 on dt*ut_4) for the
-// reference's written forms; `form` selects the expression shape so that the
-// rounding sequence equals the reference's.
-struct CombineArgs {
-    const double *un, *r1, *r2, *r3, *r4, *t, *t4;
-    double *out;
-    double dt;
-    int form;  // 0: un + dt*t; 1: a*un + b*(r + dt*t) with (a,b) = c0,c1 and r = r1 or r2;
-               // 2: RK(5,4) stages 2-4: c0*un + c1*r + (c2*dt)*t; 3: RK(5,4) stage 5;
-               // 4: RK(5,4) stage 1: un + (c2*dt)*t
-    double c0, c1, c2, c3, c4, c5;
-};
-
-template <int P>
-__global__ void __launch_bounds__(BLOCK) k_combine(KParams kp, CombineArgs a, const DevStatus *st)
+// t_RK and oedg_beta of every stage (drive.f90:142-268)
+__device__ inline void rk_times(Ctl &c, int S)
 {
-    constexpr int NN = (P + 1) * (P + 1);
-    if (st->neg_avg) return;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long per_var = (long long)kp.nx * kp.ny * NN;
-    if (tid >= per_var * 8) return;
-    const int v = (int)(tid / per_var);
-    const long long r = tid - v * per_var;
-    const int n = (int)(r % NN);
-    const long long e = r / NN;
-    const int i = (int)(e % kp.nx) + 1, j = (int)(e / kp.nx) + 1;
-    const long long gs = v * kp.vs + gidx(kp, i, j) * NN + n;
-    const long long gt = v * kp.vsi + e * NN + n;
-    const double dt = a.dt;
-    double res;
-    switch (a.form) {
-    case 0: res = a.un[gs] + dt * a.t[gt]; break;
-    case 1: res = a.c0 * a.un[gs] + a.c1 * (a.r1[gs] + dt * a.t[gt]); break;
-    case 2: res = a.c0 * a.un[gs] + a.c1 * a.r1[gs] + (a.c2 * dt) * a.t[gt]; break;
-    case 4: res = a.un[gs] + (a.c2 * dt) * a.t[gt]; break;
-    default:
-        res = a.c0 * a.r1[gs] + a.c1 * a.r2[gs] + (a.c2 * dt) * a.t4[gt] + a.c3 * a.r3[gs] + (a.c4 * dt) * a.t[gt];
-        break;
+    const double t = c.t, dt = c.dt;
+    double *tRK = c.tRK, *beta = c.beta;
+    if (S == 3) {
+        tRK[0] = t + dt * 1.0; beta[0] = 1.0;
+        tRK[1] = 0.75 * t + 0.25 * (tRK[0] + dt * 1.0); beta[1] = 0.25;
+        tRK[2] = (1.0 / 3.0) * t + (2.0 / 3.0) * (tRK[1] + dt * 1.0); beta[2] = 2.0 / 3.0;
+    } else if (S == 2) {
+        tRK[0] = t + dt * 1.0; beta[0] = 1.0;
+        tRK[1] = 0.5 * t + 0.5 * (tRK[0] + dt * 1.0); beta[1] = 0.5;
+    } else if (S == 1) {
+        tRK[0] = t + dt * 1.0; beta[0] = 1.0;
+    } else {
+        tRK[0] = t + (0.391752226571890 * dt) * 1.0; beta[0] = 0.391752226571890;
+        tRK[1] = 0.444370493651235 * t + 0.555629506348765 * tRK[0] + (0.368410593050371 * dt) * 1.0; beta[1] = 0.368410593050371;
+        tRK[2] = 0.620101851488403 * t + 0.379898148511597 * tRK[1] + (0.251891774271694 * dt) * 1.0; beta[2] = 0.251891774271694;
+        tRK[3] = 0.178079954393132 * t + 0.821920045606868 * tRK[2] + (0.544974750228521 * dt) * 1.0; beta[3] = 0.544974750228521;
+        tRK[4] = 0.517231671970585 * tRK[1] + 0.096059710526147 * tRK[2] + (0.063692468666290 * dt) * 1.0 +
+                 0.386708617503269 * tRK[3] + (0.226007483236906 * dt) * 1.0;
+        beta[4] = 0.063692468666290 + 0.226007483236906;
     }
-    a.out[gs] = res;
 }
 
-__global__ void k_status_reset(DevStatus *st, int full)
+__device__ inline double dlim_of(const Ctl &c) { return __longlong_as_double((long long)c.dtlim_bits); }
+
+// Start of an RK step for every active instance (drive.f90:84-120): the
+// first try takes dt = CFL_RK CFL dt_lim; a retry shrinks the previous dt by
+// (sqrt(1.25) - 0.5)^(try-1) and recomputes the residual of u^n; dt is clamped
+// to t_final; more than 10 tries or dt_lim < 1e-14 fails the instance.
+__global__ void k_step_begin(Inst *insts, const int *list, int n)
 {
-    st->dtlim_bits = 0x7FF0000000000000ULL;  // +inf
-    st->num_negcell = 0;
-    st->tot_ent = 0.0;
-    if (full) { st->neg_avg = 0; st->pp_fail = 0; }
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    Inst &I = insts[list[q]];
+    Ctl &c = I.ctl;
+    if (c.state != ST_ACTIVE) return;
+    const double dt_lim = dlim_of(c);
+    c.neg_avg = 0;
+    c.pp_fail = 0;
+    c.num_negcell = 0;
+    if (!c.retry_residual) {
+        if (c.max_steps > 0 && c.steps >= c.max_steps) { c.state = ST_DONE; return; }
+        c.steps++;
+        c.try_num = 1;
+        if (dt_lim < 1e-14) { c.state = ST_FAILED; c.status = S_DT_COLLAPSE; return; }
+        c.dt = c.cfl_eff * dt_lim;
+    } else {
+        c.try_num++;
+        if (c.try_num > 10 || dt_lim < 1e-14) { c.state = ST_FAILED; c.status = S_DT_COLLAPSE; return; }
+        c.retries++;
+        double base = sqrt(1.25) - 0.5, r = 1.0;
+        for (int e = c.try_num - 1; e > 0; e >>= 1) {
+            if (e & 1) r *= base;
+            base *= base;
+        }
+        c.dt = r * c.dt;
+    }
+    if (c.t_final > 0.0) {
+        if (c.t + c.dt >= c.t_final) { c.dt = c.t_final - c.t; c.flag_last = 1; }
+        else c.flag_last = 0;
+    }
+    rk_times(c, I.S);
+}
+
+// reset the dt_lim minimum before a residual (left alone when the stage
+// failed, so that it keeps the last completed residual's value, as dt_lim
+// does in drive.f90)
+__global__ void k_pre_residual(Inst *insts, const int *list, int n, int mode)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    Inst &I = insts[list[q]];
+    if (!inst_live(I)) return;
+    if (mode == 1 && !I.ctl.retry_residual) return;
+    I.ctl.dtlim_bits = 0x7FF0000000000000ULL;
+}
+
+// End of an RK step (drive.f90:285-312): an inadmissible average schedules a
+// retry; a failed PP fallback fails the instance; otherwise commit
+// u^n <- uRK(S) by swapping the slot pointers, t <- t_RK(S).
+__global__ void k_step_end(Inst *insts, const int *list, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    Inst &I = insts[list[q]];
+    Ctl &c = I.ctl;
+    if (c.state != ST_ACTIVE) return;
+    if (c.neg_avg) { c.retry_residual = 1; return; }
+    if (c.pp_fail) { c.state = ST_FAILED; c.status = S_PP_FAIL; return; }
+    double *tmp = I.u[0];
+    I.u[0] = I.u[I.S];
+    I.u[I.S] = tmp;
+    c.t = c.tRK[I.S - 1];
+    c.negcells += c.num_negcell;
+    c.stages += I.S;
+    c.retry_residual = 0;
+    c.try_num = 0;
+    if (c.flag_last) c.state = ST_DONE;
+    if (c.max_steps > 0 && c.steps >= c.max_steps) c.state = ST_DONE;
+}
+
+// host-requested resets of the per-call flags
+__global__ void k_reset_flags(Inst *insts, const int *list, int n, int mask)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    Ctl &c = insts[list[q]].ctl;
+    if (mask & 1) c.dtlim_bits = 0x7FF0000000000000ULL;
+    if (mask & 2) { c.num_negcell = 0; c.tot_ent = 0.0; }
+    if (mask & 4) { c.neg_avg = 0; c.pp_fail = 0; }
 }
 
 }  // namespace ppes

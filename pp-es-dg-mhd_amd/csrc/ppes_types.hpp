@@ -1,0 +1,120 @@
+// Per-instance records shared by the host engine and the device kernels of
+// the PP-ES nodal DG step (gfx950, FP64).
+//
+// The per-instance data -- sizes, tables, buffer pointers and the drive-loop
+// state -- is one `Inst` record in device memory; kernels read it through a
+// block map (block -> instance, local block), so one launch covers all
+// instances of one polynomial degree, and a single instance is a batch of
+// one. Everything that changes from step to step (dt, the RK pseudo-times,
+// which state slot holds u^n) lives in the record, so a whole RK step can be
+// replayed from a captured hipGraph.
+//
+// Data layout in HBM (structure of arrays, per instance):
+//   state  u[v][ey][ex][n]      v = 0..7, ex = 0..nx+1, ey = 0..ny+1 (one
+//                                ghost layer), n = k + N*l the GLL node
+//   resid  ut[v][ey-1][ex-1][n]  interior elements only
+//   avg    uavg[ey][ex][v]       cell averages, one 64-byte record per element
+// A 64-lane wavefront reading variable v of consecutive elements touches one
+// contiguous run of doubles.
+//
+// Element-group mapping: a 256-thread workgroup holds EPB = 256 / N^2
+// elements, each owned by N^2 consecutive lanes (lane = GLL node).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ppes_physics.hpp"
+
+namespace ppes {
+
+constexpr int BLOCK = 256;
+constexpr int MAXN = 5;  // p <= 4 on the device
+constexpr int MAXS = 5;  // RK stages
+
+enum InstState { ST_IDLE = 0, ST_ACTIVE = 1, ST_DONE = 2, ST_FAILED = 3 };
+enum BcKind { BC_PERIODIC = 0, BC_EXTRAP = 1, BC_FIXED = 2, BC_HOST = 3 };
+enum StatusCode { S_OK = 0, S_NEG_AVG = 1, S_DT_COLLAPSE = 2, S_PP_FAIL = 3 };
+
+// drive-loop state of one instance (drive.f90:83-320), device resident
+struct Ctl {
+    double t, dt, t_final, cfl_eff, tot_ent;
+    double tRK[MAXS], beta[MAXS];
+    double glob[16];                 // u_avg_global[8], u_vari_global[8]
+    unsigned long long dtlim_bits;   // dt_lim of the last completed residual
+    int neg_avg, pp_fail, num_negcell, try_num;
+    int state, status, flag_last, retry_residual;
+    int oedg_mode, use_ldf, pad0, pad1;
+    long long steps, retries, negcells, stages, max_steps;
+};
+
+struct Inst {
+    int nx, ny, p, S;
+    long long vs, vsi;               // variable strides of state / residual arrays
+    double hx, hy, g;
+    double qw[MAXN];
+    double D[MAXN * MAXN];           // D[k + N r] = l_r'(x_k)
+    double wavg[MAXN * MAXN];        // (0.25 w_k) w_l
+    double went[MAXN * MAXN];        // (((0.25 hx) hy) w_k) w_l
+    int bc_kind, pad0;
+    double bc_state[8];
+    double *u[MAXS + 1];             // state slots: 0 = u^n, s = uRK(s)
+    double *ut[MAXS + 1];            // residual slots 1..S
+    double *uavg, *sigx, *sigy, *ldf, *part, *part2, *entpart, *ghost0;
+    long long nblk_el, nblk_slot;    // workgroups of the element / slot grids
+    Ctl ctl;
+};
+
+// flags of the stage post-process kernel
+enum PPFlags {
+    F_COMBINE = 1,        // form the RK stage from u^n, u^(s-1), ut(s)
+    F_GHOST_MEM = 2,      // ghost strips are read from memory (else synthesised from the BC)
+    F_FORCE_OEDG = 4,     // OEDG path regardless of the instance's mode
+    F_FORCE_NO_OEDG = 8,  // PP path regardless of the instance's mode
+    F_COMBINE_ONLY = 16,  // write the combined interior and stop
+    F_ENT = 32,           // compute the entropy (else: on the final stage)
+    F_NO_ENT = 64,
+};
+
+struct Blk {
+    int inst, local;
+};
+
+template <int P>
+struct Shape {
+    static constexpr int N = P + 1;
+    static constexpr int NN = N * N;
+    static constexpr int EPB = BLOCK / NN;                 // elements per workgroup
+    static constexpr int H = N * (N - 1) / 2;              // pairs per line
+    static constexpr int PAIRS = NN * P;                   // x- plus y-pairs per element
+    static constexpr int FACE = 4 * N;                     // face nodes per element
+    static constexpr int REC = 7;                          // doubles per stored pair flux
+    static constexpr int BUF = (PAIRS * REC > FACE * 11) ? PAIRS * REC : FACE * 11;
+};
+
+__device__ __forceinline__ long long gidx(const Inst &I, int ex, int ey) { return (long long)ey * (I.nx + 2) + ex; }
+
+__device__ __forceinline__ bool inst_live(const Inst &I) { return I.ctl.state == ST_ACTIVE && !I.ctl.neg_avg; }
+
+// does stage st of this instance apply OEDG? (drive.f90:280 and :283)
+__device__ __forceinline__ bool stage_oedg(const Inst &I, int st, int flags)
+{
+    if (flags & F_FORCE_OEDG) return true;
+    if (flags & F_FORCE_NO_OEDG) return false;
+    return I.ctl.oedg_mode == 2 || (I.ctl.oedg_mode == 1 && st == I.S);
+}
+
+__device__ __forceinline__ bool stage_ent(const Inst &I, int st, int flags)
+{
+    if (flags & F_ENT) return true;
+    if (flags & F_NO_ENT) return false;
+    return st == I.S;
+}
+
+// OEDG dt of stage st: dt on the final stage (drive.f90:280) or beta_s dt on
+// every stage (drive.f90:283)
+__device__ __forceinline__ double stage_oedg_dt(const Inst &I, int st)
+{
+    return I.ctl.oedg_mode == 2 ? I.ctl.beta[st - 1] * I.ctl.dt : I.ctl.dt;
+}
+
+}  // namespace ppes

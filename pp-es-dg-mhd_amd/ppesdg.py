@@ -4,7 +4,8 @@ The class ``Solver`` mirrors the reference's module API (DG_solver.f90,
 drive.f90, problems.f90): ``post_process``, ``semidiscretize``,
 ``rk_combine``, ``apply_bc``, ``solve`` (= solve_execute). Host arrays use the
 reference's Fortran layouts: state (8, N, N, nx+2, ny+2), residual
-(8, N, N, nx, ny), averages (8, nx+2, ny+2).
+(8, N, N, nx, ny), averages (8, nx+2, ny+2). ``Batch`` runs many independent
+instances (own mesh, degree, BC, dt and end time) through shared launches.
 
 There is no CPU fallback: if the HIP library is missing, loading raises.
 """
@@ -18,7 +19,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libppesdg.so")
 
 OK, NEG_AVG, DT_COLLAPSE, PP_FAIL, BAD_ARG, HIP_ERROR, OOM, UNSUPPORTED = range(8)
 BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3}
-KERNELS = {"residual": 0, "avg_ldf": 1, "oedg": 2, "limit": 3, "combine": 4, "bc": 5}
+KERNELS = {"residual": 0, "stage": 1, "oedg": 2, "limit": 3, "reduce": 4, "bc": 5}
+INST_STATE = {0: "idle", 1: "active", 2: "done", 3: "failed"}
 PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5}
 
 _lib = None
@@ -80,11 +82,28 @@ def lib():
                              ctypes.POINTER(RunStats)],
             "ppes_set_timing": [vp, ctypes.c_int],
             "ppes_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
+            "ppes_batch_create": [ctypes.c_int, ctypes.POINTER(vp)],
+            "ppes_batch_destroy": [vp],
+            "ppes_batch_add": [vp, ctypes.POINTER(Config), ip],
+            "ppes_batch_remove": [vp, ctypes.c_int],
+            "ppes_batch_init_problem": [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double],
+            "ppes_batch_upload_state": [vp, ctypes.c_int, ctypes.c_int, dp],
+            "ppes_batch_download_state": [vp, ctypes.c_int, ctypes.c_int, dp],
+            "ppes_batch_download_avg": [vp, ctypes.c_int, dp],
+            "ppes_batch_start": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int],
+            "ppes_batch_step": [vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ip],
+            "ppes_batch_status": [vp, ctypes.c_int, ip, ctypes.POINTER(RunStats)],
+            "ppes_batch_memory": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)],
+            "ppes_batch_set_timing": [vp, ctypes.c_int],
+            "ppes_batch_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.ppes_instance_bytes.argtypes = [ctypes.POINTER(Config)]
+        L.ppes_instance_bytes.restype = ctypes.c_longlong
         L.ppes_status_string.argtypes = [ctypes.c_int]
         L.ppes_status_string.restype = ctypes.c_char_p
         _lib = L
@@ -117,18 +136,33 @@ def ldf_nodal_operator(p, hx, hy):
     return op
 
 
+def make_config(nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
+    cfg = Config()
+    cfg.nx, cfg.ny, cfg.px, cfg.py, cfg.rk_stages = nx, ny, p, p, rk
+    cfg.left, cfg.right, cfg.bottom, cfg.top = [float(v) for v in domain]
+    cfg.gamma = float(gamma)
+    cfg.bc_kind = BC[bc]
+    if bc_state is not None:
+        for v in range(8):
+            cfg.bc_state[v] = float(bc_state[v])
+    return cfg
+
+
+def instance_bytes(nx, ny, p, rk=3):
+    """Device bytes one instance of this size holds (host-side formula)."""
+    cfg = make_config(nx, ny, p, (0.0, 1.0, 0.0, 1.0), 1.4, rk=rk)
+    return lib().ppes_instance_bytes(ctypes.byref(cfg))
+
+
+def _stats(st):
+    return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
+
+
 class Solver:
     """One device-resident instance (init_DG_solver ... finalize_DG_solver)."""
 
     def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
-        cfg = Config()
-        cfg.nx, cfg.ny, cfg.px, cfg.py, cfg.rk_stages = nx, ny, p, p, rk
-        cfg.left, cfg.right, cfg.bottom, cfg.top = [float(v) for v in domain]
-        cfg.gamma = float(gamma)
-        cfg.bc_kind = BC[bc]
-        if bc_state is not None:
-            for v in range(8):
-                cfg.bc_state[v] = float(bc_state[v])
+        cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, rk)
         h = ctypes.c_void_p()
         _check(lib().ppes_create(ctypes.byref(cfg), ctypes.byref(h)), "ppes_create")
         self.h = h
@@ -231,3 +265,86 @@ class Solver:
 
     def sync(self):
         _check(lib().ppes_sync(self.h), "ppes_sync")
+
+
+class Batch:
+    """Independent instances of one RK method advanced together: one launch
+    per kernel class and degree covers every active instance."""
+
+    def __init__(self, rk=3):
+        h = ctypes.c_void_p()
+        _check(lib().ppes_batch_create(int(rk), ctypes.byref(h)), "ppes_batch_create")
+        self.h = h
+        self.rk = rk
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ppes_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None):
+        cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, self.rk)
+        i = ctypes.c_int()
+        _check(lib().ppes_batch_add(self.h, ctypes.byref(cfg), ctypes.byref(i)), "ppes_batch_add")
+        N = p + 1
+        self.shapes[i.value] = (8, N, N, nx + 2, ny + 2)
+        return i.value
+
+    def remove(self, i):
+        _check(lib().ppes_batch_remove(self.h, int(i)), "ppes_batch_remove")
+        self.shapes.pop(i, None)
+
+    def init_problem(self, i, prob, pert=None, amp=0.0):
+        if pert is None:
+            rc = lib().ppes_batch_init_problem(self.h, int(i), PROB[prob], None, 0, 0.0)
+        else:
+            pt = np.ascontiguousarray(pert, dtype=np.float64)
+            rc = lib().ppes_batch_init_problem(self.h, int(i), PROB[prob], _dp(pt), len(pt), float(amp))
+        _check(rc, "ppes_batch_init_problem")
+
+    def upload_state(self, i, slot, u):
+        a = np.asfortranarray(u, dtype=np.float64)
+        assert a.shape == self.shapes[i]
+        _check(lib().ppes_batch_upload_state(self.h, int(i), slot, _dp(a)), "ppes_batch_upload_state")
+
+    def download_state(self, i, slot=0):
+        a = np.zeros(self.shapes[i], order="F")
+        _check(lib().ppes_batch_download_state(self.h, int(i), slot, _dp(a)), "ppes_batch_download_state")
+        return a
+
+    def start(self, i, t_final=-1.0, max_steps=0, cfl=0.2, oedg_mode=1, use_ldf=True):
+        rc = lib().ppes_batch_start(self.h, int(i), float(t_final), int(max_steps), float(cfl), int(oedg_mode),
+                                    int(use_ldf))
+        _check(rc, "ppes_batch_start")
+
+    def step(self, nsteps=0, use_graph=True):
+        """Advance every active instance; returns (batch steps run, instances still active)."""
+        done, act = ctypes.c_longlong(), ctypes.c_int()
+        _check(lib().ppes_batch_step(self.h, int(nsteps), int(use_graph), ctypes.byref(done), ctypes.byref(act)),
+               "ppes_batch_step")
+        return done.value, act.value
+
+    def status(self, i):
+        stt, st = ctypes.c_int(), RunStats()
+        _check(lib().ppes_batch_status(self.h, int(i), ctypes.byref(stt), ctypes.byref(st)), "ppes_batch_status")
+        d = _stats(st)
+        d["state"] = INST_STATE[stt.value]
+        return d
+
+    def memory(self, i):
+        b = ctypes.c_longlong()
+        _check(lib().ppes_batch_memory(self.h, int(i), ctypes.byref(b)), "ppes_batch_memory")
+        return b.value
+
+    def set_timing(self, on):
+        _check(lib().ppes_batch_set_timing(self.h, int(on)), "ppes_batch_set_timing")
+
+    def kernel_time(self, kernel):
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().ppes_batch_kernel_time(self.h, KERNELS[kernel], ctypes.byref(ms), ctypes.byref(n)),
+               "ppes_batch_kernel_time")
+        return ms.value, n.value

@@ -1,0 +1,116 @@
+"""Batched instances (ppes_batch_*): every instance run inside a batch, with
+or without a captured HIP graph, and admitted mid-run, gives bit for bit the
+result of the same instance run alone through the single-solver API, and
+matches the CPU oracle within the trajectory tolerance (1e-10 normwise)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import ppesdg
+from helpers import interior, normwise, setup
+
+pytestmark = pytest.mark.gpu
+
+# (problem, nx, ny, p, run parameters): uneven sizes, degrees, BCs, OEDG
+# modes and end conditions
+MIX = [
+    ("ot2pi", 10, 9, 1, dict(max_steps=5, cfl=0.2, oedg_mode=1, use_ldf=True)),
+    ("rotor", 12, 12, 2, dict(max_steps=7, cfl=0.2, oedg_mode=1, use_ldf=True)),
+    ("blast", 10, 10, 3, dict(max_steps=4, cfl=0.15, oedg_mode=2, use_ldf=True)),
+    ("ot2pi", 8, 8, 4, dict(t_final=0.02, cfl=0.2, oedg_mode=1, use_ldf=True)),
+    ("rotor", 9, 11, 3, dict(max_steps=3, cfl=0.2, oedg_mode=0, use_ldf=True)),
+    ("ot2pi", 12, 10, 2, dict(max_steps=6, cfl=0.2, oedg_mode=1, use_ldf=False)),
+    ("blast", 7, 9, 1, dict(t_final=2e-4, cfl=0.15, oedg_mode=2, use_ldf=True)),
+]
+
+
+def run_alone(prob, nx, ny, p, kw):
+    s = setup(prob)
+    g = ppesdg.Solver(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    g.init_problem(prob)
+    st = g.solve(kw.get("t_final", -1.0), kw.get("max_steps", 0), kw["cfl"], kw["oedg_mode"], kw["use_ldf"])
+    return interior(g.download_state(0)), st
+
+
+def add_and_start(b, prob, nx, ny, p, kw):
+    s = setup(prob)
+    i = b.add(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    b.init_problem(i, prob)
+    b.start(i, kw.get("t_final", -1.0), kw.get("max_steps", 0), kw["cfl"], kw["oedg_mode"], kw["use_ldf"])
+    return i
+
+
+def drain(b, use_graph):
+    for _ in range(10000):
+        _, act = b.step(1, use_graph)
+        if act == 0:
+            return
+    raise AssertionError("batch did not finish")
+
+
+def assert_same(b, i, ref_u, ref_st):
+    st = b.status(i)
+    assert st["state"] == "done", st
+    assert st["steps"] == ref_st["steps"] and st["retries"] == ref_st["retries"]
+    assert st["t"] == ref_st["t"] and st["negcells"] == ref_st["negcells"]
+    assert st["tot_ent"] == ref_st["tot_ent"]
+    got = interior(b.download_state(i, 0))
+    assert np.array_equal(got, ref_u)
+
+
+@pytest.fixture(scope="module")
+def alone():
+    return [run_alone(*case) for case in MIX]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_equals_alone(alone, use_graph):
+    b = ppesdg.Batch(3)
+    ids = [add_and_start(b, *case) for case in MIX]
+    drain(b, use_graph)
+    for i, (u, st) in zip(ids, alone):
+        assert_same(b, i, u, st)
+
+
+def test_batch_admission_mid_run(alone):
+    """Continuous batching: instances join while others are running, and a
+    finished instance's slot is reused."""
+    b = ppesdg.Batch(3)
+    first = [add_and_start(b, *MIX[k]) for k in (0, 1, 2)]
+    b.step(2, True)
+    late = [add_and_start(b, *MIX[k]) for k in (3, 4)]
+    b.step(3, True)
+    drain(b, True)
+    for i, k in zip(first + late, (0, 1, 2, 3, 4)):
+        assert_same(b, i, *alone[k])
+    b.remove(first[0])
+    again = [add_and_start(b, *MIX[k]) for k in (5, 6)]
+    drain(b, True)
+    for i, k in zip(again, (5, 6)):
+        assert_same(b, i, *alone[k])
+
+
+def test_batch_matches_oracle_and_positivity():
+    b = ppesdg.Batch(3)
+    ids = [add_and_start(b, *case) for case in MIX]
+    drain(b, True)
+    for i, (prob, nx, ny, p, kw) in zip(ids, MIX):
+        s = setup(prob)
+        o = O.Oracle(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+        o.init_problem(prob)
+        so = o.run(kw.get("t_final", -1.0), kw.get("max_steps", 0), kw["cfl"], kw["oedg_mode"], kw["use_ldf"])
+        got = interior(b.download_state(i, 0))
+        err = normwise(got, interior(o.u))
+        print(f"batch {prob} {nx}x{ny} P{p}: normwise {err:.3e}, steps {b.status(i)['steps']}/{so['steps']}")
+        assert b.status(i)["steps"] == so["steps"]
+        assert err <= 1e-10
+        q = got
+        prs = (s["gamma"] - 1.0) * (q[7] - 0.5 * ((q[1] ** 2 + q[2] ** 2 + q[3] ** 2) / q[0] +
+                                                   (q[4] ** 2 + q[5] ** 2 + q[6] ** 2)))
+        assert q[0].min() > 0 and prs.min() > 0
+
+
+def test_batch_memory_accounting():
+    b = ppesdg.Batch(3)
+    i = b.add(16, 12, 3, (0.0, 1.0, 0.0, 1.0), 1.4)
+    assert b.memory(i) == ppesdg.instance_bytes(16, 12, 3)
