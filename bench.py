@@ -1,18 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the PP-ES nodal DG step for 2D ideal MHD on one MI355X.
 
-Workload (the GPU-scale single-instance configuration): Orszag-Tang on
-[0,2pi]^2 (gamma = 5/3) plus a seeded smooth perturbation (amplitude 1e-3 on
-rho and, through a vector potential, on B; integer modes 0 < |k| <= 4, phases
-from numpy default_rng(0)), P3 (4x4 GLL nodes), 1024 x 1024 elements, FP64,
-SSP-RK3 with CFL 0.2, LDF + positivity limiter every stage, OEDG on the final
-stage. A "step" is one SSP-RK3 step (three stages) of the whole instance.
+Headline workload (`value`): 64 uneven instances drawn with numpy
+default_rng(1) -- each one Orszag-Tang (2pi, periodic, CFL 0.2), rotor (fixed
+ambient ghosts, CFL 0.2) or blast wave (transmissive, CFL 0.15, OEDG every
+stage), grid side uniform in {32, 64, 128, 256}, degree uniform in P1-P4 --
+FP64, SSP-RK3, LDF + positivity limiter, each instance with its own dt from
+its own CFL condition. The batched path advances all of them together: one
+launch per kernel class and polynomial degree per RK stage, the whole step
+replayed from a captured hipGraph. A "step" is one RK step of every
+instance. The same instances are then run one at a time through the plain
+path (one instance per launch sequence, no graph) for the batched/plain ratio,
+and the final states of both paths are compared (matched L1 error).
 
-Prints ONE JSON line. value = DOF-stage updates per second, where a DOF is one
-GLL node of one element (8 FP64 variables) and a DOF-stage update is one node
-advanced through one RK stage (residual + combine + post-process).
+Also reported: the single-instance workload (Orszag-Tang 2pi + seeded
+perturbation, P3, 1024 x 1024, 200 RK steps), per-kernel device times (HIP
+events on the engine's stream), the roofline of the dominant kernel and a
+CPU baseline (the CPU oracle on a bounded sample of the mixed workload).
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu]
+value = DOF-stage updates per second: a DOF is one GLL node of one element
+(8 FP64 variables); a DOF-stage update is one node advanced through one RK
+stage (residual + combine + post-process).
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 3] [--no-cpu] [--no-single]
 """
 import argparse
 import json
@@ -25,6 +35,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pp-es-dg-mhd_amd"))
+
+import workloads as W  # noqa: E402
 
 METRIC = "DOF-stage updates/s, batched vs plain at matched L1 err; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
@@ -42,37 +54,156 @@ ALG_BYTES_PER_NODE = {"residual": 128.0, "stage": (192.0 + 256.0 + 256.0) / 3.0,
 STEP_BYTES_PER_NODE_STAGE = 640.0 / 3.0
 
 
-def perturbation_table(seed=0, kmax=4):
-    ks = [(kx, ky) for kx in range(-kmax, kmax + 1) for ky in range(-kmax, kmax + 1)
-          if 0 < kx * kx + ky * ky <= kmax * kmax]
-    ph = np.random.default_rng(seed).uniform(0.0, 2.0 * math.pi, size=(2, len(ks)))
-    return np.array([[kx, ky, ph[0, q], ph[1, q]] for q, (kx, ky) in enumerate(ks)])
+def add_instance(b, inst, max_steps):
+    dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[inst["prob"]]
+    i = b.add(inst["nx"], inst["ny"], inst["p"], dom, g, bc, bcs)
+    b.init_problem(i, inst["prob"])
+    b.start(i, -1.0, max_steps, cfl, oedg, True)
+    return i
 
 
-def cpu_baseline(p, cfl):
+def launches_per_step(kernel, mix):
+    """launches of one kernel class per batch step (one per degree group)"""
+    groups = len({m["p"] for m in mix})
+    oedg_stages = 3 if any(W.PROBLEMS[m["prob"]][5] == 2 for m in mix) else 1
+    return {"residual": 3, "stage": 3, "oedg": oedg_stages, "limit": oedg_stages, "bc": 3}.get(kernel, 0) * groups
+
+
+def run_batched(ppesdg, mix, steps, warmup, tsteps):
+    """Batched path: warmup steps, `steps` timed graph steps, then `tsteps`
+    steps without the graph with per-launch HIP events (kernel times)."""
+    b = ppesdg.Batch(3)
+    total = warmup + steps + tsteps
+    ids = [add_instance(b, m, total) for m in mix]
+    b.step(warmup, True)
+    t0 = time.perf_counter()
+    done, active = b.step(steps, True)
+    elapsed = time.perf_counter() - t0  # step() returns after the last step's status is on the host
+    if done != steps or active != len(mix):
+        print(f"# warning: {done} of {steps} steps, {active} of {len(mix)} instances active", file=sys.stderr)
+    b.set_timing(True)
+    b.step(tsteps, False)
+    kt = {k: b.kernel_time(k) for k in ppesdg.KERNELS}
+    b.set_timing(False)
+    stats = [b.status(i) for i in ids]
+    return b, ids, elapsed, kt, stats
+
+
+def run_plain(ppesdg, mix, steps, warmup, tsteps, batch, ids):
+    """Plain path: every instance alone (one launch sequence each, no graph);
+    the timed part is each instance's `steps` steps after its warmup. The
+    final state is compared with the batched run of the same instance."""
+    elapsed = 0.0
+    max_rel_l1 = 0.0
+    bitwise = True
+    for inst, i in zip(mix, ids):
+        dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[inst["prob"]]
+        s = ppesdg.Solver(inst["nx"], inst["ny"], inst["p"], dom, g, bc, bcs)
+        s.init_problem(inst["prob"])
+        s.solve_begin(True)
+        s.advance(-1.0, warmup, cfl, oedg)
+        s.sync()
+        t0 = time.perf_counter()
+        s.advance(-1.0, steps, cfl, oedg)
+        elapsed += time.perf_counter() - t0
+        s.advance(-1.0, tsteps, cfl, oedg)
+        a = s.download_state(0)[:, :, :, 1:-1, 1:-1]
+        s.close()
+        bb = batch.download_state(i, 0)[:, :, :, 1:-1, 1:-1]
+        bitwise = bitwise and bool(np.array_equal(a, bb))
+        max_rel_l1 = max(max_rel_l1, float(np.sum(np.abs(a[0] - bb[0])) / np.sum(np.abs(a[0]))))
+    return elapsed, max_rel_l1, bitwise
+
+
+def run_single(ppesdg, steps, warmup):
+    """Orszag-Tang 2pi + seeded perturbation, P3, 1024 x 1024, plain path."""
+    cfl, nx, p = 0.2, 1024, 3
+    nodes = nx * nx * (p + 1) ** 2
+    s = ppesdg.Solver(nx, nx, p, (0.0, 2 * math.pi, 0.0, 2 * math.pi), 5.0 / 3.0, "periodic")
+    s.init_problem("otpert", W.perturbation_table(0), 1e-3)
+    s.solve_begin(use_ldf=True)
+    if warmup:
+        s.advance(-1.0, warmup, cfl, 1)
+    s.sync()
+    before = s.advance(-1.0, 0, cfl, 1)
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    after = s.advance(-1.0, steps, cfl, 1)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    kt = {k: s.kernel_time(k) for k in ppesdg.KERNELS}
+    s.set_timing(False)
+    stages = after["stages"] - before["stages"]
+    u = s.download_state(0)[:, :, :, 1:-1, 1:-1]
+    g = 5.0 / 3.0
+    prs = (g - 1.0) * (u[7] - 0.5 * ((u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / u[0] + (u[4] ** 2 + u[5] ** 2 + u[6] ** 2)))
+    r = kt["residual"]
+    res = {"workload": f"Orszag-Tang 2pi + seeded perturbation (amplitude 1e-3, |k| <= 4, seed 0), P{p}, "
+                       f"{nx}x{nx} elements, SSP-RK3, CFL {cfl}, {steps} steps, plain path",
+           "value": nodes * stages / elapsed, "ms_per_step": elapsed / steps * 1e3, "steps": steps,
+           "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()},
+           "residual_avg_launch_ms": r[0] / max(r[1], 1),
+           "residual_hbm_frac": ALG_BYTES_PER_NODE["residual"] * nodes / (r[0] / max(r[1], 1) / 1e3) / 1e9
+           / HBM_PEAK_GBS,
+           "min_rho": float(u[0].min()), "min_p": float(prs.min()), "retries": after["retries"]}
+    s.close()
+    return res
+
+
+def cpu_baseline(mix, budget_s=15.0):
     """The CPU oracle (plain-C restatement, OpenMP over elements) on a bounded
-    sample of the same problem: 512 x 512 elements, start-up + 4 steps."""
+    sample of the mixed workload: instances in mix order, start-up + 2 steps
+    each, until about budget_s seconds of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     cores = min(16, os.cpu_count() or 1)
     O.set_threads(cores)
-    nx = 512
-    steps = 4
-    o = O.Oracle(nx, nx, p, (0.0, 2 * math.pi, 0.0, 2 * math.pi), 5.0 / 3.0, "periodic")
-    o.init_problem("otpert", perturbation_table(0), 1e-3)
-    t0 = time.perf_counter()
-    s = o.run(-1.0, steps, cfl, 1, True)
-    dt = time.perf_counter() - t0
-    stage_equiv = 3 * s["steps"] + 1  # the start-up post-process + residual counts as one stage
-    value = nx * nx * (p + 1) ** 2 * stage_equiv / dt
-    return {"value": value, "unit": "DOF-stage updates/s", "cores": cores, "kind": "port",
-            "sample": f"oracle/ppes_oracle.c, OT-2pi + perturbation P{p} {nx}x{nx}, start-up + {steps} RK3 steps "
-                      f"({stage_equiv} stage-equivalents) in {dt:.1f} s"}
+    work, spent, used = 0.0, 0.0, 0
+    for inst in mix:
+        if spent >= budget_s:
+            break
+        dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[inst["prob"]]
+        o = O.Oracle(inst["nx"], inst["ny"], inst["p"], dom, g, bc, bcs)
+        o.init_problem(inst["prob"])
+        t0 = time.perf_counter()
+        s = o.run(-1.0, 2, cfl, oedg, True)
+        spent += time.perf_counter() - t0
+        work += W.nodes(inst) * (3 * s["steps"] + 1)  # the start-up counts as one stage
+        used += 1
+        del o
+    return {"value": work / spent, "unit": "DOF-stage updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle/ppes_oracle.c on the first {used} instances of the mixed workload, "
+                      f"start-up + 2 RK3 steps each, {spent:.1f} s"}
+
+
+def run_serving(duration, rate):
+    """Scaled serving stream: Poisson arrivals (seed 2) of requests from the
+    mixed workload's problems with sides {32, 64, 128}, P1-P4 and t_final a
+    uniform fraction in [0.1, 1] of a per-problem horizon (OT 0.05, rotor
+    0.01, blast 0.001); continuous batching under a 200 GB budget against
+    the plain sequential path on the same stream."""
+    import serve
+    reqs = W.serving_stream(rate=rate, duration=duration, seed=2, sides=(32, 64, 128),
+                            horizon={"ot2pi": 0.05, "rotor": 0.01, "blast": 0.001})
+    done, wall, rejected = serve.run_stream(reqs, 200e9, use_graph=True)
+    lat = [r["t_done"] - r["t_submit"] for r in done]
+    total_work = sum(serve.work(r) for r in done)
+    seq, busy = serve.run_sequential(reqs)
+    lat_seq = [r["t_done"] - r["arrival"] for r in seq]
+    work_seq = sum(serve.work(r) for r in seq)
+    out = {"requests": len(reqs), "finished": len(done), "rejected": len(rejected),
+           "stream": f"Poisson {rate}/s for {duration} s (seed 2), sides 32-128, P1-P4, t_final in [0.1, 1] x "
+                     f"(OT 0.05, rotor 0.01, blast 0.001)",
+           "batched": {"value": total_work / wall, "wall_s": wall, **serve.percentiles(lat)},
+           "plain_sequential": {"value": work_seq / busy, "busy_s": busy, **serve.percentiles(lat_seq)},
+           "unit": "DOF-stage updates/s; latency (arrival to result) in s"}
+    out["batched_over_plain"] = out["batched"]["value"] / out["plain_sequential"]["value"]
+    return out
 
 
 def load_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_p3_1024_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_mixed64_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -82,11 +213,17 @@ def load_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--nx", type=int, default=1024)
-    ap.add_argument("--p", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--timing-steps", type=int, default=10)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--single-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-duration", type=float, default=5.0)
+    ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -107,42 +244,22 @@ def main():
 
     import ppesdg  # raises if the HIP library is missing: there is no fallback
 
-    cfl = 0.2
-    nx = ny = a.nx
-    p = a.p
-    nodes = nx * ny * (p + 1) ** 2
-    s = ppesdg.Solver(nx, ny, p, (0.0, 2 * math.pi, 0.0, 2 * math.pi), 5.0 / 3.0, "periodic")
-    s.init_problem("otpert", perturbation_table(0), 1e-3)
-    s.solve_begin(use_ldf=True)
-    if a.warmup:
-        s.advance(-1.0, a.warmup, cfl, 1)
-    s.sync()
-    before = s.advance(-1.0, 0, cfl, 1)  # no-op: reads the cumulative counters
-    s.set_timing(True)
-    t0 = time.perf_counter()
-    after = s.advance(-1.0, a.steps, cfl, 1)
-    s.sync()
-    elapsed = time.perf_counter() - t0
-    stages = after["stages"] - before["stages"]
-    value = nodes * stages / elapsed
+    mix = W.mixed_batch(a.instances, seed=1)
+    nodes_total = sum(W.nodes(m) for m in mix)
+    print(f"# batched: {len(mix)} instances, {nodes_total} nodes", file=sys.stderr, flush=True)
+    batch, ids, elapsed, kt, stats = run_batched(ppesdg, mix, a.steps, a.warmup, a.timing_steps)
+    value = nodes_total * 3 * a.steps / elapsed
+    retries = sum(s["retries"] for s in stats)
+    failed = [s for s in stats if s["status"] != 0]
 
-    # per-kernel device time over the timed region (HIP events on the solver's stream)
-    kt = {k: s.kernel_time(k) for k in ppesdg.KERNELS}
-    s.set_timing(False)
+    # roofline of the dominant kernel class over the timing steps
     dominant = max((k for k in kt if kt[k][1] > 0), key=lambda k: kt[k][0])
     tot_ms, launches = kt[dominant]
+    per_step_launches = launches_per_step(dominant, mix)
+    alg_bytes_step = ALG_BYTES_PER_NODE[dominant] * nodes_total * (3 if dominant in ("residual", "stage") else 1)
+    bytes_per_launch = alg_bytes_step / per_step_launches
     avg_s = tot_ms / launches / 1e3
-    bytes_per_launch = ALG_BYTES_PER_NODE[dominant] * nodes
     achieved = bytes_per_launch / avg_s / 1e9
-    traffic = load_traffic(dominant)
-
-    # sanity of the result: exact positivity and mass conservation (periodic)
-    u = s.download_state(0)[:, :, :, 1:-1, 1:-1]
-    g = 5.0 / 3.0
-    prs = (g - 1.0) * (u[7] - 0.5 * ((u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / u[0] + (u[4] ** 2 + u[5] ** 2 + u[6] ** 2)))
-    checks = {"min_rho": float(u[0].min()), "min_p": float(prs.min()), "retries": after["retries"],
-              "pp_limited_cells": after["negcells"]}
-    del u, prs
 
     out = {
         "metric": METRIC,
@@ -157,20 +274,37 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"Orszag-Tang 2pi + seeded perturbation, P{p}, {nx}x{ny} elements, SSP-RK3, "
-                               f"CFL {cfl}, LDF+PP every stage, OEDG final stage, plain single-instance path",
-                   "nodes": nodes, "dof_per_node": 8, "path": "plain"},
+        "config": {"workload": f"{len(mix)} uneven instances (numpy default_rng(1): Orszag-Tang 2pi / rotor / blast, "
+                               f"sides 32-256, P1-P4), SSP-RK3, LDF + PP every stage, OEDG final stage (blast: every "
+                               f"stage), per-instance CFL dt; batched path, hipGraph step",
+                   "instances": len(mix), "nodes_total": nodes_total, "dof_per_node": 8, "path": "batched",
+                   "retries": retries, "failed": len(failed)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": bytes_per_launch},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(dominant),
+                     "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": bytes_per_launch,
+                     "timing_steps": a.timing_steps},
         "step_roofline": {"alg_bytes_per_node_stage": STEP_BYTES_PER_NODE_STAGE,
                           "achieved_GBs": STEP_BYTES_PER_NODE_STAGE * value / 1e9,
                           "frac_hbm": STEP_BYTES_PER_NODE_STAGE * value / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms_per_step": {k: v[0] / a.steps for k, v in kt.items()},
-        "checks": checks,
+        "kernel_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()},
     }
+    print(f"# batched {value:.4e} DOF-stage/s", file=sys.stderr, flush=True)
+    if not a.no_plain:
+        p_elapsed, max_l1, bitwise = run_plain(ppesdg, mix, a.steps, a.warmup, a.timing_steps, batch, ids)
+        pv = nodes_total * 3 * a.steps / p_elapsed
+        out["plain"] = {"value": pv, "ms_per_step": p_elapsed / a.steps * 1e3,
+                        "batched_over_plain": value / pv}
+        out["matched_l1"] = {"max_rel_l1_rho_batched_vs_plain": max_l1, "bitwise_equal": bitwise}
+        print(f"# plain {pv:.4e} DOF-stage/s", file=sys.stderr, flush=True)
+    batch.close()
+    if not a.no_single:
+        out["single_instance"] = run_single(ppesdg, a.single_steps, 2)
+        print("# single instance done", file=sys.stderr, flush=True)
+    if not a.no_serve:
+        out["serving"] = run_serving(a.serve_duration, a.serve_rate)
+        print("# serving done", file=sys.stderr, flush=True)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, cfl)
+        out["cpu_baseline"] = cpu_baseline(mix)
     print(json.dumps(out))
 
 

@@ -125,7 +125,7 @@ program ref_harness
 	implicit none
 
 	character(len=64) :: mode = 'stages', prob = 'ot2pi', bc = 'auto'
-	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000
+	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3
 	real :: cfl = 0.2, tfinal = -1.0, oedg_dt_call = 0.01
 	real :: left, right, bottom, top, gam
 	logical :: have_init_file
@@ -140,7 +140,8 @@ program ref_harness
 	end if
 
 	call setup_problem()
-	call init_DG_solver(nx_, ny_, p_, p_, 3, left, right, bottom, top)
+	if (rk_ /= 3 .and. trim(mode) /= 'solve') error stop 'harness: rk /= 3 needs mode=solve'
+	call init_DG_solver(nx_, ny_, p_, p_, rk_, left, right, bottom, top)
 	call dump_real('xx', xx, [p_+1, nx_])
 	call dump_real('yy', yy, [p_+1, ny_])
 
@@ -181,6 +182,7 @@ contains
 			case ('nx'); read(val, *) nx_
 			case ('ny'); read(val, *) ny_
 			case ('p'); read(val, *) p_
+			case ('rk'); read(val, *) rk_
 			case ('nsteps'); read(val, *) nsteps
 			case ('dumpsteps'); read(val, *) dumpsteps
 			case ('oedg'); read(val, *) oedg

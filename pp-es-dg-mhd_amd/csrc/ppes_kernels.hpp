@@ -284,21 +284,39 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
 }
 
 // --------------------------------------------------------------------------
+// Minimum over the N^2 lanes of each element, returned to every lane of the
+// element. When N^2 divides the wavefront (P1, P3) the element's lanes are one
+// aligned group of a wavefront and a butterfly of lane exchanges does it with
+// no barrier; otherwise through LDS (called by every thread of the workgroup).
+// min is exact, so both give the value of the reference's loop.
+template <int P>
+__device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, int el, bool valid)
+{
+    constexpr int NN = (P + 1) * (P + 1);
+    if constexpr (64 % NN == 0) {
+#pragma unroll
+        for (int m = 1; m < NN; m <<= 1) x = dmin(x, __shfl_xor(x, m));
+        return x;
+    } else {
+        sred[t][0] = x;
+        __syncthreads();
+        double r = DBL_HUGE;
+        if (valid)
+            for (int m = 0; m < NN; ++m) r = dmin(r, sred[el * NN + m][0]);
+        __syncthreads();
+        return r;
+    }
+}
+
 // Zhang-Shu positivity scaling of one element toward its mean
-// (DG_solver.f90:528-603). Called by every thread of the workgroup (it
-// synchronises); lanes with valid = false pass neutral values.
+// (DG_solver.f90:528-603). Called by every thread of the workgroup (it may
+// synchronise); lanes with valid = false pass neutral values.
 template <int P>
 __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8], bool valid, double g,
                                                  double (*sred)[2], int t, int el, bool &neg_node, bool &fail)
 {
-    constexpr int NN = (P + 1) * (P + 1);
     // density, once
-    sred[t][0] = valid ? q[0] : DBL_HUGE;
-    __syncthreads();
-    double den_min = DBL_HUGE;
-    if (valid)
-        for (int m = 0; m < NN; ++m) den_min = dmin(den_min, sred[el * NN + m][0]);
-    __syncthreads();
+    const double den_min = elem_min<P>(valid ? q[0] : DBL_HUGE, sred, t, el, valid);
     neg_node = false;
     fail = false;
     if (valid && den_min < PP_EPS) {
@@ -307,18 +325,16 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
         q[0] = avg[0] + theta * (q[0] - avg[0]);
         q[0] = dmax(q[0], PP_EPS);
     }
-    // pressure: one check, then at most two shrinks by 0.6; element-uniform
-    // predicates, and every round reaches the barriers
+    // pressure: one check, then at most two shrinks by 0.6 and a last check;
+    // the later rounds run only while some element of the workgroup is
+    // still being limited (a workgroup-uniform decision)
     const double prs_avg = valid ? prs_macro(avg[0], avg[1], avg[2], avg[3], avg[4], avg[5], avg[6], avg[7], g) : 0.0;
     double theta_p = 0.0;
     bool active = false;
     for (int round = 0; round < 4; ++round) {
-        sred[t][0] = valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE;
-        __syncthreads();
-        double prs_min = DBL_HUGE;
-        if (valid)
-            for (int m = 0; m < NN; ++m) prs_min = dmin(prs_min, sred[el * NN + m][0]);
-        __syncthreads();
+        if (round > 0 && !__syncthreads_or(active ? 1 : 0)) break;
+        const double prs_min =
+            elem_min<P>(valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE, sred, t, el, valid);
         if (!valid) continue;
         if (round == 0) {
             if (prs_min < PP_EPS) {
@@ -395,6 +411,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     __shared__ int scnt[EPB];
     __shared__ double sred[BLOCK][2];
     __shared__ double sent[EPB];
+    __shared__ double sem[3][EPB][8];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -504,27 +521,37 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
         if (valid && ok)
 #pragma unroll
             for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
-        // per-workgroup (sum, min, max) over interior elements, fixed order;
-        // scons is reused for the post-LDF node values
+        // per-workgroup (sum, min, max) over interior elements: per element
+        // and variable by the element's own lanes, then across the
+        // workgroup's elements in a fixed order
         __syncthreads();
         if (valid)
 #pragma unroll
             for (int v = 0; v < 8; ++v) scons[el][v][n] = q[v];
+        if (el < EPB && n == 0) scnt[el] = (interior && ok) ? 1 : 0;
+        __syncthreads();
+        if (interior && ok) {
+            for (int v = n; v < 8; v += NN) {
+                double mn = DBL_HUGE, mxv = -DBL_HUGE;
+                for (int m = 0; m < NN; ++m) {
+                    const double x = scons[el][v][m];
+                    mn = dmin(mn, x);
+                    mxv = dmax(mxv, x);
+                }
+                sem[0][el][v] = (I.hx * I.hy) * savg[el][v];
+                sem[1][el][v] = mn;
+                sem[2][el][v] = mxv;
+            }
+        }
         __syncthreads();
         if (t < 8) {
             const int v = t;
             double s = 0.0, mn = DBL_HUGE, mxv = -DBL_HUGE;
             for (int e = 0; e < EPB; ++e) {
-                const long long egg = (long long)bm.local * EPB + e;
-                if (egg >= nslots || !sok[e]) continue;
-                const int exx = (int)(egg % (I.nx + 2)), eyy = (int)(egg / (I.nx + 2));
-                if (!(exx >= 1 && exx <= I.nx && eyy >= 1 && eyy <= I.ny)) continue;
-                s = s + (I.hx * I.hy) * savg[e][v];
-                for (int m = 0; m < NN; ++m) {
-                    const double x = scons[e][v][m];
-                    mn = dmin(mn, x);
-                    mxv = dmax(mxv, x);
-                }
+                if (!scnt[e]) continue;
+                s = s + sem[0][e][v];
+                mn = dmin(mn, sem[1][e][v]);
+                mxv = dmax(mxv, sem[2][e][v]);
             }
             I.part[(long long)bm.local * 24 + v] = s;
             I.part[(long long)bm.local * 24 + 8 + v] = mn;

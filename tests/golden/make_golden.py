@@ -76,6 +76,10 @@ END_CASES = {
     # Orszag-Tang on [0,2pi]^2, P1 and P4 on small grids to t = 0.1
     "end_ot2pi_p1_32": dict(mode="solve", prob="ot2pi", nx=32, ny=32, p=1, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
     "end_ot2pi_p4_12": dict(mode="solve", prob="ot2pi", nx=12, ny=12, p=4, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
+    # the other RK methods of solve_execute (drive.f90:132-174,213-272): RK1, RK2, RK5 (CFL_RK 1.508)
+    "end_ot2pi_p2_16_rk1": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.1, rk=1),
+    "end_ot2pi_p2_16_rk2": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.2, rk=2),
+    "end_rotor_p3_16_rk5": dict(mode="solve", prob="rotor", nx=16, ny=16, p=3, tfinal=0.005, oedg=1, ldf=1, cfl=0.2, rk=5),
 }
 
 LONG_CASES = {

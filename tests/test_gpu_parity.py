@@ -172,3 +172,44 @@ def test_mass_conservation_periodic():
     g.solve(-1.0, 10, 0.2, 1, True)
     m1 = float(np.einsum("kl,klij->", wt, interior(g.download_state(0))[0]))
     assert abs(m1 - m0) <= 1e-13 * abs(m0)
+
+
+@pytest.mark.parametrize("rk,cfl", [(1, 0.1), (2, 0.2), (5, 0.2)])
+@pytest.mark.parametrize("prob", ["ot2pi", "rotor"])
+def test_solve_other_rk_methods_match_oracle(prob, rk, cfl):
+    """RK1, RK2 and RK5 stage combines, t_RK and CFL_RK (drive.f90:57-63,132-174,213-272)."""
+    nx, ny, p = 9, 8, 2
+    o, g = make_pair(prob, nx, ny, p, rk=rk)
+    o.init_problem(prob)
+    g.init_problem(prob)
+    s_ref = o.run(-1.0, 4, cfl, 1, True)
+    s = g.solve(-1.0, 4, cfl, 1, True)
+    assert s["steps"] == s_ref["steps"] and s["retries"] == s_ref["retries"]
+    assert normwise(interior(g.download_state(0)), interior(o.u)) <= 1e-10
+    assert abs(s["t"] - s_ref["t"]) <= 1e-12 * s_ref["t"]
+
+
+GOLDEN_END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12",
+              "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5", "end_ot2pi_p2_128"]
+
+
+@pytest.mark.parametrize("name", GOLDEN_END)
+def test_solve_matches_reference_fixture(name):
+    """The HIP path against the reference solver's own output at t_final
+    (fixtures written by the flang build of the reference). end_ot2pi_p2_128
+    is the CPU-runnable configuration: OT 2pi, P2, 128x128, CFL 0.2, t = 0.5."""
+    import json
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".npz"))
+    a = json.loads(str(z["args_json"]))
+    s = setup(a["prob"])
+    g = ppesdg.Solver(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"], rk=a.get("rk", 3))
+    g.init_problem(a["prob"])
+    st = g.solve(a["tfinal"], 0, a["cfl"], a["oedg"], a["ldf"])
+    ref_steps = int(z["steps"]) if "steps" in z.files else int(z["summary"][1])
+    rho = interior(g.download_state(0))[0]
+    err = rel_l1(rho, z["rho_final"])
+    print(f"{name}: steps {st['steps']}/{ref_steps}, rel L1(rho) {err:.3e}")
+    assert st["status"] == 0 and st["steps"] == ref_steps
+    assert err <= 1e-10
+    assert rho.min() > 0
