@@ -118,6 +118,7 @@ program ref_harness
 	use MHD_PDE_m
 	use DG_solver_m
 	use drive_m
+	use io, only: save_tecbin_finite_volume
 	use problems
 	use harness_ics
 	use harness_io
@@ -125,7 +126,7 @@ program ref_harness
 	implicit none
 
 	character(len=64) :: mode = 'stages', prob = 'ot2pi', bc = 'auto'
-	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3
+	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3, plt = 0
 	real :: cfl = 0.2, tfinal = -1.0, oedg_dt_call = 0.01
 	real :: left, right, bottom, top, gam
 	logical :: have_init_file
@@ -143,6 +144,8 @@ program ref_harness
 	if (rk_ /= 3 .and. trim(mode) /= 'solve') error stop 'harness: rk /= 3 needs mode=solve'
 	call init_DG_solver(nx_, ny_, p_, p_, rk_, left, right, bottom, top)
 	call dump_real('xx', xx, [p_+1, nx_])
+	call dump_real('grid_x', grid_x, [nx_+1])
+	call dump_real('grid_y', grid_y, [ny_+1])
 	call dump_real('yy', yy, [p_+1, ny_])
 
 	select case (trim(mode))
@@ -153,6 +156,9 @@ program ref_harness
 		call solve_execute(tfinal, cfl, oedg >= 1, ldf == 1, .false.)
 		call dump_real('u_final', u_con, [8, p_+1, p_+1, nx_+2, ny_+2])
 		call dump_real('avg_final', u_avg, [8, nx_+2, ny_+2])
+		! the reference's Tecplot writer on the final averages (main.f90:67); its
+		! solve_execute has written entropy.plt (drive.f90:338-341)
+		if (plt == 1) call save_tecbin_finite_volume(u_avg, nx_, ny_, grid_x, grid_y, 0, tfinal)
 	case ('calls')
 		call run_calls()
 	case default
@@ -183,6 +189,7 @@ contains
 			case ('ny'); read(val, *) ny_
 			case ('p'); read(val, *) p_
 			case ('rk'); read(val, *) rk_
+			case ('plt'); read(val, *) plt
 			case ('nsteps'); read(val, *) nsteps
 			case ('dumpsteps'); read(val, *) dumpsteps
 			case ('oedg'); read(val, *) oedg

@@ -105,6 +105,7 @@ Engine::~Engine()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto &g : groups) free_group(g);
+    for (auto &g : api_groups) if (g.d_el) free_group(g);
     if (d_all) (void)hipFree(d_all);
     for (size_t i = 0; i < hosts.size(); ++i)
         if (hosts[i]) remove((int)i);
@@ -139,6 +140,7 @@ int Engine::ensure_capacity(int n)
     d_recs = nd;
     cap = nc;
     invalidate_plan();
+    graph_ok = false;  // captured kernels hold the old record address
     return PPES_OK;
 }
 
@@ -273,7 +275,9 @@ int Engine::remove(int id)
         memset(&h_recs[id], 0, sizeof(Inst));
         if (d_recs) (void)hipMemcpy(&d_recs[id], &h_recs[id], sizeof(Inst), hipMemcpyHostToDevice);
     }
+    if (id < (int)api_groups.size() && api_groups[id].d_el) free_group(api_groups[id]);
     invalidate_plan();
+    built_key.clear();  // the cached plan may hold this id
     return PPES_OK;
 }
 
@@ -331,9 +335,29 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
     return PPES_OK;
 }
 
+// The plan (block maps per degree, the instance list and the stages that
+// apply OEDG) depends only on which instances are active and their OEDG
+// modes. It is kept while that set is unchanged, and also while no instance
+// is active, so that a solver advanced a few steps at a time (or a batch
+// whose membership returns to a previous set) reuses both the plan and the
+// captured graph.
 int Engine::build_plan()
 {
     if (plan_ok) return PPES_OK;
+    std::vector<long long> key;
+    for (size_t i = 0; i < hosts.size(); ++i)
+        if (hosts[i] && h_recs[i].ctl.state == ST_ACTIVE) key.push_back((long long)i * 16 + h_recs[i].ctl.oedg_mode);
+    if (key.empty()) {
+        n_all = 0;
+        plan_ok = true;
+        return PPES_OK;
+    }
+    if (key == built_key) {
+        n_all = built_n_all;
+        plan_ok = true;
+        return PPES_OK;
+    }
+    graph_ok = false;
     for (auto &g : groups) free_group(g);
     groups.clear();
     if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
@@ -358,8 +382,9 @@ int Engine::build_plan()
     n_all = (int)all.size();
     ENG_TRY(hipMalloc((void **)&d_all, std::max<size_t>(all.size(), 1) * sizeof(int)));
     if (!all.empty()) ENG_TRY(hipMemcpy(d_all, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+    built_key = key;
+    built_n_all = n_all;
     plan_ok = true;
-    graph_ok = false;
     return PPES_OK;
 }
 
@@ -536,13 +561,17 @@ int Engine::launch_startup(const std::vector<int> &ids)
     return rc;
 }
 
-// single reference-API calls on one instance
+// single reference-API calls on one instance (its one-instance block maps
+// are built on first use and kept until the instance is removed)
 int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
 {
-    std::vector<int> ids{id};
-    Group g;
-    int rc = build_group(g, ids, hosts[id]->p);
-    if (rc) { free_group(g); return rc; }
+    if ((int)api_groups.size() < (int)hosts.size()) api_groups.resize(hosts.size());
+    Group &g = api_groups[id];
+    if (!g.d_el) {
+        int rc = build_group(g, std::vector<int>{id}, hosts[id]->p);
+        if (rc) { free_group(g); return rc; }
+    }
+    int rc = PPES_OK;
     int *d_ids = g.d_list;
     switch (what) {
     case API_RESET:
@@ -575,8 +604,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     default:
         rc = PPES_BAD_ARG;
     }
-    (void)hipStreamSynchronize(stream);
-    free_group(g);
+    ENG_TRY(hipStreamSynchronize(stream));
     return rc;
 }
 

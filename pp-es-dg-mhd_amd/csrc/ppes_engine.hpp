@@ -64,7 +64,7 @@ public:
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
 
     int active_count() const;
-    void invalidate_plan() { plan_ok = false; graph_ok = false; }
+    void invalidate_plan() { plan_ok = false; }  // re-check the active set before the next launch
 
     // per-kernel-class timers (HIP events on the engine's stream)
     bool timing = false;
@@ -87,6 +87,9 @@ private:
     int *d_all = nullptr;
     int n_all = 0;
     std::vector<Group> groups;
+    std::vector<Group> api_groups;     // one-instance maps of the reference-API calls
+    std::vector<long long> built_key;  // active set of the built plan
+    int built_n_all = 0;
     bool plan_ok = false;
     bool any_oedg_stage[MAXS + 1] = {};
     hipGraph_t graph = nullptr;

@@ -1,0 +1,136 @@
+"""Summaries of rocprofv3 CSV output for the kernels of this library.
+
+    python pp-es-dg-mhd_amd/profsum.py stats DIR [--title T] [--command C]
+        per-kernel table of DIR/*kernel_stats.csv (--kernel-trace --stats)
+    python pp-es-dg-mhd_amd/profsum.py traffic --fetch D1 --write D2 --copy-fetch D3 --copy-write D4
+                                              --kernel residual [--command C]
+        HBM bytes per launch of one kernel class in the mixed 64-instance
+        workload, from FETCH_SIZE / WRITE_SIZE (--pmc, KiB per dispatch), each
+        scaled by what the same counter reads for a plain 1 GiB device copy
+        (bench.py --calibrate-copy); prints a JSON record and a Markdown table
+
+Launches in which every workgroup exits at once (the retry residual of a step
+with no retry) are left out: their counters are under 2% of the largest
+launch of the same kernel and grid.
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+CLASS_OF = {"k_residual": "residual", "k_stage_pp": "stage", "k_oedg_faces": "oedg", "k_oedg_limit": "limit",
+            "k_reduce_global": "reduce", "k_bc": "bc"}
+ALG_BYTES_PER_NODE = {"residual": 128.0, "stage": (192.0 + 256.0 + 256.0) / 3.0, "oedg": 64.0, "limit": 128.0}
+LAUNCHES_PER_STEP = {"residual": 3, "stage": 3}
+
+
+def one_csv(d, pattern):
+    files = glob.glob(os.path.join(d, "**", pattern), recursive=True)
+    if not files:
+        raise SystemExit(f"no {pattern} under {d}")
+    return files[0]
+
+
+def stats(d, title, command):
+    rows = list(csv.DictReader(open(one_csv(d, "*kernel_stats.csv"))))
+    out = [f"# {title}", "", f"Command: `{command}`", "",
+           "| kernel | calls | avg (us) | min (us) | max (us) | total (ms) | % |", "|---|---|---|---|---|---|---|"]
+    for r in rows:
+        name = r["Name"].split("(")[0].replace("void ", "")
+        out.append(f"| {name} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                   f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                   f"{float(r['Percentage']):.1f} |")
+    # per-class totals (all degrees): average launch duration as bench.py reports it
+    cls = collections.defaultdict(lambda: [0.0, 0])
+    for r in rows:
+        base = r["Name"].split("<")[0].split("(")[0].replace("void ppes::", "").replace("ppes::", "")
+        if base in CLASS_OF:
+            cls[CLASS_OF[base]][0] += float(r["TotalDurationNs"])
+            cls[CLASS_OF[base]][1] += int(r["Calls"])
+    out += ["", "| class | launches | total (ms) | avg per launch (us) |", "|---|---|---|---|"]
+    for k, (t, n) in cls.items():
+        out.append(f"| {k} | {n} | {t / 1e6:.2f} | {t / n / 1e3:.1f} |")
+    return "\n".join(out) + "\n"
+
+
+def load_counter(d):
+    rows = list(csv.DictReader(open(one_csv(d, "*counter_collection.csv"))))
+    per = collections.defaultdict(list)
+    for r in rows:
+        per[(r["Kernel_Name"].split("(")[0].replace("void ", ""), int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
+    return per
+
+
+def copy_factor(d):
+    per = load_counter(d)
+    vals = [v for (name, _), vs in per.items() if "copyBuffer" in name for v in vs]
+    return (1 << 30) / (max(vals) * 1024.0)
+
+
+def traffic(a):
+    import workloads as W
+    mix = W.mixed_batch(64, seed=1)
+    nodes_p = collections.Counter()
+    for m in mix:
+        nodes_p[m["p"]] += W.nodes(m)
+    ff, fw = copy_factor(a.copy_fetch), copy_factor(a.copy_write)
+    F, Wr = load_counter(a.fetch), load_counter(a.write)
+    kname = {v: k for k, v in CLASS_OF.items()}[a.kernel]
+    per_degree = {}
+    step_read = step_write = 0.0
+    for p in (1, 2, 3, 4):
+        name = f"ppes::{kname}<{p}>"
+        keys = [k for k in F if k[0] == name]
+        if not keys:
+            continue
+        g = max(k[1] for k in keys)
+        f, w = F[(name, g)], Wr[(name, g)]
+        fmax, wmax = max(f), max(w)
+        f = [x for x in f if x > 0.02 * fmax]
+        w = [x for x in w if x > 0.02 * wmax]
+        rb = sum(f) / len(f) * 1024.0 * ff
+        wb = sum(w) / len(w) * 1024.0 * fw
+        alg = ALG_BYTES_PER_NODE[a.kernel] * nodes_p[p]
+        per_degree[p] = {"read_bytes": rb, "write_bytes": wb, "alg_bytes": alg, "traffic_over_alg": (rb + wb) / alg,
+                         "launches_counted": len(f)}
+        step_read += rb
+        step_write += wb
+    groups = len(per_degree)
+    bytes_per_launch = (step_read + step_write) / groups
+    rec = {"kernel": a.kernel, "workload": "mixed 64 instances (numpy default_rng(1)), one launch per degree group",
+           "bytes_per_launch": bytes_per_launch, "read_bytes_per_launch": step_read / groups,
+           "write_bytes_per_launch": step_write / groups,
+           "alg_bytes_per_launch": sum(v["alg_bytes"] for v in per_degree.values()) / groups,
+           "copy_calibration": {"fetch_factor": ff, "write_factor": fw,
+                                "note": "bytes = counter KiB x 1024 x factor; factor = 1 GiB / counter of a 1 GiB copy"},
+           "per_degree": per_degree, "command": a.command}
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--title", default="rocprofv3 kernel statistics")
+    s.add_argument("--command", default="")
+    t = sub.add_parser("traffic")
+    for k in ("fetch", "write", "copy_fetch", "copy_write"):
+        t.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
+    t.add_argument("--kernel", default="residual")
+    t.add_argument("--command", default="")
+    a = ap.parse_args()
+    if a.cmd == "stats":
+        sys.stdout.write(stats(a.dir, a.title, a.command))
+    else:
+        print(json.dumps(traffic(a), indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -38,11 +38,17 @@ def load_dump(d):
 def run(args, keep=None):
     with tempfile.TemporaryDirectory() as d:
         cmd = [HARNESS] + [f"{k}={v}" for k, v in args.items()] + [f"out={d}"]
-        res = subprocess.run(cmd, capture_output=True, text=True)
+        # the reference writes its Tecplot files (entropy.plt, solution_0.plt)
+        # into the working directory
+        res = subprocess.run(cmd, capture_output=True, text=True, cwd=d)
         if res.returncode != 0:
             raise RuntimeError(f"{cmd} failed:\n{res.stdout}\n{res.stderr}")
         data = load_dump(d)
         data["_stdout"] = res.stdout
+        for f in ("solution_0", "entropy"):
+            path = os.path.join(d, f + ".plt")
+            if args.get("plt") and os.path.exists(path):
+                data["plt_" + f] = np.fromfile(path, dtype=np.uint8)
     if keep is not None:
         data = {k: v for k, v in data.items() if any(k == p or k.startswith(p) for p in keep) or k == "_stdout"}
     return data
@@ -74,7 +80,8 @@ END_CASES = {
     # reference solve_execute itself (final-stage OEDG), rotor P2 32x32 to t = 0.01
     "end_rotor_p2_32": dict(mode="solve", prob="rotor", nx=32, ny=32, p=2, tfinal=0.01, oedg=1, ldf=1, cfl=0.2),
     # Orszag-Tang on [0,2pi]^2, P1 and P4 on small grids to t = 0.1
-    "end_ot2pi_p1_32": dict(mode="solve", prob="ot2pi", nx=32, ny=32, p=1, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
+    # (also keeps the reference's Tecplot output of this run: solution_0.plt, entropy.plt)
+    "end_ot2pi_p1_32": dict(mode="solve", prob="ot2pi", nx=32, ny=32, p=1, tfinal=0.1, oedg=1, ldf=1, cfl=0.2, plt=1),
     "end_ot2pi_p4_12": dict(mode="solve", prob="ot2pi", nx=12, ny=12, p=4, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
     # the other RK methods of solve_execute (drive.f90:132-174,213-272): RK1, RK2, RK5 (CFL_RK 1.508)
     "end_ot2pi_p2_16_rk1": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.1, rk=1),
@@ -113,7 +120,7 @@ def main():
         if a.only and a.only != name:
             continue
         if name.startswith("end_"):
-            keep = ["u_final", "avg_final", "summary"]
+            keep = ["u_final", "avg_final", "summary", "plt_", "grid_"]
         else:
             keep = ["u_init", "avg_init", "ent_init", "ut_init", "dtlim_init", "s1_", "u_final", "avg_final", "summary"]
         data = run(args, keep)
@@ -122,7 +129,13 @@ def main():
         if name.startswith("end_"):
             # interior density only keeps the fixture small
             data["rho_final"] = data.pop("u_final")[0, :, :, 1:-1, 1:-1]
-            data["rho_avg_final"] = data.pop("avg_final")[0, 1:-1, 1:-1]
+            avg = data.pop("avg_final")
+            data["rho_avg_final"] = avg[0, 1:-1, 1:-1]
+            if args.get("plt"):
+                data["avg_final"] = avg  # input of the Tecplot writer test
+            else:
+                data.pop("grid_x", None)
+                data.pop("grid_y", None)
         save(name, args, data)
 
 
