@@ -143,6 +143,16 @@ int ppes_advance(ppes_solver *s, double t_final, long long nsteps, double cfl, i
 int ppes_set_timing(ppes_solver *s, int on);
 int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *count);
 
+/* Arithmetic of DG_semidiscretize (no reference counterpart: the choice of
+ * rounding). EXACT evaluates the EC and ES-HLL fluxes in the reference's
+ * written order (DG_solver.f90:894-1071,1571-1681), so residuals equal the
+ * reference's bit for bit wherever log/exp do; FAST (the default) shares one
+ * reciprocal per denominator, a few ulp away. Both keep each face flux
+ * identical for the two elements sharing it. */
+#define PPES_ARITH_EXACT 0
+#define PPES_ARITH_FAST 1
+int ppes_set_arith(ppes_solver *s, int mode);
+
 /* ---- Batched instances (no reference counterpart: the reference runs one
  * problem per process, drive.f90:32-343). A batch holds independent
  * instances of one RK method, each with its own mesh, degree (P1-P4), gamma,
@@ -177,6 +187,7 @@ int ppes_batch_status(ppes_batch *b, int id, int *state, ppes_run_stats *stats);
 int ppes_batch_memory(ppes_batch *b, int id, long long *bytes);
 long long ppes_instance_bytes(const ppes_config *cfg);
 int ppes_batch_set_timing(ppes_batch *b, int on);
+int ppes_batch_set_arith(ppes_batch *b, int mode);
 int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long long *count);
 
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix

@@ -21,6 +21,13 @@ namespace ppes {
         }                                                                                                   \
     } while (0)
 
+// k_residual in the engine's arithmetic mode
+#define RESIDUAL_DISPATCH(p, grid, ...)                                                       \
+    do {                                                                                      \
+        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(p, LAUNCH((k_residual<P, true>), grid, __VA_ARGS__)); }  \
+        else { ENG_DISPATCH(p, LAUNCH((k_residual<P, false>), grid, __VA_ARGS__)); }         \
+    } while (0)
+
 #define ENG_DISPATCH(p, CALL)                     \
     switch (p) {                                  \
     case 1: { constexpr int P = 1; CALL; } break; \
@@ -220,6 +227,29 @@ int Engine::add(const ppes_config &cfg, int *id)
             r.wavg[k + N * l] = (0.25 * h->qw[k]) * h->qw[l];
             r.went[k + N * l] = (((0.25 * h->hx) * h->hy) * h->qw[k]) * h->qw[l];
         }
+    {
+        // rows N-1 and 0 of the powers of D, in long double (OEDG face traces)
+        long double Dl[MAXN][MAXN], Pw[MAXN][MAXN], T[MAXN][MAXN];
+        for (int a = 0; a < N; ++a)
+            for (int b = 0; b < N; ++b) {
+                Dl[a][b] = h->D[a + N * b];
+                Pw[a][b] = (a == b) ? 1.0L : 0.0L;
+            }
+        for (int o = 0; o < N; ++o) {
+            for (int a = 0; a < N; ++a) {
+                r.drow[0][o][a] = (double)Pw[N - 1][a];
+                r.drow[1][o][a] = (double)Pw[0][a];
+            }
+            for (int a = 0; a < N; ++a)
+                for (int b = 0; b < N; ++b) {
+                    long double acc = 0.0L;
+                    for (int c = 0; c < N; ++c) acc += Pw[a][c] * Dl[c][b];
+                    T[a][b] = acc;
+                }
+            for (int a = 0; a < N; ++a)
+                for (int b = 0; b < N; ++b) Pw[a][b] = T[a][b];
+        }
+    }
     r.bc_kind = cfg.bc_kind;
     for (int v = 0; v < 8; ++v) r.bc_state[v] = cfg.bc_state[v];
     const long long E = epb_of(h->p);
@@ -427,7 +457,7 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 1);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 1));
+            RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 1);
         }
     }
     for (int st = 1; st <= S; ++st) {
@@ -461,7 +491,7 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 0);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, st, (st % S) + 1, 0));
+            RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, st, (st % S) + 1, 0);
         }
     }
     LAUNCH1(k_step_end, n_all, d_recs, (const int *)d_all, n_all);
@@ -553,7 +583,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
-        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 0));
+        for (auto &g : tmp) RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 0);
     }
     (void)hipStreamSynchronize(stream);
     for (auto &g : tmp) free_group(g);
@@ -588,7 +618,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         break;
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
-        ENG_DISPATCH(g.p, LAUNCH(k_residual<P>, g.n_el, d_recs, (const Blk *)g.d_el, slot, ut_slot, 0));
+        RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, slot, ut_slot, 0);
         break;
     case API_POST: {
         ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, slot, flags));

@@ -61,6 +61,8 @@ public:
     int launch_step(bool all_kernels);                           // one RK step of every active instance
     int run_steps(long long nsteps, bool use_graph, long long *done_steps);
     int sync_every = 1;                                          // steps queued per host synchronisation
+    int arith = PPES_ARITH_FAST;                                 // residual arithmetic (PPES_ARITH_*)
+    void set_arith(int a) { arith = a; graph_ok = false; }
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
 
     int active_count() const;

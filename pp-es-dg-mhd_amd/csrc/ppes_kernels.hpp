@@ -9,32 +9,55 @@
 
 namespace ppes {
 
-// RK stage combine of one value (drive.f90:130-272), in the reference's
-// written form for each method and stage. gs/gt index the state / residual.
-__device__ __forceinline__ double rk_combine_value(const Inst &I, int st, int v, long long gs, long long gt)
+// Sources of one RK stage (drive.f90:130-272): u^n, the earlier stages and
+// the residuals, resolved once per workgroup.
+struct StageSrc {
+    const gdbl *un, *u1, *u2, *u3, *u4, *t, *t4;
+    double dt;
+    int S, st;
+};
+
+__device__ __forceinline__ StageSrc stage_src(const Inst &I, int st)
 {
-    const double dt = I.ctl.dt;
-    const long long os = v * I.vs + gs, ot = v * I.vsi + gt;
-    const double un = I.u[0][os], t = I.ut[st][ot];
-    switch (I.S) {
+    StageSrc r;
+    r.un = gin(I.u[0]);
+    r.u1 = gin(I.u[1]);
+    r.u2 = gin(I.u[2 <= I.S ? 2 : 1]);
+    r.u3 = gin(I.u[3 <= I.S ? 3 : 1]);
+    r.u4 = gin(I.u[4 <= I.S ? 4 : 1]);
+    r.t = gin(I.ut[st]);
+    r.t4 = gin(I.ut[4 <= I.S ? 4 : 1]);
+    r.dt = I.ctl.dt;
+    r.S = I.S;
+    r.st = st;
+    return r;
+}
+
+// RK stage combine of one value, in the reference's written form for each
+// method and stage. os / ot index the state / residual arrays.
+__device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long os, long long ot)
+{
+    const double dt = c.dt;
+    const double un = c.un[os], t = c.t[ot];
+    switch (c.S) {
     case 1:
         return un + dt * t;
     case 2:
-        if (st == 1) return un + dt * t;
-        return 0.5 * un + 0.5 * (I.u[1][os] + dt * t);
+        if (c.st == 1) return un + dt * t;
+        return 0.5 * un + 0.5 * (c.u1[os] + dt * t);
     case 3:
-        if (st == 1) return un + dt * t;
-        if (st == 2) return 0.75 * un + 0.25 * (I.u[1][os] + dt * t);
-        return (1.0 / 3.0) * un + (2.0 / 3.0) * (I.u[2][os] + dt * t);
+        if (c.st == 1) return un + dt * t;
+        if (c.st == 2) return 0.75 * un + 0.25 * (c.u1[os] + dt * t);
+        return (1.0 / 3.0) * un + (2.0 / 3.0) * (c.u2[os] + dt * t);
     default:
-        switch (st) {
+        switch (c.st) {
         case 1: return un + (0.391752226571890 * dt) * t;
-        case 2: return 0.444370493651235 * un + 0.555629506348765 * I.u[1][os] + (0.368410593050371 * dt) * t;
-        case 3: return 0.620101851488403 * un + 0.379898148511597 * I.u[2][os] + (0.251891774271694 * dt) * t;
-        case 4: return 0.178079954393132 * un + 0.821920045606868 * I.u[3][os] + (0.544974750228521 * dt) * t;
+        case 2: return 0.444370493651235 * un + 0.555629506348765 * c.u1[os] + (0.368410593050371 * dt) * t;
+        case 3: return 0.620101851488403 * un + 0.379898148511597 * c.u2[os] + (0.251891774271694 * dt) * t;
+        case 4: return 0.178079954393132 * un + 0.821920045606868 * c.u3[os] + (0.544974750228521 * dt) * t;
         default:
-            return 0.517231671970585 * I.u[2][os] + 0.096059710526147 * I.u[3][os] + (0.063692468666290 * dt) * I.ut[4][ot] +
-                   0.386708617503269 * I.u[4][os] + (0.226007483236906 * dt) * t;
+            return 0.517231671970585 * c.u2[os] + 0.096059710526147 * c.u3[os] + (0.063692468666290 * dt) * c.t4[ot] +
+                   0.386708617503269 * c.u4[os] + (0.226007483236906 * dt) * t;
         }
     }
 }
@@ -44,7 +67,9 @@ __device__ __forceinline__ double rk_combine_value(const Inst &I, int st, int v,
 // Godunov-Powell source, ES-HLL face fluxes, surface terms and dt_lim.
 // mode 0: every live instance, ut[ut_slot] = L(u[u_slot]);
 // mode 1: only instances retrying a step (ut[1] = L(u^n), drive.f90:104).
-template <int P>
+// FAST: the shared-reciprocal forms of the EC and ES-HLL fluxes (a few ulp
+// from the reference's rounding); otherwise the reference's written forms.
+template <int P, bool FAST>
 __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
     using S = Shape<P>;
@@ -57,8 +82,8 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
     Inst &I = insts[bm.inst];
     if (!inst_live(I)) return;
     if (mode == 1 && !I.ctl.retry_residual) return;
-    const double *__restrict__ u = I.u[u_slot];
-    double *__restrict__ ut = I.ut[ut_slot];
+    const gdbl *__restrict__ u = gin(I.u[u_slot]);
+    gdbl *__restrict__ ut = gout(I.ut[ut_slot]);
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
     const int k = n % N, l = n / N;
@@ -67,15 +92,41 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
     const int i = valid ? (int)(eg % I.nx) + 1 : 1;
     const int j = valid ? (int)(eg / I.nx) + 1 : 1;
     const double g = I.g, hx = I.hx, hy = I.hy;
+    const double inv_gm1 = 1.0 / (g - 1.0);
     const long long vs = I.vs;
     const long long base = gidx(I, i, j) * NN + n;
 
-    // ---- node primitives (DG_solver.f90:734-801)
+    // ---- loads: the node's state, then the face traces this lane's face
+    // fluxes need (used only after the volume phase, so their latency is
+    // hidden behind it)
+    constexpr int FR = (S::FACE + NN - 1) / NN;
     double q[8];
-    double msx = 0.0, msy = 0.0;
+    double tl[FR][8], tr[FR][8];
     if (valid) {
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = u[v * vs + base];
+    }
+#pragma unroll
+    for (int r = 0; r < FR; ++r) {
+        const int f = n + r * NN;
+        if (valid && f < S::FACE) {
+            const int side = f / N, a = f - side * N;
+            long long b1, b2;
+            if (side < 2) {
+                b1 = gidx(I, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
+                b2 = gidx(I, side == 0 ? i : i + 1, j) * NN + N * a;
+            } else {
+                b1 = gidx(I, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
+                b2 = gidx(I, i, side == 2 ? j : j + 1) * NN + a;
+            }
+#pragma unroll
+            for (int v = 0; v < 8; ++v) { tl[r][v] = u[v * vs + b1]; tr[r][v] = u[v * vs + b2]; }
+        }
+    }
+
+    // ---- node primitives (DG_solver.f90:734-801)
+    double msx = 0.0, msy = 0.0;
+    if (valid) {
         const double rho = q[0], mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
         const double inv_rho = 1.0 / rho;
         const double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
@@ -109,7 +160,8 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
                 int kk = 1;
                 while ((kk + 1) * kk / 2 <= a) ++kk;
                 const int rr = a - kk * (kk - 1) / 2;
-                ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
+                if constexpr (FAST) ec_flux_fast<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], inv_gm1, ec);
+                else ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
                 o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[5]; o[5] = ec[6]; o[6] = ec[7];
             } else {
                 const int qq = qp - NN * P / 2;
@@ -117,7 +169,8 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
                 int ll = 1;
                 while ((ll + 1) * ll / 2 <= a) ++ll;
                 const int ss = a - ll * (ll - 1) / 2;
-                ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
+                if constexpr (FAST) ec_flux_fast<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], inv_gm1, ec);
+                else ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
                 o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[4]; o[5] = ec[6]; o[6] = ec[7];
             }
         }
@@ -191,25 +244,21 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
 
     // ---- face fluxes of the element's four faces; both neighbours compute a
     // shared face from identical inputs, so they agree bit for bit
-    if (valid) {
-        for (int f = n; f < S::FACE; f += NN) {
-            const int side = f / N, a = f - side * N;
+#pragma unroll
+    for (int r = 0; r < FR; ++r) {
+        const int f = n + r * NN;
+        if (valid && f < S::FACE) {
+            const int side = f / N;
             double wl[8], wr[8], nf[8], nB, sl, sr;
             if (side < 2) {
-                const long long bl = gidx(I, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
-                const long long br = gidx(I, side == 0 ? i : i + 1, j) * NN + 0 + N * a;
-#pragma unroll
-                for (int v = 0; v < 8; ++v) { wl[v] = u[v * vs + bl]; wr[v] = u[v * vs + br]; }
-                espp_hll_flux_x(wl, wr, g, nf, nB, sl, sr);
+                if constexpr (FAST) espp_hll_flux_x_fast(tl[r], tr[r], g, inv_gm1, nf, nB, sl, sr);
+                else espp_hll_flux_x(tl[r], tr[r], g, nf, nB, sl, sr);
             } else {
-                const long long bb = gidx(I, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
-                const long long bt = gidx(I, i, side == 2 ? j : j + 1) * NN + a + N * 0;
-                double t1[8], t2[8], tf[8];
-#pragma unroll
-                for (int v = 0; v < 8; ++v) { t1[v] = u[v * vs + bb]; t2[v] = u[v * vs + bt]; }
-                swap_xy(t1, wl);
-                swap_xy(t2, wr);
-                espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
+                double tf[8];
+                swap_xy(tl[r], wl);
+                swap_xy(tr[r], wr);
+                if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
+                else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
                 swap_xy(tf, nf);
             }
             double *o = &sbuf[el][f * 11];
@@ -428,11 +477,13 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     const bool interior = valid && xin && yin;
     const long long vs = I.vs;
     const long long base = eg * NN + n;
-    double *__restrict__ out = I.u[st];
+    gdbl *__restrict__ out = gout(I.u[st]);
     const bool use_ldf = I.ctl.use_ldf != 0;
+    const StageSrc src = stage_src(I, st >= 1 ? st : 1);
 
     if (use_ldf && !(flags & F_COMBINE_ONLY)) {
-        for (int c = t; c < M * M; c += BLOCK) sldf[c] = I.ldf[c];
+        const gdbl *ldf = gin(I.ldf);
+        for (int c = t; c < M * M; c += BLOCK) sldf[c] = ldf[c];
     }
 
     // ---- the stage state at this node
@@ -444,7 +495,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
         } else if (interior) {
             const long long gt = ((long long)(ey - 1) * I.nx + (ex - 1)) * NN + n;
 #pragma unroll
-            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(I, st, v, base, gt);
+            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(src, v * vs + base, v * I.vsi + gt);
         } else if (I.bc_kind == BC_FIXED) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) q[v] = I.bc_state[v];
@@ -459,7 +510,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
             const long long gs = gidx(I, sx, sy) * NN + sn;
             const long long gt = ((long long)(sy - 1) * I.nx + (sx - 1)) * NN + sn;
 #pragma unroll
-            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(I, st, v, gs, gt);
+            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(src, v * vs + gs, v * I.vsi + gt);
         }
         if (flags & F_COMBINE_ONLY) {
             if (interior)
@@ -514,7 +565,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
         }
     }
     if (valid && ok)
-        for (int v = n; v < 8; v += NN) I.uavg[eg * 8 + v] = avg[v];
+        for (int v = n; v < 8; v += NN) gout(I.uavg)[eg * 8 + v] = avg[v];
 
     if (stage_oedg(I, st, flags)) {
         // post-LDF state (ghost strips too: the OEDG face jumps read them)
@@ -553,9 +604,10 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
                 mn = dmin(mn, sem[1][e][v]);
                 mxv = dmax(mxv, sem[2][e][v]);
             }
-            I.part[(long long)bm.local * 24 + v] = s;
-            I.part[(long long)bm.local * 24 + 8 + v] = mn;
-            I.part[(long long)bm.local * 24 + 16 + v] = mxv;
+            gdbl *part = gout(I.part) + (long long)bm.local * 24;
+            part[v] = s;
+            part[8 + v] = mn;
+            part[16 + v] = mxv;
         }
         return;
     }
@@ -570,7 +622,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     }
     if (stage_ent(I, st, flags)) {
         const double s = entropy_partial<P>(q, lim, lim, I, sred, sent, scnt, t, el, n);
-        if (t == 0) I.entpart[bm.local] = s;
+        if (t == 0) gout(I.entpart)[bm.local] = s;
     }
     if (valid && ok)
 #pragma unroll
@@ -636,158 +688,103 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk 
 
 // OEDG jump indicators (DG_solver.f90:295-422), one thread per (face,
 // variable); x-faces i = 1..nx+1 between (i-1,j) and (i,j), then y-faces.
+// For the o-th normal derivative the reference differentiates both whole
+// elements o times and takes the face traces; here the traces come straight
+// from rows of D^o (precomputed per degree): trace_o(t) = (2/h)^o
+// sum_a (D^o)(N-1 or 0, a) u(a, t), which is N^2 work per order instead of
+// N^3. The transverse derivatives of the jump and the weighted sums follow
+// the reference's loops.
 template <int P>
 __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
-    constexpr int N = P + 1, NN = N * N;
+    constexpr int N = P + 1, NN = N * N, mxp = P;
+    constexpr int den0 = 2 * (2 * P - 1 > 1 ? 2 * P - 1 : 1);
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
     const long long nxf = (long long)(I.nx + 1) * I.ny * 8, nyf = (long long)I.nx * (I.ny + 1) * 8;
     if (tid >= nxf + nyf) return;
-    const double *__restrict__ u = I.u[st];
-    const double hx = I.hx, hy = I.hy;
-    constexpr int px = P, py = P, mxp = P;
-    constexpr int den0 = 2 * (2 * P - 1 > 1 ? 2 * P - 1 : 1);
-    double A[NN], B[NN], tmp[NN], jump[N], jt[N];
-    double D2x[NN], D2y[NN];
-#pragma unroll
-    for (int m = 0; m < NN; ++m) { D2x[m] = (2.0 / hx) * I.D[m]; D2y[m] = (2.0 / hy) * I.D[m]; }
-    if (tid < nxf) {
-        const int v = (int)(tid % 8);
-        const long long f = tid / 8;
+    const gdbl *__restrict__ u = gin(I.u[st]);
+    const bool xface = tid < nxf;
+    const long long tt = xface ? tid : tid - nxf;
+    const int v = (int)(tt % 8);
+    const long long f = tt / 8;
+    long long bA, bB;
+    double h, ht;  // normal and transverse spacing
+    if (xface) {
         const int i = (int)(f % (I.nx + 1)) + 1, j = (int)(f / (I.nx + 1)) + 1;
-        const long long bl = gidx(I, i - 1, j) * NN, br = gidx(I, i, j) * NN;
-#pragma unroll
-        for (int m = 0; m < NN; ++m) { A[m] = u[v * I.vs + bl + m]; B[m] = u[v * I.vs + br + m]; }
-        double total = 0.0;
-        int fac_kk = 1;
-        double hx_pow_kk = 1.0;
-#pragma unroll
-        for (int kk = 0; kk <= px; ++kk) {
-#pragma unroll
-            for (int l = 0; l < N; ++l) jump[l] = B[0 + N * l] - A[(N - 1) + N * l];
-            int fac_ll = 1;
-            double hx_pow = hx_pow_kk;
-#pragma unroll
-            for (int lq = 0; lq <= mxp - kk; ++lq) {
-                const int mm = kk + lq;
-                double jl1 = 0.0;
-#pragma unroll
-                for (int l = 0; l < N; ++l) jl1 += (0.5 * I.qw[l]) * fabs(jump[l]);
-                const double coef = ((double)(2 * mm + 1) * hx_pow) / (double)(den0 * fac_kk * fac_ll);
-                total = total + coef * jl1;
-                if (lq < py) {
-#pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int b = 0; b < N; ++b) acc += D2y[a + N * b] * jump[b];
-                        jt[a] = acc;
-                    }
-#pragma unroll
-                    for (int a = 0; a < N; ++a) jump[a] = jt[a];
-                    fac_ll = fac_ll * (lq + 1);
-                    hx_pow = hx_pow * hx;
-                }
-            }
-            if (kk < px) {
-#pragma unroll
-                for (int l = 0; l < N; ++l)
-#pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int b = 0; b < N; ++b) acc += D2x[a + N * b] * A[b + N * l];
-                        tmp[a + N * l] = acc;
-                    }
-#pragma unroll
-                for (int m = 0; m < NN; ++m) A[m] = tmp[m];
-#pragma unroll
-                for (int l = 0; l < N; ++l)
-#pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int b = 0; b < N; ++b) acc += D2x[a + N * b] * B[b + N * l];
-                        tmp[a + N * l] = acc;
-                    }
-#pragma unroll
-                for (int m = 0; m < NN; ++m) B[m] = tmp[m];
-                fac_kk = fac_kk * (kk + 1);
-                hx_pow_kk = hx_pow_kk * hx;
-            }
-        }
-        I.sigx[f * 8 + v] = total;
+        bA = gidx(I, i - 1, j) * NN;
+        bB = gidx(I, i, j) * NN;
+        h = I.hx;
+        ht = I.hy;
     } else {
-        const long long tt = tid - nxf;
-        const int v = (int)(tt % 8);
-        const long long f = tt / 8;
         const int i = (int)(f % I.nx) + 1, j = (int)(f / I.nx) + 1;
-        const long long bb = gidx(I, i, j - 1) * NN, bt = gidx(I, i, j) * NN;
+        bA = gidx(I, i, j - 1) * NN;
+        bB = gidx(I, i, j) * NN;
+        h = I.hy;
+        ht = I.hx;
+    }
+    const gdbl *uA = u + v * I.vs + bA;
+    const gdbl *uB = u + v * I.vs + bB;
+    // jmp[o][t]: jump of the o-th normal derivative at transverse node t
+    double jmp[P + 1][N];
 #pragma unroll
-        for (int m = 0; m < NN; ++m) { A[m] = u[v * I.vs + bb + m]; B[m] = u[v * I.vs + bt + m]; }
-        double total = 0.0;
-        int fac_ll = 1;
-        double hy_pow_ll = 1.0;
+    for (int o = 0; o <= P; ++o)
 #pragma unroll
-        for (int lq = 0; lq <= py; ++lq) {
+        for (int t = 0; t < N; ++t) jmp[o][t] = 0.0;
 #pragma unroll
-            for (int k = 0; k < N; ++k) jump[k] = B[k + N * 0] - A[k + N * (N - 1)];
-            int fac_kk = 1;
-            double hy_pow = hy_pow_ll;
+    for (int a = 0; a < N; ++a) {
 #pragma unroll
-            for (int kk = 0; kk <= mxp - lq; ++kk) {
-                const int mm = kk + lq;
-                double jl1 = 0.0;
+        for (int t = 0; t < N; ++t) {
+            const int m = xface ? a + N * t : t + N * a;
+            const double xa = uA[m], xb = uB[m];
 #pragma unroll
-                for (int k = 0; k < N; ++k) jl1 += (0.5 * I.qw[k]) * fabs(jump[k]);
-                const double coef = ((double)(2 * mm + 1) * hy_pow) / (double)(den0 * fac_kk * fac_ll);
-                total = total + coef * jl1;
-                if (kk < px) {
+            for (int o = 0; o <= P; ++o) jmp[o][t] += I.drow[1][o][a] * xb - I.drow[0][o][a] * xa;
+        }
+    }
+    double D2t[NN];
 #pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        double acc = 0.0;
+    for (int m = 0; m < NN; ++m) D2t[m] = (2.0 / ht) * I.D[m];
+    double total = 0.0, scale = 1.0, h_pow_o = 1.0;
+    int fac_o = 1;
 #pragma unroll
-                        for (int b = 0; b < N; ++b) acc += D2x[a + N * b] * jump[b];
-                        jt[a] = acc;
-                    }
+    for (int o = 0; o <= P; ++o) {
+        double jump[N], jt[N];
 #pragma unroll
-                    for (int a = 0; a < N; ++a) jump[a] = jt[a];
-                    fac_kk = fac_kk * (kk + 1);
-                    hy_pow = hy_pow * hy;
+        for (int t = 0; t < N; ++t) jump[t] = scale * jmp[o][t];
+        int fac_q = 1;
+        double h_pow = h_pow_o;
+#pragma unroll
+        for (int q = 0; q <= mxp - o; ++q) {
+            const int mm = o + q;
+            double jl1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < N; ++t) jl1 += (0.5 * I.qw[t]) * fabs(jump[t]);
+            const double coef = ((double)(2 * mm + 1) * h_pow) / (double)(den0 * fac_o * fac_q);
+            total = total + coef * jl1;
+            if (q < P) {
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int b = 0; b < N; ++b) acc += D2t[a + N * b] * jump[b];
+                    jt[a] = acc;
                 }
-            }
-            if (lq < py) {
 #pragma unroll
-                for (int a = 0; a < N; ++a)
-#pragma unroll
-                    for (int k = 0; k < N; ++k) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int b = 0; b < N; ++b) acc += A[k + N * b] * D2y[a + N * b];
-                        tmp[k + N * a] = acc;
-                    }
-#pragma unroll
-                for (int m = 0; m < NN; ++m) A[m] = tmp[m];
-#pragma unroll
-                for (int a = 0; a < N; ++a)
-#pragma unroll
-                    for (int k = 0; k < N; ++k) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int b = 0; b < N; ++b) acc += B[k + N * b] * D2y[a + N * b];
-                        tmp[k + N * a] = acc;
-                    }
-#pragma unroll
-                for (int m = 0; m < NN; ++m) B[m] = tmp[m];
-                fac_ll = fac_ll * (lq + 1);
-                hy_pow_ll = hy_pow_ll * hy;
+                for (int a = 0; a < N; ++a) jump[a] = jt[a];
+                fac_q = fac_q * (q + 1);
+                h_pow = h_pow * h;
             }
         }
-        I.sigy[f * 8 + v] = total;
+        if (o < P) {
+            fac_o = fac_o * (o + 1);
+            h_pow_o = h_pow_o * h;
+            scale = scale * (2.0 / h);
+        }
     }
+    if (xface) gout(I.sigx)[f * 8 + v] = total;
+    else gout(I.sigy)[f * 8 + v] = total;
 }
 
 // OEDG path, limiter loop (DG_solver.f90:428-656): OEDG damping toward the
@@ -814,12 +811,13 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     const long long vs = I.vs;
     const long long ge = gidx(I, i, j);
     const long long base = ge * NN + n;
-    double *__restrict__ u = I.u[st];
+    gdbl *__restrict__ u = gout(I.u[st]);
 
     double q[8], avg[8];
     if (valid) {
+        const gdbl *ua = gin(I.uavg) + ge * 8;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; avg[v] = I.uavg[ge * 8 + v]; }
+        for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; avg[v] = ua[v]; }
     }
     {
         double sx = 0.0, sy = 0.0;
@@ -830,10 +828,10 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
         if (valid) {
             double mex = 0.0, mey = 0.0;
             for (int m = 0; m < NN; ++m) { mex = dmax(mex, sred[el * NN + m][0]); mey = dmax(mey, sred[el * NN + m][1]); }
-            const double *sxL = I.sigx + 8 * ((long long)(i - 1) + (long long)(I.nx + 1) * (j - 1));
-            const double *sxR = I.sigx + 8 * ((long long)i + (long long)(I.nx + 1) * (j - 1));
-            const double *syB = I.sigy + 8 * ((long long)(i - 1) + (long long)I.nx * (j - 1));
-            const double *syT = I.sigy + 8 * ((long long)(i - 1) + (long long)I.nx * j);
+            const gdbl *sxL = gin(I.sigx) + 8 * ((long long)(i - 1) + (long long)(I.nx + 1) * (j - 1));
+            const gdbl *sxR = gin(I.sigx) + 8 * ((long long)i + (long long)(I.nx + 1) * (j - 1));
+            const gdbl *syB = gin(I.sigy) + 8 * ((long long)(i - 1) + (long long)I.nx * (j - 1));
+            const gdbl *syT = gin(I.sigy) + 8 * ((long long)(i - 1) + (long long)I.nx * j);
             double dm = 0.0;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
@@ -859,7 +857,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     }
     if (stage_ent(I, st, flags)) {
         const double s = entropy_partial<P>(q, valid, valid, I, sred, sent, scnt, t, el, n);
-        if (t == 0) I.entpart[bm.local] = s;
+        if (t == 0) gout(I.entpart)[bm.local] = s;
     }
     if (valid) {
 #pragma unroll
@@ -902,7 +900,7 @@ __global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int s
     const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
     const long long nstrip = 2LL * I.ny + 2LL * I.nx;
     if (tid >= nstrip * NN) return;
-    double *__restrict__ u = I.u[slot];
+    gdbl *__restrict__ u = gout(I.u[slot]);
     const int n = (int)(tid % NN);
     const long long s = tid / NN;
     const int k = n % N, l = n / N;
@@ -951,12 +949,13 @@ __global__ void __launch_bounds__(BLOCK) k_ghost_copy(Inst *insts, const Blk *ma
     else if (s < 2LL * I.ny + I.nx) { gx = (int)(s - 2LL * I.ny) + 1; gy = 0; }
     else { gx = (int)(s - 2LL * I.ny - I.nx) + 1; gy = I.ny + 1; }
     const long long gg = gidx(I, gx, gy) * NN + n;
-    double *__restrict__ u = I.u[0];
+    gdbl *__restrict__ u = gout(I.u[0]);
+    gdbl *__restrict__ g0 = gout(I.ghost0);
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
         const long long b = (long long)v * nstrip * NN + tid;
-        if (dir == 0) I.ghost0[b] = u[v * I.vs + gg];
-        else u[v * I.vs + gg] = I.ghost0[b];
+        if (dir == 0) g0[b] = u[v * I.vs + gg];
+        else u[v * I.vs + gg] = g0[b];
     }
 }
 

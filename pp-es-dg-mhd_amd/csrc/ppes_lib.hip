@@ -599,6 +599,13 @@ extern "C" int ppes_solve(ppes_solver *s, double t_final, long long nsteps, doub
 
 extern "C" int ppes_set_timing(ppes_solver *s, int on) { return s ? engine_set_timing(s->eng, on) : PPES_BAD_ARG; }
 
+extern "C" int ppes_set_arith(ppes_solver *s, int mode)
+{
+    if (!s || (mode != PPES_ARITH_EXACT && mode != PPES_ARITH_FAST)) return PPES_BAD_ARG;
+    s->eng.set_arith(mode);
+    return PPES_OK;
+}
+
 extern "C" int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *count)
 {
     return s ? engine_kernel_time(s->eng, kernel, total_ms, count) : PPES_BAD_ARG;
@@ -699,6 +706,13 @@ extern "C" int ppes_batch_memory(ppes_batch *b, int id, long long *bytes)
 }
 
 extern "C" int ppes_batch_set_timing(ppes_batch *b, int on) { return b ? engine_set_timing(b->eng, on) : PPES_BAD_ARG; }
+
+extern "C" int ppes_batch_set_arith(ppes_batch *b, int mode)
+{
+    if (!b || (mode != PPES_ARITH_EXACT && mode != PPES_ARITH_FAST)) return PPES_BAD_ARG;
+    b->eng.set_arith(mode);
+    return PPES_OK;
+}
 
 extern "C" int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long long *count)
 {

@@ -177,6 +177,145 @@ __device__ __forceinline__ void espp_hll_flux_x(const double wl[8], const double
     sr_out = sr;
 }
 
+// ---------------------------------------------------------------------------
+// Shared-reciprocal forms of the two flux functions (the library's fast
+// arithmetic mode): the same mathematics with one division per distinct
+// denominator -- (zeta - 1) / (zeta + 1) as (L - R) / (L + R), x / p as
+// x * (1 / p), / (g - 1) as * (1 / (g - 1)) -- and no fused multiply-adds,
+// so that both elements sharing a face still compute bit-identical face
+// fluxes. Results differ from the reference's rounding by a few ulp.
+
+__device__ __forceinline__ double log_avg_divisor_fast(double LL, double RR)
+{
+    const double f = (LL - RR) / (LL + RR);
+    const double u = f * f;
+    if (u < 1.4e-4) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
+    return (0.5 * log(LL / RR)) / f;
+}
+
+template <int DIR>
+__device__ __forceinline__ void ec_flux_fast(const Prim &L, const Prim &R, double inv_gm1, double f[8])
+{
+    const double rho_bar = 0.5 * (L.rho + R.rho);
+    const double vx_bar = 0.5 * (L.vx + R.vx);
+    const double vy_bar = 0.5 * (L.vy + R.vy);
+    const double vz_bar = 0.5 * (L.vz + R.vz);
+    const double bx_bar = 0.5 * (L.bx + R.bx);
+    const double by_bar = 0.5 * (L.by + R.by);
+    const double bz_bar = 0.5 * (L.bz + R.bz);
+    const double beta_bar = 0.5 * (L.beta + R.beta);
+    const double v_sq_bar = 0.5 * ((L.vx * L.vx + L.vy * L.vy + L.vz * L.vz) + (R.vx * R.vx + R.vy * R.vy + R.vz * R.vz));
+    const double b_sq_bar = 0.5 * ((L.bx * L.bx + L.by * L.by + L.bz * L.bz) + (R.bx * R.bx + R.by * R.by + R.bz * R.bz));
+    const double betavx_bar = 0.5 * (L.beta * L.vx + R.beta * R.vx);
+    const double betavy_bar = 0.5 * (L.beta * L.vy + R.beta * R.vy);
+    const double betavz_bar = 0.5 * (L.beta * L.vz + R.beta * R.vz);
+    const double inv_bb = 1.0 / beta_bar;
+    const double rho_hat = rho_bar / log_avg_divisor_fast(L.rho, R.rho);
+    // 1 / ((g - 1) beta_hat) with beta_hat = beta_bar / divisor
+    const double e_coef = log_avg_divisor_fast(L.beta, R.beta) * inv_bb * inv_gm1;
+    if (DIR == 0) {
+        f[0] = rho_hat * vx_bar;
+        f[1] = vx_bar * f[0] - bx_bar * bx_bar + 0.5 * (b_sq_bar + rho_bar * inv_bb);
+        f[2] = vy_bar * f[0] - bx_bar * by_bar;
+        f[3] = vz_bar * f[0] - bx_bar * bz_bar;
+        f[4] = 0.0;
+        f[5] = (betavx_bar * by_bar - betavy_bar * bx_bar) * inv_bb;
+        f[6] = (betavx_bar * bz_bar - betavz_bar * bx_bar) * inv_bb;
+        f[7] = 0.5 * (e_coef - v_sq_bar) * f[0] + (vx_bar * f[1] + vy_bar * f[2] + vz_bar * f[3]) +
+               (by_bar * f[5] + bz_bar * f[6]) - 0.5 * vx_bar * b_sq_bar +
+               (vx_bar * bx_bar + vy_bar * by_bar + vz_bar * bz_bar) * bx_bar;
+    } else {
+        f[0] = rho_hat * vy_bar;
+        f[1] = vx_bar * f[0] - by_bar * bx_bar;
+        f[2] = vy_bar * f[0] - by_bar * by_bar + 0.5 * (b_sq_bar + rho_bar * inv_bb);
+        f[3] = vz_bar * f[0] - by_bar * bz_bar;
+        f[4] = (betavy_bar * bx_bar - betavx_bar * by_bar) * inv_bb;
+        f[5] = 0.0;
+        f[6] = (betavy_bar * bz_bar - betavz_bar * by_bar) * inv_bb;
+        f[7] = 0.5 * (e_coef - v_sq_bar) * f[0] + (vx_bar * f[1] + vy_bar * f[2] + vz_bar * f[3]) +
+               (bx_bar * f[4] + bz_bar * f[6]) - 0.5 * vy_bar * b_sq_bar +
+               (vx_bar * bx_bar + vy_bar * by_bar + vz_bar * bz_bar) * by_bar;
+    }
+}
+
+__device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const double wr[8], double g, double inv_gm1,
+                                                     double nflux[8], double &nBn, double &sl_out, double &sr_out)
+{
+    const double rho_l = wl[0], mx_l = wl[1], my_l = wl[2], mz_l = wl[3], bx_l = wl[4], by_l = wl[5], bz_l = wl[6], e_l = wl[7];
+    const double rho_r = wr[0], mx_r = wr[1], my_r = wr[2], mz_r = wr[3], bx_r = wr[4], by_r = wr[5], bz_r = wr[6], e_r = wr[7];
+    const double inv_rho_l = 1.0 / rho_l, inv_rho_r = 1.0 / rho_r;
+    const double vx_l = inv_rho_l * mx_l, vx_r = inv_rho_r * mx_r;
+    const double vy_l = inv_rho_l * my_l, vy_r = inv_rho_r * my_r;
+    const double vz_l = inv_rho_l * mz_l, vz_r = inv_rho_r * mz_r;
+    const double e_knt_l = 0.5 * inv_rho_l * (mx_l * mx_l + my_l * my_l + mz_l * mz_l);
+    const double e_knt_r = 0.5 * inv_rho_r * (mx_r * mx_r + my_r * my_r + mz_r * mz_r);
+    const double ey_l = bz_l * vx_l - bx_l * vz_l, ey_r = bz_r * vx_r - bx_r * vz_r;
+    const double ez_l = bx_l * vy_l - by_l * vx_l, ez_r = bx_r * vy_r - by_r * vx_r;
+    const double b_sq_l = bx_l * bx_l + by_l * by_l + bz_l * bz_l;
+    const double b_sq_r = bx_r * bx_r + by_r * by_r + bz_r * bz_r;
+    const double p_mag_l = 0.5 * b_sq_l, p_mag_r = 0.5 * b_sq_r;
+    const double p_fld_l = dmax((g - 1.0) * (e_l - (e_knt_l + p_mag_l)), PP_EPS);
+    const double p_fld_r = dmax((g - 1.0) * (e_r - (e_knt_r + p_mag_r)), PP_EPS);
+    const double inv_p_l = 1.0 / p_fld_l, inv_p_r = 1.0 / p_fld_r;
+    const double p_tot_l = p_fld_l + p_mag_l, p_tot_r = p_fld_r + p_mag_r;
+    const double b_dot_u_l = bx_l * vx_l + by_l * vy_l + bz_l * vz_l;
+    const double b_dot_u_r = bx_r * vx_r + by_r * vy_r + bz_r * vz_r;
+    const double snc_sq_l = g * p_fld_l * inv_rho_l, snc_sq_r = g * p_fld_r * inv_rho_r;
+    const double sum_sq_l = snc_sq_l + b_sq_l * inv_rho_l, sum_sq_r = snc_sq_r + b_sq_r * inv_rho_r;
+    const double cfx_l = sqrt(0.5 * (sum_sq_l + sqrt(dmax(sum_sq_l * sum_sq_l - 4.0 * snc_sq_l * inv_rho_l * (bx_l * bx_l), 0.0))));
+    const double cfx_r = sqrt(0.5 * (sum_sq_r + sqrt(dmax(sum_sq_r * sum_sq_r - 4.0 * snc_sq_r * inv_rho_r * (bx_r * bx_r), 0.0))));
+    const double inv_theta_l = rho_l * inv_p_l, inv_theta_r = rho_r * inv_p_r;
+    const double sp_ent_l = log(p_fld_l) - g * log(rho_l);
+    const double sp_ent_r = log(p_fld_r) - g * log(rho_r);
+    const double ent_flux_l = -(rho_l * sp_ent_l) * inv_gm1 * vx_l;
+    const double ent_flux_r = -(rho_r * sp_ent_r) * inv_gm1 * vx_r;
+    double fl[8], fr[8];
+    fl[0] = mx_l; fl[1] = vx_l * mx_l - bx_l * bx_l + p_tot_l; fl[2] = vx_l * my_l - by_l * bx_l;
+    fl[3] = vx_l * mz_l - bz_l * bx_l; fl[4] = 0.0; fl[5] = -ez_l; fl[6] = ey_l;
+    fl[7] = vx_l * (e_l + p_tot_l) - bx_l * b_dot_u_l;
+    fr[0] = mx_r; fr[1] = vx_r * mx_r - bx_r * bx_r + p_tot_r; fr[2] = vx_r * my_r - by_r * bx_r;
+    fr[3] = vx_r * mz_r - bz_r * bx_r; fr[4] = 0.0; fr[5] = -ez_r; fr[6] = ey_r;
+    fr[7] = vx_r * (e_r + p_tot_r) - bx_r * b_dot_u_r;
+    // entropy variables (v[4..6] = B / theta, v[7] = -1 / theta)
+    double vl[8], vr[8];
+    vl[0] = (g - sp_ent_l) * inv_gm1 - e_knt_l * inv_p_l;
+    vl[1] = mx_l * inv_p_l; vl[2] = my_l * inv_p_l; vl[3] = mz_l * inv_p_l;
+    vl[4] = inv_theta_l * bx_l; vl[5] = inv_theta_l * by_l; vl[6] = inv_theta_l * bz_l; vl[7] = -inv_theta_l;
+    vr[0] = (g - sp_ent_r) * inv_gm1 - e_knt_r * inv_p_r;
+    vr[1] = mx_r * inv_p_r; vr[2] = my_r * inv_p_r; vr[3] = mz_r * inv_p_r;
+    vr[4] = inv_theta_r * bx_r; vr[5] = inv_theta_r * by_r; vr[6] = inv_theta_r * bz_r; vr[7] = -inv_theta_r;
+    const double sqrt_rho_l = sqrt(rho_l), sqrt_rho_r = sqrt(rho_r);
+    const double inv_ssr = 1.0 / (sqrt_rho_l + sqrt_rho_r);
+    const double vx_roe = (vx_l * sqrt_rho_l + vx_r * sqrt_rho_r) * inv_ssr;
+    const double db_aux = sqrt((bx_l - bx_r) * (bx_l - bx_r) + (by_l - by_r) * (by_l - by_r) + (bz_l - bz_r) * (bz_l - bz_r)) * inv_ssr;
+    const double sl_pp = dmin(vx_l, vx_roe) - cfx_l - db_aux;
+    const double sr_pp = dmax(vx_r, vx_roe) + cfx_r + db_aux;
+    double dot_a = 0.0, dot_bl = 0.0, dot_br = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const double df = fr[i] - fl[i];
+        dot_a += (vr[i] - vl[i]) * (wr[i] - wl[i]);
+        dot_bl += vl[i] * df;
+        dot_br += vr[i] * df;
+    }
+    const double inv_a = 1.0 / (dmax(dot_a, 0.0) + 1e-8);
+    const double dflux = ent_flux_r - ent_flux_l;
+    const double bl_form = dflux - dot_bl - (inv_theta_l * b_dot_u_l) * (bx_r - bx_l);
+    const double br_form = dflux - dot_br - (inv_theta_r * b_dot_u_r) * (bx_r - bx_l);
+    const double al_form = dmax(bl_form * inv_a, 0.0);
+    const double ar_form = dmax(br_form * inv_a, 0.0);
+    const double aa_form = sqrt(al_form * ar_form);
+    const double sl = dmin(dmin(-(ar_form + aa_form), sl_pp), 0.0);
+    const double sr = dmax(dmax(al_form + aa_form, sr_pp), 0.0);
+    const double inv_d = 1.0 / (sr - sl);
+    const double wei_l = sr * inv_d, wei_r = -sl * inv_d, c = (sl * sr) * inv_d;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) nflux[i] = wei_l * fl[i] + wei_r * fr[i] + c * (wr[i] - wl[i]);
+    nBn = wei_l * bx_l + wei_r * bx_r;
+    sl_out = sl;
+    sr_out = sr;
+}
+
 // x <-> y component swap (swap_xy, DG_solver.f90:1556-1568)
 __device__ __forceinline__ void swap_xy(const double a[8], double b[8])
 {

@@ -28,6 +28,16 @@
 namespace ppes {
 
 constexpr int BLOCK = 256;
+
+// Device global memory seen through an explicit address space. The state
+// arrays are reached through pointers stored in the instance records, so the
+// compiler cannot infer that they are global; through these casts their
+// loads and stores are global-memory instructions (vector-memory counter
+// only) rather than flat ones, which also hold the LDS counter and so make
+// LDS waits wait for HBM.
+typedef __attribute__((address_space(1))) double gdbl;
+__device__ __forceinline__ const gdbl *gin(const double *p) { return (const gdbl *)p; }
+__device__ __forceinline__ gdbl *gout(double *p) { return (gdbl *)p; }
 constexpr int MAXN = 5;  // p <= 4 on the device
 constexpr int MAXS = 5;  // RK stages
 
@@ -55,6 +65,7 @@ struct Inst {
     double D[MAXN * MAXN];           // D[k + N r] = l_r'(x_k)
     double wavg[MAXN * MAXN];        // (0.25 w_k) w_l
     double went[MAXN * MAXN];        // (((0.25 hx) hy) w_k) w_l
+    double drow[2][MAXN][MAXN];      // drow[0][o][a] = (D^o)(N-1, a), drow[1][o][a] = (D^o)(0, a)
     int bc_kind, pad0;
     double bc_state[8];
     double *u[MAXS + 1];             // state slots: 0 = u^n, s = uRK(s)

@@ -83,7 +83,8 @@ contains
 		rc = ppes_solve_begin(ppes_handle, merge(1, 0, use_LDF))
 		if (rc == PPES_NEG_AVG) error stop 'Error: Initial value has negative cell average!'
 		if (rc /= PPES_OK) error stop 'drive: solve_execute: device start-up failed'
-		rc = ppes_advance(ppes_handle, real(final_T, c_double), 0_c_long_long, real(CFL, c_double), oedg_mode, st)
+		! no steps: reads the start-up's entropy and dt_lim
+		rc = ppes_advance(ppes_handle, -1.0_c_double, 0_c_long_long, real(CFL, c_double), oedg_mode, st)
 		write(stdout, "(A, ES14.7, A, ES14.7, A, ES14.7, A, ES14.7)") 't=', 0.0, ', ent=', st%tot_ent, &
 			', dt_lim=', st%dt_lim, ', time per it:', 0.0
 		allocate(my_log(2, 64))

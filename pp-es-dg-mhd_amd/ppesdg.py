@@ -21,6 +21,7 @@ OK, NEG_AVG, DT_COLLAPSE, PP_FAIL, BAD_ARG, HIP_ERROR, OOM, UNSUPPORTED = range(
 BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3}
 KERNELS = {"residual": 0, "stage": 1, "oedg": 2, "limit": 3, "reduce": 4, "bc": 5}
 INST_STATE = {0: "idle", 1: "active", 2: "done", 3: "failed"}
+ARITH = {"exact": 0, "fast": 1}
 PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5}
 
 _lib = None
@@ -81,6 +82,8 @@ def lib():
             "ppes_advance": [vp, ctypes.c_double, ctypes.c_longlong, ctypes.c_double, ctypes.c_int,
                              ctypes.POINTER(RunStats)],
             "ppes_set_timing": [vp, ctypes.c_int],
+            "ppes_set_arith": [vp, ctypes.c_int],
+            "ppes_batch_set_arith": [vp, ctypes.c_int],
             "ppes_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
             "ppes_batch_create": [ctypes.c_int, ctypes.POINTER(vp)],
             "ppes_batch_destroy": [vp],
@@ -161,11 +164,13 @@ def _stats(st):
 class Solver:
     """One device-resident instance (init_DG_solver ... finalize_DG_solver)."""
 
-    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
+    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3, arith=None):
         cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, rk)
         h = ctypes.c_void_p()
         _check(lib().ppes_create(ctypes.byref(cfg), ctypes.byref(h)), "ppes_create")
         self.h = h
+        if arith is not None:
+            self.set_arith(arith)
         self.nx, self.ny, self.p, self.N, self.rk = nx, ny, p, p + 1, rk
         N = self.N
         self.shape_u = (8, N, N, nx + 2, ny + 2)
@@ -257,6 +262,10 @@ class Solver:
     def set_timing(self, on):
         _check(lib().ppes_set_timing(self.h, int(on)), "ppes_set_timing")
 
+    def set_arith(self, mode):
+        """"exact" (the reference's rounding order) or "fast" (default)"""
+        _check(lib().ppes_set_arith(self.h, ARITH[mode]), "ppes_set_arith")
+
     def kernel_time(self, kernel):
         """(total ms, launches) of one kernel class while timing is on."""
         ms, n = ctypes.c_double(), ctypes.c_longlong()
@@ -342,6 +351,9 @@ class Batch:
 
     def set_timing(self, on):
         _check(lib().ppes_batch_set_timing(self.h, int(on)), "ppes_batch_set_timing")
+
+    def set_arith(self, mode):
+        _check(lib().ppes_batch_set_arith(self.h, ARITH[mode]), "ppes_batch_set_arith")
 
     def kernel_time(self, kernel):
         ms, n = ctypes.c_double(), ctypes.c_longlong()

@@ -55,21 +55,42 @@ def test_init_perturbed_ot_matches_oracle():
     assert normwise(interior(g.download_state(0)), interior(o.u)) <= 1e-15
 
 
+@pytest.mark.parametrize("arith", ["exact", "fast"])
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
 @pytest.mark.parametrize("prob", PROBS)
-def test_residual_matches_oracle(prob, p):
+def test_residual_matches_oracle(prob, p, arith):
     nx, ny = 9, 7
     u = advanced_state(prob, nx, ny, p, 2)
     o, g = make_pair(prob, nx, ny, p)
+    g.set_arith(arith)
     ut_ref = np.zeros(o.shape_ut, order="F")
     dtl_ref = o.semidiscretize(ut_ref, u)
     g.upload_state(0, u)
     dtl = g.semidiscretize(1, 0)
     ut = g.download_residual(1)
     err = normwise(ut, ut_ref)
-    print(f"residual {prob} P{p}: normwise {err:.3e}, dt_lim {dtl} vs {dtl_ref}")
+    print(f"residual {prob} P{p} {arith}: normwise {err:.3e}, dt_lim {dtl} vs {dtl_ref}")
     assert err <= 1e-12
     assert abs(dtl - dtl_ref) <= 1e-14 * dtl_ref
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("prob", ["ot2pi", "blast"])
+def test_residual_exact_mode_is_bitwise(prob, p):
+    """In the exact mode the residual is the oracle's (and so the
+    reference's) bit for bit on these states, where no log/exp last-bit
+    difference reaches it."""
+    nx, ny = 9, 7
+    u = advanced_state(prob, nx, ny, p, 2)
+    o, g = make_pair(prob, nx, ny, p)
+    g.set_arith("exact")
+    ut_ref = np.zeros(o.shape_ut, order="F")
+    dtl_ref = o.semidiscretize(ut_ref, u)
+    g.upload_state(0, u)
+    dtl = g.semidiscretize(1, 0)
+    ut = g.download_residual(1)
+    print(f"exact residual {prob} P{p}: {np.count_nonzero(ut != ut_ref)} of {ut.size} values differ")
+    assert np.array_equal(ut, ut_ref) and dtl == dtl_ref
 
 
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
