@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ppes_types.hpp"
 
 namespace ppes {
@@ -69,13 +71,25 @@ __device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long 
 // mode 1: only instances retrying a step (ut[1] = L(u^n), drive.f90:104).
 // FAST: the shared-reciprocal forms of the EC and ES-HLL fluxes (a few ulp
 // from the reference's rounding); otherwise the reference's written forms.
+// Build-time shape of the fast residual: waves per SIMD asked of the register
+// allocator, early trace loads, one-round pair buffer.
+#ifndef PPES_FAST_WAVES
+#define PPES_FAST_WAVES 1
+#endif
+#ifndef PPES_FAST_PREFETCH
+#define PPES_FAST_PREFETCH 0
+#endif
+#ifndef PPES_FAST_ROUND
+#define PPES_FAST_ROUND 1
+#endif
+
 template <int P, bool FAST>
-__global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FAST ? PPES_FAST_WAVES : 1))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
     using S = Shape<P>;
     constexpr int N = S::N, NN = S::NN, EPB = S::EPB, REC = S::REC;
     __shared__ Prim sprim[EPB][NN];
-    __shared__ double sbuf[EPB][S::BUF];
+    __shared__ double sbuf[EPB][(FAST && PPES_FAST_ROUND) ? S::BUF_ROUND : S::BUF];
     __shared__ double sred[BLOCK][2];
 
     const Blk bm = map[blockIdx.x];
@@ -96,33 +110,40 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
     const long long vs = I.vs;
     const long long base = gidx(I, i, j) * NN + n;
 
-    // ---- loads: the node's state, then the face traces this lane's face
-    // fluxes need (used only after the volume phase, so their latency is
-    // hidden behind it)
+    // ---- loads: the node's state, and the face traces this lane's face
+    // fluxes need. The exact form loads the traces here (used only after the
+    // volume phase, so their latency hides behind it); the fast form, which
+    // trades those registers for a third wave per SIMD, loads them just
+    // before the face phase.
     constexpr int FR = (S::FACE + NN - 1) / NN;
+    constexpr bool PREFETCH = !FAST || PPES_FAST_PREFETCH;
+    constexpr bool ROUND = FAST && PPES_FAST_ROUND;
     double q[8];
     double tl[FR][8], tr[FR][8];
     if (valid) {
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = u[v * vs + base];
     }
+    auto load_traces = [&]() {
 #pragma unroll
-    for (int r = 0; r < FR; ++r) {
-        const int f = n + r * NN;
-        if (valid && f < S::FACE) {
-            const int side = f / N, a = f - side * N;
-            long long b1, b2;
-            if (side < 2) {
-                b1 = gidx(I, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
-                b2 = gidx(I, side == 0 ? i : i + 1, j) * NN + N * a;
-            } else {
-                b1 = gidx(I, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
-                b2 = gidx(I, i, side == 2 ? j : j + 1) * NN + a;
+        for (int r = 0; r < FR; ++r) {
+            const int f = n + r * NN;
+            if (valid && f < S::FACE) {
+                const int side = f / N, a = f - side * N;
+                long long b1, b2;
+                if (side < 2) {
+                    b1 = gidx(I, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
+                    b2 = gidx(I, side == 0 ? i : i + 1, j) * NN + N * a;
+                } else {
+                    b1 = gidx(I, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
+                    b2 = gidx(I, i, side == 2 ? j : j + 1) * NN + a;
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) { tl[r][v] = u[v * vs + b1]; tr[r][v] = u[v * vs + b2]; }
             }
-#pragma unroll
-            for (int v = 0; v < 8; ++v) { tl[r][v] = u[v * vs + b1]; tr[r][v] = u[v * vs + b2]; }
         }
-    }
+    };
+    if constexpr (PREFETCH) load_traces();
 
     // ---- node primitives (DG_solver.f90:734-801)
     double msx = 0.0, msy = 0.0;
@@ -147,40 +168,13 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
     }
     __syncthreads();
 
-    // ---- EC pair fluxes, each pair evaluated once (P rounds, all lanes busy);
-    // the component that is identically zero (4 in x, 5 in y) is not stored
-    if (valid) {
-#pragma unroll
-        for (int c = 0; c < P; ++c) {
-            const int qp = n + c * NN;
-            double ec[8];
-            double *o = &sbuf[el][qp * REC];
-            if (qp < NN * P / 2) {
-                const int lr = qp / S::H, a = qp - lr * S::H;
-                int kk = 1;
-                while ((kk + 1) * kk / 2 <= a) ++kk;
-                const int rr = a - kk * (kk - 1) / 2;
-                if constexpr (FAST) ec_flux_fast<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], inv_gm1, ec);
-                else ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
-                o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[5]; o[5] = ec[6]; o[6] = ec[7];
-            } else {
-                const int qq = qp - NN * P / 2;
-                const int kc = qq / S::H, a = qq - kc * S::H;
-                int ll = 1;
-                while ((ll + 1) * ll / 2 <= a) ++ll;
-                const int ss = a - ll * (ll - 1) / 2;
-                if constexpr (FAST) ec_flux_fast<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], inv_gm1, ec);
-                else ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
-                o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[4]; o[5] = ec[6]; o[6] = ec[7];
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- volume terms in the reference's accumulation order (DG_solver.f90:828-851)
+    // ---- volume terms (DG_solver.f90:813-851): divB, the diagonal terms, then
+    // the EC pair fluxes, each pair evaluated once (P rounds, all lanes
+    // busy); the component that is identically zero (4 in x, 5 in y) is not
+    // stored
     double acc[8];
+    const double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
     if (valid) {
-        const double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
         double divB;
         {
             double a = 0.0, b = 0.0;
@@ -190,60 +184,103 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Inst *insts, const Blk *map,
             for (int s = 0; s < N; ++s) b += sprim[el][k + N * s].by * I.D[l + N * s];
             divB = (2.0 / hx) * a + (2.0 / hy) * b;
         }
-        {
-            const Prim &pr = sprim[el][n];
-            const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
-            const double vx = pr.vx, vy = pr.vy, vz = pr.vz;
-            const double inv_rho = 1.0 / q[0];
-            const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
-            const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
-            const double b_sq = bx * bx + by * by + bz * bz;
-            const double p_mag = 0.5 * b_sq;
-            const double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
-            const double p_tot = p_fld + p_mag;
-            const double b_dot_u = bx * vx + by * vy + bz * vz;
-            const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
-                                 0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
-            const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
-                                 ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
-            const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
-            const double dxx = cx * I.D[k + N * k], dyy = cy * I.D[l + N * l];
+        const Prim &pr = sprim[el][n];
+        const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
+        const double vx = pr.vx, vy = pr.vy, vz = pr.vz;
+        const double inv_rho = 1.0 / q[0];
+        const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
+        const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
+        const double b_sq = bx * bx + by * by + bz * bz;
+        const double p_mag = 0.5 * b_sq;
+        const double p_fld = dmax((g - 1.0) * (e_tot - (e_knt + p_mag)), PP_EPS);
+        const double p_tot = p_fld + p_mag;
+        const double b_dot_u = bx * vx + by * vy + bz * vz;
+        const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
+                             0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
+        const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
+                             ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
+        const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
+        const double dxx = cx * I.D[k + N * k], dyy = cy * I.D[l + N * l];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) acc[v] = 0.0 - dxx * F[v] - dyy * G[v] - divB * GP[v];
+        for (int v = 0; v < 8; ++v) acc[v] = 0.0 - dxx * F[v] - dyy * G[v] - divB * GP[v];
+    }
+
+    // pair qp of the element: x-pairs (kk, rr), rr < kk, of row lr first, then
+    // y-pairs (ll, ss), ss < ll, of column kc
+    auto pair_flux = [&](int qp, double *o) {
+        double ec[8];
+        if (qp < NN * P / 2) {
+            const int lr = qp / S::H, a = qp - lr * S::H;
+            int kk = 1;
+            while ((kk + 1) * kk / 2 <= a) ++kk;
+            const int rr = a - kk * (kk - 1) / 2;
+            if constexpr (FAST) ec_flux_fast<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], inv_gm1, ec);
+            else ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
+            o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[5]; o[5] = ec[6]; o[6] = ec[7];
+        } else {
+            const int qq = qp - NN * P / 2;
+            const int kc = qq / S::H, a = qq - kc * S::H;
+            int ll = 1;
+            while ((ll + 1) * ll / 2 <= a) ++ll;
+            const int ss = a - ll * (ll - 1) / 2;
+            if constexpr (FAST) ec_flux_fast<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], inv_gm1, ec);
+            else ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
+            o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[4]; o[5] = ec[6]; o[6] = ec[7];
         }
-        // x-pairs (k, r), r < k
-        for (int r = 0; r < k; ++r) {
-            const double *o = &sbuf[el][(l * S::H + k * (k - 1) / 2 + r) * REC];
-            const double a = cx * I.D[k + N * r];
-            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
-            acc[5] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+    };
+    auto sub_x = [&](double a, const double *o) {
+        acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
+        acc[5] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+    };
+    auto sub_y = [&](double a, const double *o) {
+        acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
+        acc[4] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+    };
+
+    if constexpr (!ROUND) {
+        // all pair records stored, then gathered in the reference's
+        // accumulation order (DG_solver.f90:828-851)
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < P; ++c) pair_flux(n + c * NN, &sbuf[el][(n + c * NN) * REC]);
         }
-        // y-pairs ((k,l), (k,s)), s < l
-        for (int s = 0; s < l; ++s) {
-            const double *o = &sbuf[el][(NN * P / 2 + k * S::H + l * (l - 1) / 2 + s) * REC];
-            const double a = cy * I.D[l + N * s];
-            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
-            acc[4] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+        __syncthreads();
+        if (valid) {
+            for (int r = 0; r < k; ++r) sub_x(cx * I.D[k + N * r], &sbuf[el][(l * S::H + k * (k - 1) / 2 + r) * REC]);
+            for (int s = 0; s < l; ++s)
+                sub_y(cy * I.D[l + N * s], &sbuf[el][(NN * P / 2 + k * S::H + l * (l - 1) / 2 + s) * REC]);
+            for (int kk = k + 1; kk < N; ++kk)
+                sub_x(cx * I.D[k + N * kk], &sbuf[el][(l * S::H + kk * (kk - 1) / 2 + k) * REC]);
+            for (int ll = l + 1; ll < N; ++ll)
+                sub_y(cy * I.D[l + N * ll], &sbuf[el][(NN * P / 2 + k * S::H + ll * (ll - 1) / 2 + l) * REC]);
         }
-        // x-pairs (k', k), k' > k
-        for (int kk = k + 1; kk < N; ++kk) {
-            const double *o = &sbuf[el][(l * S::H + kk * (kk - 1) / 2 + k) * REC];
-            const double a = cx * I.D[k + N * kk];
-            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
-            acc[5] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
-        }
-        // y-pairs ((k,l'), (k,l)), l' > l
-        for (int ll = l + 1; ll < N; ++ll) {
-            const double *o = &sbuf[el][(NN * P / 2 + k * S::H + ll * (ll - 1) / 2 + l) * REC];
-            const double a = cy * I.D[l + N * ll];
-            acc[0] -= a * o[0]; acc[1] -= a * o[1]; acc[2] -= a * o[2]; acc[3] -= a * o[3];
-            acc[4] -= a * o[4]; acc[6] -= a * o[5]; acc[7] -= a * o[6];
+    } else {
+        // one round's records at a time (half the LDS of the ordered form):
+        // after round c every node adds the pairs of that round it belongs to
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+            if (valid) pair_flux(n + c * NN, &sbuf[el][n * REC]);
+            __syncthreads();
+            if (valid) {
+                for (int r = 0; r < N; ++r) {
+                    if (r == k) continue;
+                    const int qp = l * S::H + (r > k ? r * (r - 1) / 2 + k : k * (k - 1) / 2 + r);
+                    if (qp / NN == c) sub_x(cx * I.D[k + N * r], &sbuf[el][(qp - c * NN) * REC]);
+                }
+                for (int s2 = 0; s2 < N; ++s2) {
+                    if (s2 == l) continue;
+                    const int qp = NN * P / 2 + k * S::H + (s2 > l ? s2 * (s2 - 1) / 2 + l : l * (l - 1) / 2 + s2);
+                    if (qp / NN == c) sub_y(cy * I.D[l + N * s2], &sbuf[el][(qp - c * NN) * REC]);
+                }
+            }
+            if (c + 1 < P) __syncthreads();
         }
     }
     __syncthreads();  // pair records consumed; the buffer now holds face data
 
     // ---- face fluxes of the element's four faces; both neighbours compute a
     // shared face from identical inputs, so they agree bit for bit
+    if constexpr (!PREFETCH) load_traces();
 #pragma unroll
     for (int r = 0; r < FR; ++r) {
         const int f = n + r * NN;
@@ -704,10 +741,10 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
     const long long nxf = (long long)(I.nx + 1) * I.ny * 8, nyf = (long long)I.nx * (I.ny + 1) * 8;
-    if (tid >= nxf + nyf) return;
+    const bool live = tid < nxf + nyf;
     const gdbl *__restrict__ u = gin(I.u[st]);
     const bool xface = tid < nxf;
-    const long long tt = xface ? tid : tid - nxf;
+    const long long tt = live ? (xface ? tid : tid - nxf) : 0;
     const int v = (int)(tt % 8);
     const long long f = tt / 8;
     long long bA, bB;
@@ -725,24 +762,40 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
         h = I.hy;
         ht = I.hx;
     }
+    // the rows of D^o, shared by the workgroup (one instance per workgroup)
+    __shared__ double srow[2][P + 1][N];
+    if (threadIdx.x < 2 * (P + 1) * N) {
+        const int x = threadIdx.x;
+        srow[x / ((P + 1) * N)][(x / N) % (P + 1)][x % N] = I.drow[x / ((P + 1) * N)][(x / N) % (P + 1)][x % N];
+    }
+    __syncthreads();
+    if (!live) return;
     const gdbl *uA = u + v * I.vs + bA;
     const gdbl *uB = u + v * I.vs + bB;
+    double A[NN], B[NN];
+#pragma unroll
+    for (int m = 0; m < NN; ++m) { A[m] = uA[m]; B[m] = uB[m]; }
     // jmp[o][t]: jump of the o-th normal derivative at transverse node t
     double jmp[P + 1][N];
 #pragma unroll
     for (int o = 0; o <= P; ++o)
 #pragma unroll
         for (int t = 0; t < N; ++t) jmp[o][t] = 0.0;
+    // node (a, t) of the face-normal / transverse index pair: a + N t on
+    // x-faces, t + N a on y-faces (two code paths, no per-value selects)
+    auto traces = [&](auto XF) {
 #pragma unroll
-    for (int a = 0; a < N; ++a) {
+        for (int a = 0; a < N; ++a) {
 #pragma unroll
-        for (int t = 0; t < N; ++t) {
-            const int m = xface ? a + N * t : t + N * a;
-            const double xa = uA[m], xb = uB[m];
+            for (int t = 0; t < N; ++t) {
+                const int m = decltype(XF)::value ? a + N * t : t + N * a;
 #pragma unroll
-            for (int o = 0; o <= P; ++o) jmp[o][t] += I.drow[1][o][a] * xb - I.drow[0][o][a] * xa;
+                for (int o = 0; o <= P; ++o) jmp[o][t] += srow[1][o][a] * B[m] - srow[0][o][a] * A[m];
+            }
         }
-    }
+    };
+    if (xface) traces(std::true_type{});
+    else traces(std::false_type{});
     double D2t[NN];
 #pragma unroll
     for (int m = 0; m < NN; ++m) D2t[m] = (2.0 / ht) * I.D[m];

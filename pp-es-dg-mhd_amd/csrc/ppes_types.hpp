@@ -100,6 +100,7 @@ struct Shape {
     static constexpr int FACE = 4 * N;                     // face nodes per element
     static constexpr int REC = 7;                          // doubles per stored pair flux
     static constexpr int BUF = (PAIRS * REC > FACE * 11) ? PAIRS * REC : FACE * 11;
+    static constexpr int BUF_ROUND = (NN * REC > FACE * 11) ? NN * REC : FACE * 11;  // one pair round
 };
 
 __device__ __forceinline__ long long gidx(const Inst &I, int ex, int ey) { return (long long)ey * (I.nx + 2) + ex; }
