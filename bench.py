@@ -75,14 +75,16 @@ def run_batched(ppesdg, mix, steps, warmup, tsteps):
     b = ppesdg.Batch(3)
     total = warmup + steps + tsteps
     ids = [add_instance(b, m, total) for m in mix]
-    b.step(warmup, True)
+    if warmup > 0:
+        b.step(warmup, True)
     t0 = time.perf_counter()
     done, active = b.step(steps, True)
     elapsed = time.perf_counter() - t0  # step() returns after the last step's status is on the host
     if done != steps or active != len(mix):
         print(f"# warning: {done} of {steps} steps, {active} of {len(mix)} instances active", file=sys.stderr)
     b.set_timing(True)
-    b.step(tsteps, False)
+    if tsteps > 0:
+        b.step(tsteps, False)
     kt = {k: b.kernel_time(k) for k in ppesdg.KERNELS}
     b.set_timing(False)
     stats = [b.status(i) for i in ids]
@@ -152,7 +154,7 @@ def run_single(ppesdg, steps, warmup):
 
 def cpu_baseline(mix, budget_s=15.0):
     """The CPU oracle (plain-C restatement, OpenMP over elements) on a bounded
-    sample of the mixed workload: instances in mix order, start-up + 2 steps
+    sample of the mixed workload: instances in mix order, start-up + 6 steps
     each, until about budget_s seconds of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -166,14 +168,14 @@ def cpu_baseline(mix, budget_s=15.0):
         o = O.Oracle(inst["nx"], inst["ny"], inst["p"], dom, g, bc, bcs)
         o.init_problem(inst["prob"])
         t0 = time.perf_counter()
-        s = o.run(-1.0, 2, cfl, oedg, True)
+        s = o.run(-1.0, 6, cfl, oedg, True)
         spent += time.perf_counter() - t0
         work += W.nodes(inst) * (3 * s["steps"] + 1)  # the start-up counts as one stage
         used += 1
         del o
     return {"value": work / spent, "unit": "DOF-stage updates/s", "cores": cores, "kind": "port",
             "sample": f"oracle/ppes_oracle.c on the first {used} instances of the mixed workload, "
-                      f"start-up + 2 RK3 steps each, {spent:.1f} s"}
+                      f"start-up + 6 RK3 steps each, {spent:.1f} s"}
 
 
 def run_serving(duration, rate):

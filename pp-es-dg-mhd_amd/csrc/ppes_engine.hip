@@ -21,11 +21,18 @@ namespace ppes {
         }                                                                                                   \
     } while (0)
 
-// k_residual in the engine's arithmetic mode
-#define RESIDUAL_DISPATCH(p, grid, ...)                                                       \
-    do {                                                                                      \
-        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(p, LAUNCH((k_residual<P, true>), grid, __VA_ARGS__)); }  \
-        else { ENG_DISPATCH(p, LAUNCH((k_residual<P, false>), grid, __VA_ARGS__)); }         \
+// k_residual in the engine's arithmetic mode, with its own workgroup size
+#define LAUNCH_TB(kernel, grid, tb, ...)                                                       \
+    do {                                                                                       \
+        if ((grid) > 0) {                                                                      \
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(grid)), dim3(tb), 0, stream, __VA_ARGS__); \
+            ENG_TRY(hipGetLastError());                                                        \
+        }                                                                                      \
+    } while (0)
+#define RESIDUAL_DISPATCH(p, grid, ...)                                                                          \
+    do {                                                                                                         \
+        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, true>), grid, resid_tb(P), __VA_ARGS__)); } \
+        else { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, false>), grid, resid_tb(P), __VA_ARGS__)); }            \
     } while (0)
 
 #define ENG_DISPATCH(p, CALL)                     \
@@ -323,7 +330,7 @@ int Engine::active_count() const
 
 void Engine::free_group(Group &g)
 {
-    Blk *m[] = {g.d_el, g.d_slot, g.d_face, g.d_bc, g.d_red1, g.d_red2};
+    Blk *m[] = {g.d_el, g.d_slot, g.d_face, g.d_bc, g.d_red1, g.d_red2, g.d_res};
     for (Blk *p : m) if (p) (void)hipFree(p);
     if (g.d_list) (void)hipFree(g.d_list);
     g = Group();
@@ -333,11 +340,13 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
 {
     g.p = p;
     g.members = members;
-    std::vector<Blk> el, sl, fc, bc, r1, r2;
+    std::vector<Blk> el, sl, fc, bc, r1, r2, rs;
     for (int id : members) {
         const Inst &r = h_recs[id];
         const int NN = (p + 1) * (p + 1);
         for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
+        const long long nres = cdiv((long long)r.nx * r.ny, resid_tb(p) / NN);
+        for (long long b = 0; b < nres; ++b) rs.push_back({id, (int)b});
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
         const long long nf = ((long long)(r.nx + 1) * r.ny + (long long)r.nx * (r.ny + 1)) * 8;
         for (long long b = 0; b < cdiv(nf, BLOCK); ++b) fc.push_back({id, (int)b});
@@ -353,7 +362,7 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
     };
     int rc;
     if ((rc = up(&g.d_el, el)) || (rc = up(&g.d_slot, sl)) || (rc = up(&g.d_face, fc)) || (rc = up(&g.d_bc, bc)) ||
-        (rc = up(&g.d_red1, r1)) || (rc = up(&g.d_red2, r2)))
+        (rc = up(&g.d_red1, r1)) || (rc = up(&g.d_red2, r2)) || (rc = up(&g.d_res, rs)))
         return rc;
     ENG_TRY(hipMalloc((void **)&g.d_list, std::max<size_t>(members.size(), 1) * sizeof(int)));
     ENG_TRY(hipMemcpy(g.d_list, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -362,6 +371,7 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
     g.n_face = (long long)fc.size();
     g.n_bc = (long long)bc.size();
     g.n_red1 = (long long)r1.size();
+    g.n_res = (long long)rs.size();
     return PPES_OK;
 }
 
@@ -457,13 +467,13 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 1);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 1);
+            RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, 0, 1, 1);
         }
     }
     for (int st = 1; st <= S; ++st) {
         for (auto &g : groups) {
             Timed tm(this, KT_STAGE);
-            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, st, (int)F_COMBINE));
+            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
         }
         if (any_oedg_stage[st]) {
             for (auto &g : groups) {
@@ -491,7 +501,7 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 0);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, st, (st % S) + 1, 0);
+            RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, st, (st % S) + 1, 0);
         }
     }
     LAUNCH1(k_step_end, n_all, d_recs, (const int *)d_all, n_all);
@@ -577,13 +587,13 @@ int Engine::launch_startup(const std::vector<int> &ids)
         LAUNCH1(k_reset_flags, n, d_recs, (const int *)d_ids, n, 1 | 2 | 4);
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp)
-            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, 0,
+            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, 0,
                                      (int)(F_GHOST_MEM | F_FORCE_NO_OEDG | F_ENT)));
         for (auto &g : tmp) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, 0, (int)(F_FORCE_NO_OEDG | F_ENT));
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
-        for (auto &g : tmp) RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, 0, 1, 0);
+        for (auto &g : tmp) RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, 0, 1, 0);
     }
     (void)hipStreamSynchronize(stream);
     for (auto &g : tmp) free_group(g);
@@ -614,14 +624,14 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         break;
     case API_COMBINE:
-        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
+        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
         break;
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
-        RESIDUAL_DISPATCH(g.p, g.n_el, d_recs, (const Blk *)g.d_el, slot, ut_slot, 0);
+        RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, slot, ut_slot, 0);
         break;
     case API_POST: {
-        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, g.n_slot, d_recs, (const Blk *)g.d_slot, slot, flags));
+        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
         if (flags & F_FORCE_OEDG) {
             LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, slot, flags, 0);
             LAUNCH(k_reduce_global, 1, d_recs, (const Blk *)g.d_red2, slot, flags, 1);

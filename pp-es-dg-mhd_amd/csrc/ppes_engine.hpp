@@ -29,8 +29,9 @@ struct Group {
     int p = 0;
     std::vector<int> members;
     Blk *d_el = nullptr, *d_slot = nullptr, *d_face = nullptr, *d_bc = nullptr, *d_red1 = nullptr, *d_red2 = nullptr;
+    Blk *d_res = nullptr;  // element map of the residual kernel (its own workgroup size)
     int *d_list = nullptr;
-    long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0;
+    long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
 };
 
 enum KTimerId { KT_RESIDUAL = 0, KT_STAGE = 1, KT_OEDG = 2, KT_LIMIT = 3, KT_REDUCE = 4, KT_BC = 5, KT_COUNT = 6 };
@@ -62,6 +63,7 @@ public:
     int run_steps(long long nsteps, bool use_graph, long long *done_steps);
     int sync_every = 1;                                          // steps queued per host synchronisation
     int arith = PPES_ARITH_FAST;                                 // residual arithmetic (PPES_ARITH_*)
+    long long stage_blocks(int, long long nmap) const { return nmap; }  // one element group per workgroup
     void set_arith(int a) { arith = a; graph_ok = false; }
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
 

@@ -15,6 +15,7 @@ namespace ppes {
 // the residuals, resolved once per workgroup.
 struct StageSrc {
     const gdbl *un, *u1, *u2, *u3, *u4, *t, *t4;
+    const gdbl *prev;  // the earlier stage combine3 reads (u(st-1)), or nullptr
     double dt;
     int S, st;
 };
@@ -29,39 +30,52 @@ __device__ __forceinline__ StageSrc stage_src(const Inst &I, int st)
     r.u4 = gin(I.u[4 <= I.S ? 4 : 1]);
     r.t = gin(I.ut[st]);
     r.t4 = gin(I.ut[4 <= I.S ? 4 : 1]);
+    r.prev = (st >= 2 && I.S >= 2) ? gin(I.u[st - 1]) : nullptr;
     r.dt = I.ctl.dt;
     r.S = I.S;
     r.st = st;
     return r;
 }
 
-// RK stage combine of one value, in the reference's written form for each
-// method and stage. os / ot index the state / residual arrays.
-__device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long os, long long ot)
+// RK stage combine of one value from u^n, the residual and the earlier stage
+// the method uses (u1 in RK2 / RK3 stage 2, u2 in RK3 stage 3, u(st-1) in RK5
+// stages 2-4), in the reference's written form for each method and stage
+// (drive.f90:130-272). RK5 stage 5 has five sources: rk_combine_value.
+__device__ __forceinline__ double combine3(int S, int st, double dt, double un, double t, double up)
 {
-    const double dt = c.dt;
-    const double un = c.un[os], t = c.t[ot];
-    switch (c.S) {
+    switch (S) {
     case 1:
         return un + dt * t;
     case 2:
-        if (c.st == 1) return un + dt * t;
-        return 0.5 * un + 0.5 * (c.u1[os] + dt * t);
+        if (st == 1) return un + dt * t;
+        return 0.5 * un + 0.5 * (up + dt * t);
     case 3:
-        if (c.st == 1) return un + dt * t;
-        if (c.st == 2) return 0.75 * un + 0.25 * (c.u1[os] + dt * t);
-        return (1.0 / 3.0) * un + (2.0 / 3.0) * (c.u2[os] + dt * t);
+        if (st == 1) return un + dt * t;
+        if (st == 2) return 0.75 * un + 0.25 * (up + dt * t);
+        return (1.0 / 3.0) * un + (2.0 / 3.0) * (up + dt * t);
     default:
-        switch (c.st) {
+        switch (st) {
         case 1: return un + (0.391752226571890 * dt) * t;
-        case 2: return 0.444370493651235 * un + 0.555629506348765 * c.u1[os] + (0.368410593050371 * dt) * t;
-        case 3: return 0.620101851488403 * un + 0.379898148511597 * c.u2[os] + (0.251891774271694 * dt) * t;
-        case 4: return 0.178079954393132 * un + 0.821920045606868 * c.u3[os] + (0.544974750228521 * dt) * t;
-        default:
-            return 0.517231671970585 * c.u2[os] + 0.096059710526147 * c.u3[os] + (0.063692468666290 * dt) * c.t4[ot] +
-                   0.386708617503269 * c.u4[os] + (0.226007483236906 * dt) * t;
+        case 2: return 0.444370493651235 * un + 0.555629506348765 * up + (0.368410593050371 * dt) * t;
+        case 3: return 0.620101851488403 * un + 0.379898148511597 * up + (0.251891774271694 * dt) * t;
+        default: return 0.178079954393132 * un + 0.821920045606868 * up + (0.544974750228521 * dt) * t;
         }
     }
+}
+
+// the earlier stage combine3 reads (nullptr when it reads none): u1 in RK2 /
+// RK3 stage 2, u2 in RK3 stage 3, u(st-1) in RK5 stages 2-4
+__device__ __forceinline__ const gdbl *combine_prev(const StageSrc &c) { return c.prev; }
+
+// RK stage combine of one value read straight from memory. os / ot index the
+// state / residual arrays.
+__device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long os, long long ot)
+{
+    if (c.S == 5 && c.st == 5)
+        return 0.517231671970585 * c.u2[os] + 0.096059710526147 * c.u3[os] + (0.063692468666290 * c.dt) * c.t4[ot] +
+               0.386708617503269 * c.u4[os] + (0.226007483236906 * c.dt) * c.t[ot];
+    const gdbl *pv = combine_prev(c);
+    return combine3(c.S, c.st, c.dt, c.un[os], c.t[ot], pv ? pv[os] : 0.0);
 }
 
 // --------------------------------------------------------------------------
@@ -83,14 +97,14 @@ __device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long 
 #define PPES_FAST_ROUND 1
 #endif
 
-template <int P, bool FAST>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FAST ? PPES_FAST_WAVES : 1))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
+template <int P, bool FAST, int TB = resid_tb(P)>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? PPES_FAST_WAVES : 1))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
     using S = Shape<P>;
-    constexpr int N = S::N, NN = S::NN, EPB = S::EPB, REC = S::REC;
+    constexpr int N = S::N, NN = S::NN, EPB = TB / NN, REC = S::REC;
     __shared__ Prim sprim[EPB][NN];
     __shared__ double sbuf[EPB][(FAST && PPES_FAST_ROUND) ? S::BUF_ROUND : S::BUF];
-    __shared__ double sred[BLOCK][2];
+    __shared__ double sred[TB][2];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -209,6 +223,36 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FAST
     // y-pairs (ll, ss), ss < ll, of column kc
     auto pair_flux = [&](int qp, double *o) {
         double ec[8];
+        if constexpr (FAST) {
+            // one evaluation for x- and y-pairs alike: a y-pair is the x-flux
+            // of the x/y-swapped primitives, its output swapped back
+            const bool ydir = qp >= NN * P / 2;
+            const int qq = ydir ? qp - NN * P / 2 : qp;
+            const int line = qq / S::H, a = qq - line * S::H;
+            int hi = 1;
+            while ((hi + 1) * hi / 2 <= a) ++hi;
+            const int lo = a - hi * (hi - 1) / 2;
+            const int nA = ydir ? line + N * hi : hi + N * line;
+            const int nB2 = ydir ? line + N * lo : lo + N * line;
+            Prim A = sprim[el][nA], B = sprim[el][nB2];
+            if (ydir) {
+                double x;
+                x = A.vx; A.vx = A.vy; A.vy = x;
+                x = A.bx; A.bx = A.by; A.by = x;
+                x = B.vx; B.vx = B.vy; B.vy = x;
+                x = B.bx; B.bx = B.by; B.by = x;
+            }
+            ec_flux_fast<0>(A, B, inv_gm1, ec);
+            // x-record (0,1,2,3,5,6,7); y-record (0,1,2,3,4,6,7) of the swapped-back flux
+            o[0] = ec[0];
+            o[1] = ydir ? ec[2] : ec[1];
+            o[2] = ydir ? ec[1] : ec[2];
+            o[3] = ec[3];
+            o[4] = ec[5];
+            o[5] = ec[6];
+            o[6] = ec[7];
+            return;
+        }
         if (qp < NN * P / 2) {
             const int lr = qp / S::H, a = qp - lr * S::H;
             int kk = 1;
@@ -285,19 +329,16 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FAST
     for (int r = 0; r < FR; ++r) {
         const int f = n + r * NN;
         if (valid && f < S::FACE) {
-            const int side = f / N;
-            double wl[8], wr[8], nf[8], nB, sl, sr;
-            if (side < 2) {
-                if constexpr (FAST) espp_hll_flux_x_fast(tl[r], tr[r], g, inv_gm1, nf, nB, sl, sr);
-                else espp_hll_flux_x(tl[r], tr[r], g, nf, nB, sl, sr);
-            } else {
-                double tf[8];
-                swap_xy(tl[r], wl);
-                swap_xy(tr[r], wr);
-                if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
-                else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
-                swap_xy(tf, nf);
-            }
+            // y-faces go through the x-flux with the x/y components swapped
+            // (swap_xy); the swap is a selection per component, so lanes of
+            // x- and y-faces share one evaluation of the flux
+            const bool yf = f / N >= 2;
+            double wl[8], wr[8], tf[8], nf[8], nB, sl, sr;
+            swap_xy_if(yf, tl[r], wl);
+            swap_xy_if(yf, tr[r], wr);
+            if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
+            else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
+            swap_xy_if(yf, tf, nf);
             double *o = &sbuf[el][f * 11];
 #pragma unroll
             for (int v = 0; v < 8; ++v) o[v] = nf[v];
@@ -483,22 +524,30 @@ __device__ __forceinline__ double entropy_partial(const double q[8], bool valid,
 // Ghost strips are synthesised from the BC of the combined source elements,
 // exactly what BC_handle would copy (problems.f90:312-348,386-423,597-654),
 // or read from memory (F_GHOST_MEM).
+
 template <int P>
-__global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map, int st, int flags)
+__global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
 {
     using S = Shape<P>;
     constexpr int N = S::N, NN = S::NN, EPB = S::EPB;
     constexpr int M = 2 * NN;
-    __shared__ double scons[EPB][8][NN];
+    // scons (node values: averages, OEDG extrema), sdb (LDF input) and sred
+    // (limiter, entropy) are never live at the same time: one region, which
+    // leaves room for a fifth workgroup per CU
+    constexpr int U1 = EPB * 8 * NN, U2 = EPB * M, U3 = BLOCK * 2;
+    constexpr int UN = U1 > U2 ? (U1 > U3 ? U1 : U3) : (U2 > U3 ? U2 : U3);
+    __shared__ double sunion[UN];
+    double(*scons)[8][NN] = reinterpret_cast<double(*)[8][NN]>(sunion);
+    double(*sdb)[M] = reinterpret_cast<double(*)[M]>(sunion);
+    double(*sred)[2] = reinterpret_cast<double(*)[2]>(sunion);
     __shared__ double savg[EPB][8];
-    __shared__ double sdb[EPB][M];
     __shared__ double sldf[M * M];
     __shared__ int sok[EPB];
     __shared__ int scnt[EPB];
-    __shared__ double sred[BLOCK][2];
     __shared__ double sent[EPB];
     __shared__ double sem[3][EPB][8];
 
+    if ((int)blockIdx.x >= nmap) return;
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I)) return;
@@ -600,9 +649,10 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
             q[4] = bxs + avg[4];
             q[5] = bys + avg[5];
         }
+        __syncthreads();  // sdb is read; the region is reused for sred / scons
     }
     if (valid && ok)
-        for (int v = n; v < 8; v += NN) gout(I.uavg)[eg * 8 + v] = avg[v];
+        for (int v = n; v < 8; v += NN) gout(I.uavg)[eg * 8 + v] = savg[el][v];
 
     if (stage_oedg(I, st, flags)) {
         // post-LDF state (ghost strips too: the OEDG face jumps read them)

@@ -323,6 +323,13 @@ __device__ __forceinline__ void swap_xy(const double a[8], double b[8])
     b[4] = a[5]; b[5] = a[4]; b[6] = a[6]; b[7] = a[7];
 }
 
+// swap_xy when `on`, else a copy (per-component selects, no branch)
+__device__ __forceinline__ void swap_xy_if(bool on, const double a[8], double b[8])
+{
+    b[0] = a[0]; b[1] = on ? a[2] : a[1]; b[2] = on ? a[1] : a[2]; b[3] = a[3];
+    b[4] = on ? a[5] : a[4]; b[5] = on ? a[4] : a[5]; b[6] = a[6]; b[7] = a[7];
+}
+
 // fast magnetosonic spectral radius in x and y at one node
 // (DG_solver.f90:749-770 and :456-486)
 __device__ __forceinline__ void node_speeds(double rho, double mx, double my, double mz, double bx, double by,

@@ -90,6 +90,15 @@ struct Blk {
     int inst, local;
 };
 
+// Workgroup size of the residual kernel: 256 lanes by default; smaller
+// workgroups leave fewer waves waiting at each barrier (P4 keeps at least 128
+// so that five 25-lane elements fill it).
+#ifndef PPES_RESID_TB
+#define PPES_RESID_TB 64
+#endif
+constexpr int resid_tb(int p) { return (p == 4 && PPES_RESID_TB < 128) ? 128 : PPES_RESID_TB; }
+
+
 template <int P>
 struct Shape {
     static constexpr int N = P + 1;

@@ -53,9 +53,37 @@ def stats(d, title, command):
         if base in CLASS_OF:
             cls[CLASS_OF[base]][0] += float(r["TotalDurationNs"])
             cls[CLASS_OF[base]][1] += int(r["Calls"])
-    out += ["", "| class | launches | total (ms) | avg per launch (us) |", "|---|---|---|---|"]
+    out += ["", "All launches of a class (start-up and single-instance launches, and retry launches whose",
+            "workgroups exit at once, included):", "",
+            "| class | launches | total (ms) | avg per launch (us) |", "|---|---|---|---|"]
     for k, (t, n) in cls.items():
         out.append(f"| {k} | {n} | {t / 1e6:.2f} | {t / n / 1e3:.1f} |")
+    # full launches only: the largest grid of each kernel (one launch covers a
+    # whole degree group) and not an immediate exit (over 5% of the longest)
+    tr = list(csv.DictReader(open(one_csv(d, "*kernel_trace.csv"))))
+    per = collections.defaultdict(list)
+    for r in tr:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        per[name].append((int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]), dur))
+    out += ["", "Full launches (largest grid of each kernel, duration over 5% of the longest): the launches",
+            "bench.py times; the class average is the mean over the degree groups, as each group has",
+            "the same number of launches per step.", "",
+            "| kernel | full launches | avg (us) |", "|---|---|---|"]
+    cls_full = collections.defaultdict(list)
+    for name, v in sorted(per.items()):
+        base = name.split("<")[0].replace("ppes::", "")
+        if base not in CLASS_OF:
+            continue
+        g = max(x[0] for x in v)
+        dmax = max(x[1] for x in v if x[0] == g)
+        full = [x[1] for x in v if x[0] == g and x[1] > 0.05 * dmax]
+        avg = sum(full) / len(full)
+        cls_full[CLASS_OF[base]].append(avg)
+        out.append(f"| {name} | {len(full)} | {avg / 1e3:.1f} |")
+    out += ["", "| class | avg per full launch over degree groups (us) |", "|---|---|"]
+    for k, v in cls_full.items():
+        out.append(f"| {k} | {sum(v) / len(v) / 1e3:.1f} |")
     return "\n".join(out) + "\n"
 
 
@@ -85,11 +113,11 @@ def traffic(a):
     per_degree = {}
     step_read = step_write = 0.0
     for p in (1, 2, 3, 4):
-        name = f"ppes::{kname}<{p}>"
-        keys = [k for k in F if k[0] == name]
+        keys = [k for k in F if k[0] == f"ppes::{kname}<{p}>" or k[0].startswith(f"ppes::{kname}<{p},")]
         if not keys:
             continue
-        g = max(k[1] for k in keys)
+        name = keys[0][0]
+        g = max(k[1] for k in keys if k[0] == name)
         f, w = F[(name, g)], Wr[(name, g)]
         fmax, wmax = max(f), max(w)
         f = [x for x in f if x > 0.02 * fmax]
