@@ -184,6 +184,8 @@ int Engine::add(const ppes_config &cfg, int *id)
     if (cfg.rk_stages != S) return PPES_BAD_ARG;
     if (!(cfg.right > cfg.left) || !(cfg.top > cfg.bottom) || !(cfg.gamma > 1.0)) return PPES_BAD_ARG;
     if (cfg.bc_kind < PPES_BC_PERIODIC || cfg.bc_kind > PPES_BC_HOST) return PPES_BAD_ARG;
+    // kernels index one variable plane with 32-bit offsets
+    if ((long long)(cfg.nx + 2) * (cfg.ny + 2) * (cfg.px + 1) * (cfg.px + 1) >= (1LL << 31)) return PPES_UNSUPPORTED;
     int slot = -1;
     for (size_t i = 0; i < hosts.size(); ++i)
         if (!hosts[i]) { slot = (int)i; break; }

@@ -88,7 +88,7 @@ __device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long 
 // Build-time shape of the fast residual: waves per SIMD asked of the register
 // allocator, early trace loads, one-round pair buffer.
 #ifndef PPES_FAST_WAVES
-#define PPES_FAST_WAVES 1
+#define PPES_FAST_WAVES 3
 #endif
 #ifndef PPES_FAST_PREFETCH
 #define PPES_FAST_PREFETCH 0
@@ -98,7 +98,7 @@ __device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long 
 #endif
 
 template <int P, bool FAST, int TB = resid_tb(P)>
-__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? PPES_FAST_WAVES : 1))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST && P == 4 ? PPES_FAST_WAVES : 1))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
     using S = Shape<P>;
     constexpr int N = S::N, NN = S::NN, EPB = TB / NN, REC = S::REC;
@@ -115,14 +115,22 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
     const int k = n % N, l = n / N;
-    const long long eg = (long long)bm.local * EPB + el;
-    const bool valid = (el < EPB) && (eg < (long long)I.nx * I.ny);
-    const int i = valid ? (int)(eg % I.nx) + 1 : 1;
-    const int j = valid ? (int)(eg / I.nx) + 1 : 1;
+    // 32-bit element and node indices (one variable plane holds < 2^31
+    // doubles, which the host checks); 64-bit arithmetic only in the uniform
+    // plane base addresses
+    const unsigned eg = (unsigned)bm.local * EPB + el;
+    const unsigned nx = (unsigned)I.nx, nx2 = nx + 2;
+    const bool valid = (el < EPB) && (eg < nx * (unsigned)I.ny);
+    const int i = valid ? (int)(eg % nx) + 1 : 1;
+    const int j = valid ? (int)(eg / nx) + 1 : 1;
     const double g = I.g, hx = I.hx, hy = I.hy;
     const double inv_gm1 = 1.0 / (g - 1.0);
     const long long vs = I.vs;
-    const long long base = gidx(I, i, j) * NN + n;
+    const gdbl *up[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) up[v] = u + v * vs;
+    auto node = [&](int ex, int ey) { return ((unsigned)ey * nx2 + (unsigned)ex) * NN; };
+    const unsigned base = node(i, j) + n;
 
     // ---- loads: the node's state, and the face traces this lane's face
     // fluxes need. The exact form loads the traces here (used only after the
@@ -136,7 +144,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
     double tl[FR][8], tr[FR][8];
     if (valid) {
 #pragma unroll
-        for (int v = 0; v < 8; ++v) q[v] = u[v * vs + base];
+        for (int v = 0; v < 8; ++v) q[v] = up[v][base];
     }
     auto load_traces = [&]() {
 #pragma unroll
@@ -144,16 +152,16 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
             const int f = n + r * NN;
             if (valid && f < S::FACE) {
                 const int side = f / N, a = f - side * N;
-                long long b1, b2;
+                unsigned b1, b2;
                 if (side < 2) {
-                    b1 = gidx(I, side == 0 ? i - 1 : i, j) * NN + (N - 1) + N * a;
-                    b2 = gidx(I, side == 0 ? i : i + 1, j) * NN + N * a;
+                    b1 = node(side == 0 ? i - 1 : i, j) + (N - 1) + N * a;
+                    b2 = node(side == 0 ? i : i + 1, j) + N * a;
                 } else {
-                    b1 = gidx(I, i, side == 2 ? j - 1 : j) * NN + a + N * (N - 1);
-                    b2 = gidx(I, i, side == 2 ? j : j + 1) * NN + a;
+                    b1 = node(i, side == 2 ? j - 1 : j) + a + N * (N - 1);
+                    b2 = node(i, side == 2 ? j : j + 1) + a;
                 }
 #pragma unroll
-                for (int v = 0; v < 8; ++v) { tl[r][v] = u[v * vs + b1]; tr[r][v] = u[v * vs + b2]; }
+                for (int v = 0; v < 8; ++v) { tl[r][v] = up[v][b1]; tr[r][v] = up[v][b2]; }
             }
         }
     };
@@ -322,6 +330,13 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
     }
     __syncthreads();  // pair records consumed; the buffer now holds face data
 
+    // acc waits in LDS (in the place of the primitives, which are no longer
+    // read) while the face fluxes take the registers
+    double(*sacc)[8] = reinterpret_cast<double(*)[8]>(&sprim[0][0].rho);
+    if (valid)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) sacc[t][v] = acc[v];
+
     // ---- face fluxes of the element's four faces; both neighbours compute a
     // shared face from identical inputs, so they agree bit for bit
     if constexpr (!PREFETCH) load_traces();
@@ -333,27 +348,33 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
             // (swap_xy); the swap is a selection per component, so lanes of
             // x- and y-faces share one evaluation of the flux
             const bool yf = f / N >= 2;
-            double wl[8], wr[8], tf[8], nf[8], nB, sl, sr;
+            double wl[8], wr[8], tf[8], nB, sl, sr;
             swap_xy_if(yf, tl[r], wl);
             swap_xy_if(yf, tr[r], wr);
             if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
             else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
-            swap_xy_if(yf, tf, nf);
-            double *o = &sbuf[el][f * 11];
-#pragma unroll
-            for (int v = 0; v < 8; ++v) o[v] = nf[v];
+            double *o = &sbuf[el][f * S::FREC];
+            swap_xy_if(yf, tf, o);
             o[8] = nB; o[9] = sl; o[10] = sr;
         }
     }
     __syncthreads();
 
-    // ---- surface terms, x-faces then y-faces (DG_solver.f90:872-882)
+    // ---- surface terms, x-faces then y-faces (DG_solver.f90:872-882). The
+    // node's state is read again (a cache hit) rather than kept in registers
+    // through the flux phases, its velocity recomputed as the primitives
+    // were, and acc comes back from LDS.
     double ax = msx, ay = msy;
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acc[v] = sacc[t][v];
+    }
     if (valid && (k == 0 || k == N - 1 || l == 0 || l == N - 1)) {
-        const Prim &pr = sprim[el][n];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) q[v] = up[v][base];
         const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
-        const double vx = pr.vx, vy = pr.vy, vz = pr.vz;
         const double inv_rho = 1.0 / q[0];
+        const double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
         const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
         const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
         const double b_sq = bx * bx + by * by + bz * bz;
@@ -365,7 +386,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
         if (k == 0 || k == N - 1) {
             const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
                                  0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
-            const double *o = &sbuf[el][((k == 0) ? 0 : 1) * N * 11 + l * 11];
+            const double *o = &sbuf[el][(((k == 0) ? 0 : 1) * N + l) * S::FREC];
             const double c = (k == 0) ? (2.0 / hx) / I.qw[0] : (2.0 / hx) / I.qw[N - 1];
             if (k == 0) {
 #pragma unroll
@@ -380,7 +401,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
         if (l == 0 || l == N - 1) {
             const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
                                  ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
-            const double *o = &sbuf[el][((l == 0) ? 2 : 3) * N * 11 + k * 11];
+            const double *o = &sbuf[el][(((l == 0) ? 2 : 3) * N + k) * S::FREC];
             const double c = (l == 0) ? (2.0 / hy) / I.qw[0] : (2.0 / hy) / I.qw[N - 1];
             if (l == 0) {
 #pragma unroll
@@ -394,9 +415,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST ? 
         }
     }
     if (valid) {
-        const long long ob = ((long long)(j - 1) * I.nx + (i - 1)) * NN + n;
+        const unsigned ob = eg * NN + n;
+        const long long vsi = I.vsi;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) ut[v * I.vsi + ob] = acc[v];
+        for (int v = 0; v < 8; ++v) (ut + v * vsi)[ob] = acc[v];
     }
     // ---- element dt_lim (DG_solver.f90:806); max is exact in any order
     sred[t][0] = ax;

@@ -108,8 +108,9 @@ struct Shape {
     static constexpr int PAIRS = NN * P;                   // x- plus y-pairs per element
     static constexpr int FACE = 4 * N;                     // face nodes per element
     static constexpr int REC = 7;                          // doubles per stored pair flux
-    static constexpr int BUF = (PAIRS * REC > FACE * 11) ? PAIRS * REC : FACE * 11;
-    static constexpr int BUF_ROUND = (NN * REC > FACE * 11) ? NN * REC : FACE * 11;  // one pair round
+    static constexpr int FREC = 11;                        // doubles per face node: flux, B_n*, sl, sr
+    static constexpr int BUF = (PAIRS * REC > FACE * FREC) ? PAIRS * REC : FACE * FREC;
+    static constexpr int BUF_ROUND = (NN * REC > FACE * FREC) ? NN * REC : FACE * FREC;  // one pair round
 };
 
 __device__ __forceinline__ long long gidx(const Inst &I, int ex, int ey) { return (long long)ey * (I.nx + 2) + ex; }
