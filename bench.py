@@ -247,7 +247,7 @@ def main():
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(instances shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "(a batch would shard across GPUs by whole instances, with no communication)", file=sys.stderr)
         sys.exit(2)
 
     import ppesdg  # raises if the HIP library is missing: there is no fallback

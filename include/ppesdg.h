@@ -38,6 +38,9 @@ extern "C" {
 #define PPES_BC_EXTRAP 1    /* BC_YeeSjogreen   problems.f90:386-423 (trace extrapolation) */
 #define PPES_BC_FIXED 2     /* BC_rotor         problems.f90:597-654 (fixed conservative state) */
 #define PPES_BC_HOST 3      /* ghosts supplied by the caller through ppes_upload_state */
+#define PPES_BC_BRIOWU 4    /* BC_BrioWu        problems.f90:350-384 (x: trace extrapolation, y: periodic) */
+#define PPES_BC_CLOUD 5     /* BC_cloud_shock   problems.f90:425-472 (element copy; inflow state on the right) */
+#define PPES_BC_JET 6       /* BC_jet1_cell_ext problems.f90:474-595 (mirror at left; nozzle on bottom cells 1..nx/10) */
 
 /* built-in initial data (nodal point sampling as init_by_fun_pri, problems.f90:25-51) */
 #define PPES_PROB_OT2PI 0     /* Orszag-Tang on [0,2pi]^2: rho=g^2, p=g, u=(-sin y, sin x), B=(-sin y, sin 2x) */
@@ -46,6 +49,13 @@ extern "C" {
 #define PPES_PROB_OTUNIT 3    /* OrszagTang_init_pri problems.f90:155-168 */
 #define PPES_PROB_BLASTREF 4  /* Blast_init_pri     problems.f90:201-218 */
 #define PPES_PROB_OTPERT 5    /* OT2PI plus a smooth perturbation (pert table, amplitude) */
+#define PPES_PROB_ALFVEN 6    /* smooth_alfven_init_fun problems.f90:74-98 */
+#define PPES_PROB_KHI 7       /* KHI_init_pri          problems.f90:100-113 */
+#define PPES_PROB_BRIOWU 8    /* Brio_Wu_init_fun      problems.f90:115-138 */
+#define PPES_PROB_WAVE1D 9    /* smooth_1Dwave_init_fun problems.f90:140-153 */
+#define PPES_PROB_CLOUD 10    /* cloud_shock_init_pri  problems.f90:220-247 */
+#define PPES_PROB_JET 11      /* jet1_init_pri         problems.f90:249-257 */
+#define PPES_PROB_YEE 12      /* YeeSjogreenInitCon    problems.f90:259-308 (conservative, init_by_fun_con) */
 
 typedef struct ppes_config {
     int nx, ny;            /* elements */
@@ -54,7 +64,8 @@ typedef struct ppes_config {
     double left, right, bottom, top;
     double gamma;
     int bc_kind;           /* PPES_BC_* */
-    double bc_state[8];    /* conservative state for PPES_BC_FIXED */
+    double bc_state[8];    /* conservative state for PPES_BC_FIXED (the CLOUD and JET
+                              states are built in, from gamma) */
 } ppes_config;
 
 typedef struct ppes_solver ppes_solver;
@@ -77,7 +88,8 @@ int ppes_destroy(ppes_solver *s);
 int ppes_set_gamma(ppes_solver *s, double gamma);
 
 /* choose the boundary kind after creation (the BC_handle binding of
- * drive.f90:25); state = conservative ghost state for PPES_BC_FIXED */
+ * drive.f90:25); state = conservative ghost state for PPES_BC_FIXED, else
+ * ignored (may be NULL) */
 int ppes_set_bc(ppes_solver *s, int kind, const double *state);
 
 /* geometry (DG_solver.f90:19-24,93-106): hx, hy, grid_x(nx+1), grid_y(ny+1),
@@ -98,6 +110,14 @@ int ppes_init_problem(ppes_solver *s, int prob, const double *pert, int nk, doub
 
 /* BC_handle on a state slot (drive.f90:68,277,291) for the device BC kinds */
 int ppes_apply_bc(ppes_solver *s, int slot);
+
+/* compute_error_smooth_alfven (problems.f90:659-697): L2 and max error of
+ * B_perp in state slot `slot` against the initial (period-1) wave.
+ * compute_error_smooth1D (problems.f90:699-742): L2 and max error of the 8
+ * conservative variables against the density wave advected to time T.
+ * Both sum in the reference's serial order. */
+int ppes_error_smooth_alfven(ppes_solver *s, int slot, double *err_l2, double *err_max);
+int ppes_error_smooth1d(ppes_solver *s, int slot, double T, double *err_l2, double *err_max);
 
 /* post_process (DG_solver.f90:128-661) on a state slot. ent_offset must be 0
  * (the reference's option is off in every shipped case). Returns

@@ -14,8 +14,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libppes_oracle.so")
 
-BC = {"periodic": 0, "extrap": 1, "fixed": 2}
-PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5}
+BC = {"periodic": 0, "extrap": 1, "fixed": 2, "briowu": 4, "cloud": 5, "jet": 6}
+PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5, "alfven": 6, "khi": 7,
+        "briowu": 8, "wave1d": 9, "cloud": 10, "jet": 11, "yee": 12}
 
 _lib = None
 
@@ -55,6 +56,8 @@ def lib():
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(Summary)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_error_smooth_alfven.argtypes = [ctypes.c_void_p, dp, dp, dp]
+        L.orc_error_smooth1d.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, dp, dp]
         L.orc_log_avg_divisor.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_log_avg_divisor.restype = ctypes.c_double
         _lib = L
@@ -169,6 +172,20 @@ class Oracle:
         dtl = ctypes.c_double()
         lib().orc_semidiscretize(self.h, _dp(ut), ctypes.byref(dtl), _dp(arr))
         return dtl.value
+
+    def error_smooth_alfven(self, arr=None):
+        """compute_error_smooth_alfven (problems.f90:659-697) -> (l2, max)"""
+        u = np.asfortranarray(self.u if arr is None else arr, dtype=np.float64)
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        lib().orc_error_smooth_alfven(self.h, _dp(u), ctypes.byref(l2), ctypes.byref(mx))
+        return l2.value, mx.value
+
+    def error_smooth1d(self, T, arr=None):
+        """compute_error_smooth1D (problems.f90:699-742) -> (l2, max)"""
+        u = np.asfortranarray(self.u if arr is None else arr, dtype=np.float64)
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        lib().orc_error_smooth1d(self.h, _dp(u), float(T), ctypes.byref(l2), ctypes.byref(mx))
+        return l2.value, mx.value
 
     def run(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1, use_ldf=True):
         s = Summary()

@@ -11,6 +11,7 @@
  * Reductions whose order OpenMP would change (u_avg_global, total entropy) are
  * summed serially in the reference's serial order.
  */
+#define _GNU_SOURCE /* sincos */
 #include "ppes_oracle.h"
 
 #include <math.h>
@@ -550,12 +551,84 @@ static inline void swap_xy(const double *a, double *b)
 
 /* ---------------------------------------------------------- boundaries */
 
+/* BC_cloud_shock inflow and BC_jet1_cell_ext nozzle states
+ * (problems.f90:443-449,490-496), through MHD_pri2con */
+static void cloud_inflow_state(double g, double *uc)
+{
+    const double up[8] = {1.0, -11.2536, 0.0, 0.0, 0.0, 0.56418958, 0.56418958, 1.0};
+    pri2con(up, uc, g);
+}
+static void jet_nozzle_state(double g, double *uc)
+{
+    const double up[8] = {g, 0.0, 800.0, 0.0, 0.0, sqrt(200.0), 0.0, 1.0};
+    pri2con(up, uc, g);
+}
+
 /* BC_periodic / BC_YeeSjogreen / BC_rotor (problems.f90:312-348,386-423,597-654);
- * ORC_BC_FIXED writes bc_state (conservative) into every ghost node. */
+ * ORC_BC_FIXED writes bc_state (conservative) into every ghost node.
+ * BC_BrioWu (:350-384), BC_cloud_shock (:425-472), BC_jet1_cell_ext (:474-595). */
 void orc_apply_bc(orc_solver *s, double *u)
 {
     int nx = s->nx, ny = s->ny, N = s->N;
     size_t elem = (size_t)8 * N * N;
+    if (s->bc_kind == ORC_BC_BRIOWU) {
+        /* x: trace extrapolation, y: periodic */
+        for (int j = 1; j <= ny; ++j)
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k)
+                    for (int v = 0; v < 8; ++v) {
+                        u[UIDX(s, v, k, l, 0, j)] = u[UIDX(s, v, 0, l, 1, j)];
+                        u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, N - 1, l, nx, j)];
+                    }
+        for (int i = 1; i <= nx; ++i) {
+            memcpy(u + UIDX(s, 0, 0, 0, i, 0), u + UIDX(s, 0, 0, 0, i, ny), elem * sizeof(double));
+            memcpy(u + UIDX(s, 0, 0, 0, i, ny + 1), u + UIDX(s, 0, 0, 0, i, 1), elem * sizeof(double));
+        }
+        return;
+    }
+    if (s->bc_kind == ORC_BC_CLOUD) {
+        double inflow[8];
+        cloud_inflow_state(s->gamma, inflow);
+        for (int j = 1; j <= ny; ++j) {
+            memcpy(u + UIDX(s, 0, 0, 0, 0, j), u + UIDX(s, 0, 0, 0, 1, j), elem * sizeof(double));
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k)
+                    for (int v = 0; v < 8; ++v) u[UIDX(s, v, k, l, nx + 1, j)] = inflow[v];
+        }
+        for (int i = 1; i <= nx; ++i) {
+            memcpy(u + UIDX(s, 0, 0, 0, i, 0), u + UIDX(s, 0, 0, 0, i, 1), elem * sizeof(double));
+            memcpy(u + UIDX(s, 0, 0, 0, i, ny + 1), u + UIDX(s, 0, 0, 0, i, ny), elem * sizeof(double));
+        }
+        return;
+    }
+    if (s->bc_kind == ORC_BC_JET) {
+        double nozzle[8];
+        jet_nozzle_state(s->gamma, nozzle);
+        for (int j = 1; j <= ny; ++j)
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k) {
+                    /* mirror symmetry at x = left: normal momentum and normal B change sign */
+                    const int km = N - 1 - k;
+                    u[UIDX(s, 0, k, l, 0, j)] = u[UIDX(s, 0, km, l, 1, j)];
+                    u[UIDX(s, 1, k, l, 0, j)] = -u[UIDX(s, 1, km, l, 1, j)];
+                    u[UIDX(s, 2, k, l, 0, j)] = u[UIDX(s, 2, km, l, 1, j)];
+                    u[UIDX(s, 3, k, l, 0, j)] = u[UIDX(s, 3, km, l, 1, j)];
+                    u[UIDX(s, 4, k, l, 0, j)] = -u[UIDX(s, 4, km, l, 1, j)];
+                    u[UIDX(s, 5, k, l, 0, j)] = u[UIDX(s, 5, km, l, 1, j)];
+                    u[UIDX(s, 6, k, l, 0, j)] = u[UIDX(s, 6, km, l, 1, j)];
+                    u[UIDX(s, 7, k, l, 0, j)] = u[UIDX(s, 7, km, l, 1, j)];
+                    for (int v = 0; v < 8; ++v) u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, k, l, nx, j)];
+                }
+        for (int i = 1; i <= nx; ++i)
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k)
+                    for (int v = 0; v < 8; ++v) {
+                        /* nozzle inflow over the first nx/10 bottom cells */
+                        u[UIDX(s, v, k, l, i, 0)] = (i <= nx / 10) ? nozzle[v] : u[UIDX(s, v, k, l, i, 1)];
+                        u[UIDX(s, v, k, l, i, ny + 1)] = u[UIDX(s, v, k, l, i, ny)];
+                    }
+        return;
+    }
     if (s->bc_kind == ORC_BC_PERIODIC) {
         for (int j = 1; j <= ny; ++j) {
             memcpy(u + UIDX(s, 0, 0, 0, 0, j), u + UIDX(s, 0, 0, 0, nx, j), elem * sizeof(double));
@@ -665,10 +738,90 @@ static void pri_otpert(double x, double y, double g, const double *pert, int nk,
     up[5] += amp * dby / nk;
 }
 
-/* init_by_fun_pri (problems.f90:25-51) */
+/* smooth_alfven_init_fun (problems.f90:74-98) */
+static void pri_alfven(double x, double y, double *up)
+{
+    const double pi = 3.14159265358979323846;
+    const double alpha = pi / 6.0;
+    const double x_para = x * cos(alpha) + y * sin(alpha);
+    /* sin and cos of the one phase come from one sincos call, as in the
+     * reference's compiled code (the two can differ in the last bit from
+     * separate sin and cos calls) */
+    double sp, cp;
+    sincos(2 * pi * x_para, &sp, &cp);
+    const double v_perp = 0.1 * sp;
+    const double B_perp = v_perp, B_para = 1.0;
+    const double uz = 0.1 * cp, Bz = uz;
+    up[0] = 1.0;
+    up[1] = v_perp * (-sin(alpha));
+    up[2] = v_perp * cos(alpha);
+    up[3] = uz;
+    up[4] = B_para * cos(alpha) - B_perp * sin(alpha);
+    up[5] = B_para * sin(alpha) + B_perp * cos(alpha);
+    up[6] = Bz;
+    up[7] = 0.1;
+}
+/* KHI_init_pri (problems.f90:100-113) */
+static void pri_khi(double x, double y, double *up)
+{
+    const double pi = 3.14159265358979323846;
+    up[0] = 1.0;
+    up[1] = 5.0 * ((tanh(20 * (y + 0.5)) - tanh(20 * (y - 0.5))) - 1.0);
+    up[2] = (0.25 * sin(2 * pi * x)) * (exp(-(100 * ((y + 0.5) * (y + 0.5)))) - exp(-(100 * ((y - 0.5) * (y - 0.5)))));
+    up[3] = 0.0; up[4] = 1.0; up[5] = 0.0; up[6] = 0.0; up[7] = 50.0;
+}
+/* Brio_Wu_init_fun (problems.f90:115-138) */
+static void pri_briowu(double x, double y, double *up)
+{
+    (void)y;
+    if (x < 0.0) {
+        up[0] = 1.0; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0; up[4] = 0.75; up[5] = 1.0; up[6] = 0.0; up[7] = 1.0;
+    } else {
+        up[0] = 0.125; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0; up[4] = 0.75; up[5] = -1.0; up[6] = 0.0; up[7] = 0.1;
+    }
+}
+/* smooth_1Dwave_init_fun (problems.f90:140-153) */
+static void pri_wave1d(double x, double y, double *up)
+{
+    (void)y;
+    up[0] = 1.0 + 0.2 * sin(x);
+    up[1] = 1.0; up[2] = 0.0; up[3] = 0.0; up[4] = 0.5; up[5] = 1.0; up[6] = 1.5; up[7] = 2.0;
+}
+/* cloud_shock_init_pri (problems.f90:220-247), the original formulation */
+static void pri_cloud(double x, double y, double *up)
+{
+    const double x0 = 0.8, y0 = 0.5, r0 = 0.15;
+    if (x < 0.6) {
+        const double a[8] = {3.86859, 0.0, 0.0, 0.0, 0.0, 2.1826182, -2.1826182, 167.345};
+        memcpy(up, a, sizeof a);
+    } else {
+        const double a[8] = {1.0, -11.2536, 0.0, 0.0, 0.0, 0.56418958, 0.56418958, 1.0};
+        memcpy(up, a, sizeof a);
+        if ((x - x0) * (x - x0) + (y - y0) * (y - y0) <= r0 * r0) up[0] = 10.0;
+    }
+}
+/* jet1_init_pri (problems.f90:249-257): the ambient state */
+static void pri_jet(double g, double *up)
+{
+    up[0] = 0.1 * g; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0;
+    up[4] = 0.0; up[5] = sqrt(200.0); up[6] = 0.0; up[7] = 1.0;
+}
+/* YeeSjogreenInitCon (problems.f90:259-308): conservative variables */
+static void con_yee(double x, double y, double *uc)
+{
+    static const double q1[8] = {0.9308, 1.4557, -0.4633, 0.0575, 0.3501, 0.9830, 0.3050, 5.0838};
+    static const double q2[8] = {1.0304, 1.5774, -1.0455, -0.1016, 0.3501, 0.5078, 0.1576, 5.7813};
+    static const double q3[8] = {1.0000, 1.7500, -1.0000, 0.0000, 0.5642, 0.5078, 0.2539, 6.0000};
+    static const double q4[8] = {1.8887, 0.2334, -1.7422, 0.0733, 0.5642, 0.9830, 0.4915, 12.999};
+    const double *q = (y >= 0.0) ? ((x >= 0.0) ? q1 : q2) : ((x <= 0.0) ? q3 : q4);
+    memcpy(uc, q, 8 * sizeof(double));
+}
+
+/* init_by_fun_pri / init_by_fun_con (problems.f90:25-72) */
 int orc_init_problem(orc_solver *s, int prob, const double *pert, int nk, double amp)
 {
     int N = s->N;
+    if (prob < ORC_PROB_OT2PI || prob > ORC_PROB_YEE) return ORC_BAD_ARG;
     for (int j = 1; j <= s->ny; ++j)
         for (int i = 1; i <= s->nx; ++i)
             for (int l = 0; l < N; ++l)
@@ -681,12 +834,68 @@ int orc_init_problem(orc_solver *s, int prob, const double *pert, int nk, double
                     case ORC_PROB_OTUNIT: pri_otunit(x, y, up); break;
                     case ORC_PROB_BLASTREF: pri_blastref(x, y, up); break;
                     case ORC_PROB_OTPERT: pri_otpert(x, y, s->gamma, pert, nk, amp, up); break;
-                    default: return ORC_BAD_ARG;
+                    case ORC_PROB_ALFVEN: pri_alfven(x, y, up); break;
+                    case ORC_PROB_KHI: pri_khi(x, y, up); break;
+                    case ORC_PROB_BRIOWU: pri_briowu(x, y, up); break;
+                    case ORC_PROB_WAVE1D: pri_wave1d(x, y, up); break;
+                    case ORC_PROB_CLOUD: pri_cloud(x, y, up); break;
+                    case ORC_PROB_JET: pri_jet(s->gamma, up); break;
+                    default: con_yee(x, y, uc); break;
                     }
-                    pri2con(up, uc, s->gamma);
+                    if (prob != ORC_PROB_YEE) pri2con(up, uc, s->gamma);
                     for (int v = 0; v < 8; ++v) s->u[UIDX(s, v, k, l, i, j)] = uc[v];
                 }
     return ORC_OK;
+}
+
+/* compute_error_smooth_alfven (problems.f90:659-697): L2 and max error of
+ * B_perp against the exact travelling wave (period 1: the state at t = 5 is
+ * the initial one), serial order */
+void orc_error_smooth_alfven(const orc_solver *s, const double *u, double *err_l2, double *err_max)
+{
+    const double pi = 3.14159265358979323846, alpha = pi / 6.0;
+    int N = s->N;
+    double l2 = 0.0, mx = 0.0;
+    for (int j = 1; j <= s->ny; ++j)
+        for (int i = 1; i <= s->nx; ++i)
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k) {
+                    const double x = s->xx[k + N * (i - 1)], y = s->yy[l + N * (j - 1)];
+                    const double x_para = x * cos(alpha) + y * sin(alpha);
+                    const double B_perp = 0.1 * sin(2 * pi * x_para);
+                    const double Bh = u[UIDX(s, 4, k, l, i, j)] * (-sin(alpha)) + u[UIDX(s, 5, k, l, i, j)] * cos(alpha);
+                    const double d = Bh - B_perp;
+                    l2 = l2 + ((((0.25 * s->hx) * s->hy) * s->qw[k]) * s->qw[l]) * (d * d);
+                    mx = dmax(mx, fabs(d));
+                }
+    *err_l2 = sqrt(l2);
+    *err_max = mx;
+}
+
+/* compute_error_smooth1D (problems.f90:699-742): all 8 conservative
+ * variables against the exact density wave at time T, serial order */
+void orc_error_smooth1d(const orc_solver *s, const double *u, double T, double *err_l2, double *err_max)
+{
+    int N = s->N;
+    double l2 = 0.0, mx = 0.0;
+    for (int j = 1; j <= s->ny; ++j)
+        for (int i = 1; i <= s->nx; ++i)
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k) {
+                    const double x = s->xx[k + N * (i - 1)];
+                    double up[8] = {1.0 + 0.2 * sin(x - T), 1.0, 0.0, 0.0, 0.5, 1.0, 1.5, 2.0}, ex[8];
+                    pri2con(up, ex, s->gamma);
+                    double sq = 0.0, am = 0.0;
+                    for (int v = 0; v < 8; ++v) {
+                        const double d = u[UIDX(s, v, k, l, i, j)] - ex[v];
+                        sq = sq + d * d;
+                        am = dmax(am, fabs(d));
+                    }
+                    l2 = l2 + ((((0.25 * s->hx) * s->hy) * s->qw[k]) * s->qw[l]) * sq;
+                    mx = dmax(mx, am);
+                }
+    *err_l2 = sqrt(l2);
+    *err_max = mx;
 }
 
 /* ------------------------------------------------------- post_process */

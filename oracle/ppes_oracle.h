@@ -19,10 +19,12 @@
 extern "C" {
 #endif
 
-enum { ORC_BC_PERIODIC = 0, ORC_BC_EXTRAP = 1, ORC_BC_FIXED = 2 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_EXTRAP = 1, ORC_BC_FIXED = 2, ORC_BC_BRIOWU = 4, ORC_BC_CLOUD = 5,
+       ORC_BC_JET = 6 };
 enum { ORC_OK = 0, ORC_NEG_AVG = 1, ORC_DT_COLLAPSE = 2, ORC_PP_FAIL = 3, ORC_BAD_ARG = 4 };
 enum { ORC_PROB_OT2PI = 0, ORC_PROB_ROTOR = 1, ORC_PROB_BLAST = 2, ORC_PROB_OTUNIT = 3,
-       ORC_PROB_BLASTREF = 4, ORC_PROB_OTPERT = 5 };
+       ORC_PROB_BLASTREF = 4, ORC_PROB_OTPERT = 5, ORC_PROB_ALFVEN = 6, ORC_PROB_KHI = 7,
+       ORC_PROB_BRIOWU = 8, ORC_PROB_WAVE1D = 9, ORC_PROB_CLOUD = 10, ORC_PROB_JET = 11, ORC_PROB_YEE = 12 };
 
 typedef struct orc_solver orc_solver;
 
@@ -56,6 +58,10 @@ const double *orc_yy(orc_solver *s);     /* (N, ny) */
 int orc_init_problem(orc_solver *s, int prob, const double *pert, int nk, double amp);
 
 void orc_apply_bc(orc_solver *s, double *u);
+
+/* compute_error_smooth_alfven / compute_error_smooth1D (problems.f90:659-742) */
+void orc_error_smooth_alfven(const orc_solver *s, const double *u, double *err_l2, double *err_max);
+void orc_error_smooth1d(const orc_solver *s, const double *u, double T, double *err_l2, double *err_max);
 int orc_post_process(orc_solver *s, double *u, double *u_avg, int apply_oedg, int apply_ldf,
                      double oedg_dt, int comput_ent, int *neg_avg, int *num_negcell,
                      double *tot_ent);

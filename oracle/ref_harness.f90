@@ -6,8 +6,9 @@
 !   init_DG_solver / post_process / DG_semidiscretize / finalize_DG_solver
 !   (DG_solver.f90:47,128,663,109), solve_execute (drive.f90:32), the plug-in
 !   pointers Init_handle / BC_handle (drive.f90:24-25), init_by_fun_pri
-!   (problems.f90:25), BC_periodic / BC_YeeSjogreen / BC_rotor
-!   (problems.f90:312,386,597), LDF_projector%init/apply (LDF.f90:114,219).
+!   / init_by_fun_con (problems.f90:25,53), the shipped initial data and
+!   boundary fillers (problems.f90:74-654), the error functionals
+!   (problems.f90:659-742), LDF_projector%init/apply (LDF.f90:114,219).
 ! Its step loop restates drive.f90:83-320 so that every RK stage can be
 ! dumped; mode=solve calls solve_execute itself for end-to-end checks.
 !
@@ -129,6 +130,7 @@ program ref_harness
 	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3, plt = 0
 	real :: cfl = 0.2, tfinal = -1.0, oedg_dt_call = 0.01
 	real :: left, right, bottom, top, gam
+	real :: err2(2)
 	logical :: have_init_file
 
 	call parse_args()
@@ -159,6 +161,14 @@ program ref_harness
 		! the reference's Tecplot writer on the final averages (main.f90:67); its
 		! solve_execute has written entropy.plt (drive.f90:338-341)
 		if (plt == 1) call save_tecbin_finite_volume(u_avg, nx_, ny_, grid_x, grid_y, 0, tfinal)
+		! the error functionals of the smooth problems (main.f90:78-80,91-93)
+		if (trim(prob) == 'alfven') then
+			call compute_error_smooth_alfven(u_con, hx, hy, xx, yy, nx, ny, px, py, err2(1), err2(2))
+			call dump_real('err', err2, [2])
+		else if (trim(prob) == 'wave1d') then
+			call compute_error_smooth1D(u_con, hx, hy, xx, yy, nx, ny, px, py, tfinal, err2(1), err2(2))
+			call dump_real('err', err2, [2])
+		end if
 	case ('calls')
 		call run_calls()
 	case default
@@ -240,6 +250,49 @@ contains
 			init_pri_fun => Blast_init_pri
 			Init_handle => init_by_fun_pri
 			if (trim(bc) == 'auto') bc = 'periodic'
+		! the rest of the reference's problem catalogue (main.f90:57-190)
+		case ('alfven')
+			left = 0.0; right = 1.0/cos(pi/6.0); bottom = 0.0; top = 1.0/sin(pi/6.0)
+			call set_gamma(5.0/3.0)
+			init_pri_fun => smooth_alfven_init_fun
+			Init_handle => init_by_fun_pri
+			if (trim(bc) == 'auto') bc = 'periodic'
+		case ('khi')
+			left = 0.0; right = 1.0; bottom = -1.0; top = 1.0
+			call set_gamma(1.4)
+			init_pri_fun => KHI_init_pri
+			Init_handle => init_by_fun_pri
+			if (trim(bc) == 'auto') bc = 'periodic'
+		case ('briowu')
+			left = -1.0; right = 1.0; bottom = -0.5; top = 0.5
+			call set_gamma(5.0/3.0)
+			init_pri_fun => Brio_Wu_init_fun
+			Init_handle => init_by_fun_pri
+			if (trim(bc) == 'auto') bc = 'briowu'
+		case ('wave1d')
+			left = 0.0; right = 2*pi; bottom = -0.5; top = 0.5
+			call set_gamma(5.0/3.0)
+			init_pri_fun => smooth_1Dwave_init_fun
+			Init_handle => init_by_fun_pri
+			if (trim(bc) == 'auto') bc = 'periodic'
+		case ('cloud')
+			left = 0.0; right = 1.0; bottom = 0.0; top = 1.0
+			call set_gamma(5.0/3.0)
+			init_pri_fun => cloud_shock_init_pri
+			Init_handle => init_by_fun_pri
+			if (trim(bc) == 'auto') bc = 'cloud'
+		case ('jet')
+			left = 0.0; right = 0.5; bottom = 0.0; top = 1.5
+			call set_gamma(1.4)
+			init_pri_fun => jet1_init_pri
+			Init_handle => init_by_fun_pri
+			if (trim(bc) == 'auto') bc = 'jet'
+		case ('yee')
+			left = -1.0; right = 1.0; bottom = -1.0; top = 1.0
+			call set_gamma(5.0/3.0)
+			init_con_fun => YeeSjogreenInitCon
+			Init_handle => init_by_fun_con
+			if (trim(bc) == 'auto') bc = 'extrap'
 		case ('file')
 			! domain and gamma come from the command line
 			call set_gamma(gam)
@@ -252,6 +305,9 @@ contains
 		case ('periodic'); BC_handle => BC_periodic
 		case ('extrap'); BC_handle => BC_YeeSjogreen
 		case ('rotor'); BC_handle => BC_rotor
+		case ('briowu'); BC_handle => BC_BrioWu
+		case ('cloud'); BC_handle => BC_cloud_shock
+		case ('jet'); BC_handle => BC_jet1_cell_ext
 		case default; error stop 'harness: unknown bc'
 		end select
 	end subroutine setup_problem

@@ -178,12 +178,75 @@ int Engine::sync()
     return PPES_OK;
 }
 
+// MHD_pri2con (MHD_PDE.f90:58-70)
+static void pri2con_host(const double *up, double *uc, double g)
+{
+    uc[0] = up[0];
+    uc[1] = up[0] * up[1];
+    uc[2] = up[0] * up[2];
+    uc[3] = up[0] * up[3];
+    uc[4] = up[4];
+    uc[5] = up[5];
+    uc[6] = up[6];
+    uc[7] = up[7] / (g - 1.0) + 0.5 * (up[0] * (up[1] * up[1] + up[2] * up[2] + up[3] * up[3]) +
+                                        (up[4] * up[4] + up[5] * up[5] + up[6] * up[6]));
+}
+
+int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g)
+{
+    int side[4];
+    int nfix[4] = {0, 0, 0, 0};
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    switch (kind) {
+    case PPES_BC_PERIODIC:  // BC_periodic (problems.f90:312-348)
+    case PPES_BC_HOST:      // ghosts come from the caller; periodic rules are never used
+        for (int q = 0; q < 4; ++q) side[q] = SIDE_PERIODIC;
+        break;
+    case PPES_BC_EXTRAP:  // BC_YeeSjogreen (problems.f90:386-423)
+        for (int q = 0; q < 4; ++q) side[q] = SIDE_TRACE;
+        break;
+    case PPES_BC_FIXED:  // BC_rotor (problems.f90:597-654) with the caller's state
+        if (!state) return PPES_BAD_ARG;
+        for (int q = 0; q < 4; ++q) side[q] = SIDE_FIXED;
+        for (int v = 0; v < 8; ++v) st[v] = state[v];
+        break;
+    case PPES_BC_BRIOWU:  // BC_BrioWu (problems.f90:350-384)
+        side[0] = side[1] = SIDE_TRACE;
+        side[2] = side[3] = SIDE_PERIODIC;
+        break;
+    case PPES_BC_CLOUD: {  // BC_cloud_shock (problems.f90:425-472): inflow from the right
+        const double up[8] = {1.0, -11.2536, 0.0, 0.0, 0.0, 0.56418958, 0.56418958, 1.0};
+        pri2con_host(up, st, g);
+        side[0] = side[2] = side[3] = SIDE_COPY;
+        side[1] = SIDE_FIXED;
+        break;
+    }
+    case PPES_BC_JET: {  // BC_jet1_cell_ext (problems.f90:474-595): nozzle on bottom cells 1..nx/10
+        const double up[8] = {g, 0.0, 800.0, 0.0, 0.0, sqrt(200.0), 0.0, 1.0};
+        pri2con_host(up, st, g);
+        side[0] = SIDE_MIRROR;
+        side[1] = side[2] = side[3] = SIDE_COPY;
+        nfix[2] = nx / 10;
+        break;
+    }
+    default:
+        return PPES_BAD_ARG;
+    }
+    r.bc_kind = kind;
+    for (int q = 0; q < 4; ++q) {
+        r.side[q] = side[q];
+        r.nfix[q] = nfix[q];
+    }
+    for (int v = 0; v < 8; ++v) r.bc_state[v] = st[v];
+    return PPES_OK;
+}
+
 int Engine::add(const ppes_config &cfg, int *id)
 {
     if (cfg.nx < 1 || cfg.ny < 1 || cfg.px != cfg.py || cfg.px < 1 || cfg.px > 4) return PPES_BAD_ARG;
     if (cfg.rk_stages != S) return PPES_BAD_ARG;
     if (!(cfg.right > cfg.left) || !(cfg.top > cfg.bottom) || !(cfg.gamma > 1.0)) return PPES_BAD_ARG;
-    if (cfg.bc_kind < PPES_BC_PERIODIC || cfg.bc_kind > PPES_BC_HOST) return PPES_BAD_ARG;
+    if (cfg.bc_kind < PPES_BC_PERIODIC || cfg.bc_kind > PPES_BC_JET) return PPES_BAD_ARG;
     // kernels index one variable plane with 32-bit offsets
     if ((long long)(cfg.nx + 2) * (cfg.ny + 2) * (cfg.px + 1) * (cfg.px + 1) >= (1LL << 31)) return PPES_UNSUPPORTED;
     int slot = -1;
@@ -259,8 +322,10 @@ int Engine::add(const ppes_config &cfg, int *id)
                 for (int b = 0; b < N; ++b) Pw[a][b] = T[a][b];
         }
     }
-    r.bc_kind = cfg.bc_kind;
-    for (int v = 0; v < 8; ++v) r.bc_state[v] = cfg.bc_state[v];
+    if (set_bc_rules(r, cfg.bc_kind, cfg.bc_state, nx, cfg.gamma) != PPES_OK) {
+        delete h;
+        return PPES_BAD_ARG;
+    }
     const long long E = epb_of(h->p);
     r.nblk_el = cdiv((long long)nx * ny, E);
     r.nblk_slot = cdiv((long long)(nx + 2) * (ny + 2), E);
@@ -338,6 +403,25 @@ void Engine::free_group(Group &g)
     g = Group();
 }
 
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs,
+// each with its own L2. Kernels that read neighbouring elements (the
+// residual's face traces, the OEDG face indicators) want neighbours on one
+// XCD: reorder a block map so that XCD x runs one contiguous run of logical
+// blocks, in order (dispatch slot w -> XCD w % 8, its (w / 8)-th block).
+constexpr int N_XCD = 8;
+
+static void xcd_order(std::vector<Blk> &v)
+{
+    const size_t n = v.size(), q = n / N_XCD, r = n % N_XCD;
+    if (q == 0) return;
+    std::vector<Blk> out(n);
+    for (size_t w = 0; w < n; ++w) {
+        const size_t x = w % N_XCD;
+        out[w] = v[x * q + std::min(x, r) + w / N_XCD];
+    }
+    v.swap(out);
+}
+
 int Engine::build_group(Group &g, const std::vector<int> &members, int p)
 {
     g.p = p;
@@ -357,6 +441,8 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
         for (long long b = 0; b < cdiv(r.nblk_slot, BLOCK); ++b) r1.push_back({id, (int)b});
         r2.push_back({id, 0});
     }
+    xcd_order(rs);
+    xcd_order(fc);
     auto up = [&](Blk **d, const std::vector<Blk> &h) -> int {
         ENG_TRY(hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(Blk)));
         if (!h.empty()) ENG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(Blk), hipMemcpyHostToDevice));

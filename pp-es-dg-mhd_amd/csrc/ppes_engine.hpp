@@ -103,4 +103,10 @@ private:
 
 enum ApiCall { API_BC = 0, API_POST = 1, API_RESID = 2, API_COMBINE = 3, API_GHOST_SAVE = 4, API_RESET = 5 };
 
+// Side rules, nozzle count and fixed state of a PPES_BC_* kind (the BC_handle
+// fillers of problems.f90:312-654) for an instance with nx columns and ratio
+// of specific heats g. Returns PPES_BAD_ARG for an unknown kind or a missing
+// PPES_BC_FIXED state.
+int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g);
+
 }  // namespace ppes

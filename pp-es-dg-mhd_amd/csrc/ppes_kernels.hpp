@@ -537,6 +537,49 @@ __device__ __forceinline__ double entropy_partial(const double q[8], bool valid,
 }
 
 // --------------------------------------------------------------------------
+// Where ghost node (ex, ey, k, l) takes its value from under the instance's
+// side rules: the BC_handle fillers of problems.f90:312-654 each read only
+// interior elements, so every ghost node is one interior node (possibly
+// reflected) or the fixed state.
+struct GhostSrc {
+    int sx, sy, sn;
+    int mode;  // 0 the value, 1 the fixed state, 2 reflected in x (m_x, B_x negated), 3 reflected in y
+};
+
+template <int N>
+__device__ inline GhostSrc ghost_src(const Inst &I, int ex, int ey, int k, int l)
+{
+    int side, along;
+    if (ex == 0) { side = 0; along = ey; }
+    else if (ex == I.nx + 1) { side = 1; along = ey; }
+    else if (ey == 0) { side = 2; along = ex; }
+    else { side = 3; along = ex; }
+    const int rule = I.side[side];
+    GhostSrc g{ex, ey, k + N * l, 0};
+    if (rule == SIDE_FIXED || along <= I.nfix[side]) {
+        g.mode = 1;
+        return g;
+    }
+    const bool xs = side < 2, lo = (side & 1) == 0;
+    if (xs) g.sx = (rule == SIDE_PERIODIC) == lo ? I.nx : 1;
+    else g.sy = (rule == SIDE_PERIODIC) == lo ? I.ny : 1;
+    if (rule == SIDE_TRACE) {
+        g.sn = xs ? (lo ? 0 : N - 1) + N * l : k + N * (lo ? 0 : N - 1);
+    } else if (rule == SIDE_MIRROR) {
+        g.sn = xs ? (N - 1 - k) + N * l : k + N * (N - 1 - l);
+        g.mode = xs ? 2 : 3;
+    }
+    return g;
+}
+
+// the value a reflected ghost node takes from its source value
+__device__ __forceinline__ double ghost_value(int mode, int v, double x)
+{
+    const bool flip = (mode == 2 && (v == 1 || v == 4)) || (mode == 3 && (v == 2 || v == 5));
+    return flip ? -x : x;
+}
+
+// --------------------------------------------------------------------------
 // Stage post-process over the interior and the four ghost strips:
 // [RK combine] -> cell averages and the inadmissible-average flag -> LDF
 // projection -> either (PP path) the positivity limiter, entropy and the
@@ -604,21 +647,18 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
             const long long gt = ((long long)(ey - 1) * I.nx + (ex - 1)) * NN + n;
 #pragma unroll
             for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(src, v * vs + base, v * I.vsi + gt);
-        } else if (I.bc_kind == BC_FIXED) {
-#pragma unroll
-            for (int v = 0; v < 8; ++v) q[v] = I.bc_state[v];
         } else {
-            // source element and node of this ghost node
-            int sx, sy, sn;
-            const bool per = I.bc_kind == BC_PERIODIC;
-            if (ex == 0) { sx = per ? I.nx : 1; sy = ey; sn = per ? n : 0 + N * l; }
-            else if (ex == I.nx + 1) { sx = per ? 1 : I.nx; sy = ey; sn = per ? n : (N - 1) + N * l; }
-            else if (ey == 0) { sx = ex; sy = per ? I.ny : 1; sn = per ? n : k + N * 0; }
-            else { sx = ex; sy = per ? 1 : I.ny; sn = per ? n : k + N * (N - 1); }
-            const long long gs = gidx(I, sx, sy) * NN + sn;
-            const long long gt = ((long long)(sy - 1) * I.nx + (sx - 1)) * NN + sn;
+            const GhostSrc gsrc = ghost_src<N>(I, ex, ey, k, l);
+            if (gsrc.mode == 1) {
 #pragma unroll
-            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(src, v * vs + gs, v * I.vsi + gt);
+                for (int v = 0; v < 8; ++v) q[v] = I.bc_state[v];
+            } else {
+                const long long gs = gidx(I, gsrc.sx, gsrc.sy) * NN + gsrc.sn;
+                const long long gt = ((long long)(gsrc.sy - 1) * I.nx + (gsrc.sx - 1)) * NN + gsrc.sn;
+#pragma unroll
+                for (int v = 0; v < 8; ++v)
+                    q[v] = ghost_value(gsrc.mode, v, rk_combine_value(src, v * vs + gs, v * I.vsi + gt));
+            }
         }
         if (flags & F_COMBINE_ONLY) {
             if (interior)
@@ -1012,9 +1052,9 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_ent(Inst *insts, const int *li
     if (t == 0) I.ctl.tot_ent = ss[0];
 }
 
-// Ghost strips of state slot `slot` (BC_periodic / BC_YeeSjogreen /
-// BC_rotor-type fixed state, problems.f90:312-348,386-423,597-654); one
-// thread per (ghost element, node)
+// Ghost strips of state slot `slot` by the instance's side rules (the
+// BC_handle fillers of problems.f90:312-654); one thread per (ghost element,
+// node)
 template <int P>
 __global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int slot)
 {
@@ -1029,25 +1069,21 @@ __global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int s
     const int n = (int)(tid % NN);
     const long long s = tid / NN;
     const int k = n % N, l = n / N;
-    const bool per = I.bc_kind == BC_PERIODIC;
-    int gx, gy, sx, sy, sn;
-    if (s < I.ny) {
-        const int j = (int)s + 1;
-        gx = 0; gy = j; sx = per ? I.nx : 1; sy = j; sn = per ? n : 0 + N * l;
-    } else if (s < 2LL * I.ny) {
-        const int j = (int)(s - I.ny) + 1;
-        gx = I.nx + 1; gy = j; sx = per ? 1 : I.nx; sy = j; sn = per ? n : (N - 1) + N * l;
-    } else if (s < 2LL * I.ny + I.nx) {
-        const int i = (int)(s - 2LL * I.ny) + 1;
-        gx = i; gy = 0; sx = i; sy = per ? I.ny : 1; sn = per ? n : k + N * 0;
-    } else {
-        const int i = (int)(s - 2LL * I.ny - I.nx) + 1;
-        gx = i; gy = I.ny + 1; sx = i; sy = per ? 1 : I.ny; sn = per ? n : k + N * (N - 1);
-    }
+    int gx, gy;
+    if (s < I.ny) { gx = 0; gy = (int)s + 1; }
+    else if (s < 2LL * I.ny) { gx = I.nx + 1; gy = (int)(s - I.ny) + 1; }
+    else if (s < 2LL * I.ny + I.nx) { gx = (int)(s - 2LL * I.ny) + 1; gy = 0; }
+    else { gx = (int)(s - 2LL * I.ny - I.nx) + 1; gy = I.ny + 1; }
+    const GhostSrc g = ghost_src<N>(I, gx, gy, k, l);
     const long long dst = gidx(I, gx, gy) * NN + n;
-    const long long src = gidx(I, sx, sy) * NN + sn;
+    if (g.mode == 1) {
 #pragma unroll
-    for (int v = 0; v < 8; ++v) u[v * I.vs + dst] = (I.bc_kind == BC_FIXED) ? I.bc_state[v] : u[v * I.vs + src];
+        for (int v = 0; v < 8; ++v) u[v * I.vs + dst] = I.bc_state[v];
+    } else {
+        const long long src = gidx(I, g.sx, g.sy) * NN + g.sn;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) u[v * I.vs + dst] = ghost_value(g.mode, v, u[v * I.vs + src]);
+    }
 }
 
 // Copy the ghost strips of u^n into (dir 0) or out of (dir 1) the saved

@@ -51,10 +51,13 @@ namespace ppes {
 // (problems.f90:25-51, MHD_PDE.f90:58-70). Transcendental functions of x and
 // of y are tabulated on the host (per GLL x- and y-coordinate), so the device
 // only combines them with correctly rounded operations.
+// A non-separable function of x and y (the Alfven wave's phase) comes as a
+// per-node table `tn`.
 template <int P>
 __global__ void __launch_bounds__(BLOCK) k_init(Inst *insts, int id, int prob, const double *__restrict__ tx,
-                                                const double *__restrict__ ty, const double *__restrict__ modes,
-                                                int nk, double amp, double c0, double c1, double c2)
+                                                const double *__restrict__ ty, const double *__restrict__ tn,
+                                                const double *__restrict__ modes, int nk, double amp, double c0,
+                                                double c1, double c2)
 {
     constexpr int N = P + 1, NN = N * N;
     const Inst &I = insts[id];
@@ -114,13 +117,61 @@ __global__ void __launch_bounds__(BLOCK) k_init(Inst *insts, int id, int prob, c
         up[4] = c0; up[5] = c0; up[6] = 0.0;
         up[7] = (x * x + y * y < 0.01) ? 1000.0 : 0.1;
         break;
-    default:  // PPES_PROB_BLASTREF
+    case PPES_PROB_BLASTREF:
         up[0] = 1.0; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0;
         up[4] = c0; up[5] = 0.0; up[6] = 0.0;
         up[7] = ((x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) < 0.01) ? 1000.0 : 0.1;
         break;
+    case PPES_PROB_ALFVEN: {  // c0 = sin(alpha), c1 = cos(alpha); tn = sin, cos of 2 pi x_para
+        const double v_perp = 0.1 * tn[2 * tid], uz = 0.1 * tn[2 * tid + 1];
+        up[0] = 1.0; up[1] = v_perp * (-c0); up[2] = v_perp * c1; up[3] = uz;
+        up[4] = 1.0 * c1 - v_perp * c0; up[5] = 1.0 * c0 + v_perp * c1; up[6] = uz; up[7] = 0.1;
+        break;
+    }
+    case PPES_PROB_KHI:
+        up[0] = 1.0; up[1] = 5.0 * ((ty[NY + yi] - ty[2 * NY + yi]) - 1.0);
+        up[2] = (0.25 * tx[NX + xi]) * (ty[3 * NY + yi] - ty[4 * NY + yi]);
+        up[3] = 0.0; up[4] = 1.0; up[5] = 0.0; up[6] = 0.0; up[7] = 50.0;
+        break;
+    case PPES_PROB_BRIOWU: {
+        const bool left = x < 0.0;
+        up[0] = left ? 1.0 : 0.125; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0;
+        up[4] = 0.75; up[5] = left ? 1.0 : -1.0; up[6] = 0.0; up[7] = left ? 1.0 : 0.1;
+        break;
+    }
+    case PPES_PROB_WAVE1D:
+        up[0] = 1.0 + 0.2 * tx[NX + xi]; up[1] = 1.0; up[2] = 0.0; up[3] = 0.0;
+        up[4] = 0.5; up[5] = 1.0; up[6] = 1.5; up[7] = 2.0;
+        break;
+    case PPES_PROB_CLOUD:
+        if (x < 0.6) {
+            up[0] = 3.86859; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0;
+            up[4] = 0.0; up[5] = 2.1826182; up[6] = -2.1826182; up[7] = 167.345;
+        } else {
+            up[0] = ((x - 0.8) * (x - 0.8) + (y - 0.5) * (y - 0.5) <= 0.15 * 0.15) ? 10.0 : 1.0;
+            up[1] = -11.2536; up[2] = 0.0; up[3] = 0.0;
+            up[4] = 0.0; up[5] = 0.56418958; up[6] = 0.56418958; up[7] = 1.0;
+        }
+        break;
+    case PPES_PROB_JET:  // c0 = sqrt(200)
+        up[0] = 0.1 * g; up[1] = 0.0; up[2] = 0.0; up[3] = 0.0;
+        up[4] = 0.0; up[5] = c0; up[6] = 0.0; up[7] = 1.0;
+        break;
+    default:  // PPES_PROB_YEE: conservative variables of the quadrant
+        break;
     }
     double uc[8];
+    if (prob == PPES_PROB_YEE) {
+        const double q[4][8] = {{0.9308, 1.4557, -0.4633, 0.0575, 0.3501, 0.9830, 0.3050, 5.0838},
+                                {1.0304, 1.5774, -1.0455, -0.1016, 0.3501, 0.5078, 0.1576, 5.7813},
+                                {1.0000, 1.7500, -1.0000, 0.0000, 0.5642, 0.5078, 0.2539, 6.0000},
+                                {1.8887, 0.2334, -1.7422, 0.0733, 0.5642, 0.9830, 0.4915, 12.999}};
+        const int quad = (y >= 0.0) ? ((x >= 0.0) ? 0 : 1) : ((x <= 0.0) ? 2 : 3);
+        const long long b = gidx(I, i, j) * NN + n;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) I.u[0][v * I.vs + b] = q[quad][v];
+        return;
+    }
     uc[0] = up[0];
     uc[1] = up[0] * up[1];
     uc[2] = up[0] * up[2];
@@ -179,18 +230,18 @@ static int inst_download_avg(Engine &e, int id, double *u_avg)
     return PPES_OK;
 }
 
-// Init_handle => <problem>_init (problems.f90:25-51,155-218) on the interior
-// of slot 0
+// Init_handle => init_by_fun_pri / init_by_fun_con with the shipped initial
+// data (problems.f90:25-308) on the interior of slot 0
 static int inst_init_problem(Engine &e, int id, int prob, const double *pert, int nk, double amp)
 {
-    if (!e.valid(id) || prob < PPES_PROB_OT2PI || prob > PPES_PROB_OTPERT) return PPES_BAD_ARG;
+    if (!e.valid(id) || prob < PPES_PROB_OT2PI || prob > PPES_PROB_YEE) return PPES_BAD_ARG;
     if (prob == PPES_PROB_OTPERT && (!pert || nk < 1)) return PPES_BAD_ARG;
     if (prob != PPES_PROB_OTPERT) nk = 0;
     HostInst &h = e.host(id);
     const double pi = 3.14159265358979323846;
     const size_t NX = h.xx.size(), NY = h.yy.size();
-    const int sx = 3 + 2 * nk, sy = 2 + 4 * nk;
-    std::vector<double> tx(NX * sx, 0.0), ty(NY * sy, 0.0), modes(2 * std::max(nk, 1), 0.0);
+    const int sx = 3 + 2 * nk, sy = std::max(2 + 4 * nk, 5);
+    std::vector<double> tx(NX * sx, 0.0), ty(NY * sy, 0.0), modes(2 * std::max(nk, 1), 0.0), tn(2, 0.0);
     double c0 = 0.0, c1 = 0.0, c2 = 0.0;
     for (size_t a = 0; a < NX; ++a) tx[a] = h.xx[a];
     for (size_t a = 0; a < NY; ++a) ty[a] = h.yy[a];
@@ -222,26 +273,59 @@ static int inst_init_problem(Engine &e, int id, int prob, const double *pert, in
         c0 = 5.0 / sqrt(4 * pi);
     } else if (prob == PPES_PROB_BLAST) {
         c0 = (100.0 / sqrt(4.0 * pi)) / sqrt(2.0);
-    } else {
+    } else if (prob == PPES_PROB_BLASTREF) {
         c0 = 100.0 / sqrt(4 * pi);
+    } else if (prob == PPES_PROB_ALFVEN) {
+        // smooth_alfven_init_fun (problems.f90:74-98): the phase 2 pi x_para per node
+        const double alpha = pi / 6.0;
+        c0 = sin(alpha);
+        c1 = cos(alpha);
+        const int N = h.N, NN = h.NN, nx = h.cfg.nx, ny = h.cfg.ny;
+        tn.assign(2 * (size_t)nx * ny * NN, 0.0);
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                for (int n = 0; n < NN; ++n) {
+                    const double x = h.xx[n % N + (size_t)N * (i - 1)], y = h.yy[n / N + (size_t)N * (j - 1)];
+                    const double x_para = x * cos(alpha) + y * sin(alpha);
+                    const size_t t = ((size_t)(j - 1) * nx + (i - 1)) * NN + n;
+                    // one sincos call, as the reference's compiled code makes for the
+                    // sin and cos of one argument (separate calls can differ in the last bit)
+                    sincos(2 * pi * x_para, &tn[2 * t], &tn[2 * t + 1]);
+                }
+    } else if (prob == PPES_PROB_KHI) {
+        for (size_t a = 0; a < NX; ++a) tx[NX + a] = sin(2 * pi * h.xx[a]);
+        for (size_t a = 0; a < NY; ++a) {
+            const double y = h.yy[a];
+            ty[NY + a] = tanh(20 * (y + 0.5));
+            ty[2 * NY + a] = tanh(20 * (y - 0.5));
+            ty[3 * NY + a] = exp(-(100 * ((y + 0.5) * (y + 0.5))));
+            ty[4 * NY + a] = exp(-(100 * ((y - 0.5) * (y - 0.5))));
+        }
+    } else if (prob == PPES_PROB_WAVE1D) {
+        for (size_t a = 0; a < NX; ++a) tx[NX + a] = sin(h.xx[a]);
+    } else if (prob == PPES_PROB_JET) {
+        c0 = sqrt(200.0);
     }
     int rc = e.pull();
     if (rc) return rc;
-    double *d_tx = nullptr, *d_ty = nullptr, *d_modes = nullptr;
+    double *d_tx = nullptr, *d_ty = nullptr, *d_tn = nullptr, *d_modes = nullptr;
     HIP_TRY(hipMalloc((void **)&d_tx, tx.size() * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&d_ty, ty.size() * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&d_tn, tn.size() * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&d_modes, modes.size() * sizeof(double)));
     HIP_TRY(hipMemcpyAsync(d_tx, tx.data(), tx.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     HIP_TRY(hipMemcpyAsync(d_ty, ty.data(), ty.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIP_TRY(hipMemcpyAsync(d_tn, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     HIP_TRY(hipMemcpyAsync(d_modes, modes.data(), modes.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     const long long nthreads = (long long)h.cfg.nx * h.cfg.ny * h.NN;
     DISPATCH_P(h.p, hipLaunchKernelGGL(k_init<P>, dim3((unsigned)cdiv(nthreads, BLOCK)), dim3(BLOCK), 0, e.stream,
                                        e.d_recs, id, prob, (const double *)d_tx, (const double *)d_ty,
-                                       (const double *)d_modes, nk, amp, c0, c1, c2));
+                                       (const double *)d_tn, (const double *)d_modes, nk, amp, c0, c1, c2));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e.stream));
     HIP_TRY(hipFree(d_tx));
     HIP_TRY(hipFree(d_ty));
+    HIP_TRY(hipFree(d_tn));
     HIP_TRY(hipFree(d_modes));
     return PPES_OK;
 }
@@ -432,21 +516,27 @@ extern "C" int ppes_set_gamma(ppes_solver *s, double gamma)
     if (!s || !(gamma > 1.0)) return PPES_BAD_ARG;
     int rc = s->eng.pull();
     if (rc) return rc;
-    s->eng.rec(s->id).g = gamma;
-    s->eng.host(s->id).cfg.gamma = gamma;
+    Inst &r = s->eng.rec(s->id);
+    HostInst &h = s->eng.host(s->id);
+    r.g = gamma;
+    h.cfg.gamma = gamma;
+    // the built-in ghost states of the cloud-shock and jet fillers depend on gamma
+    if (r.bc_kind == PPES_BC_CLOUD || r.bc_kind == PPES_BC_JET) set_bc_rules(r, r.bc_kind, nullptr, r.nx, gamma);
     return s->eng.push();
 }
 
 extern "C" int ppes_set_bc(ppes_solver *s, int kind, const double *state)
 {
-    if (!s || kind < PPES_BC_PERIODIC || kind > PPES_BC_HOST) return PPES_BAD_ARG;
+    if (!s || kind < PPES_BC_PERIODIC || kind > PPES_BC_JET) return PPES_BAD_ARG;
     if (kind == PPES_BC_FIXED && !state) return PPES_BAD_ARG;
     int rc = s->eng.pull();
     if (rc) return rc;
     Inst &r = s->eng.rec(s->id);
-    r.bc_kind = kind;
-    for (int v = 0; v < 8; ++v) r.bc_state[v] = state ? state[v] : 0.0;
-    s->eng.host(s->id).cfg.bc_kind = kind;
+    rc = set_bc_rules(r, kind, state, r.nx, r.g);
+    if (rc) return rc;
+    HostInst &h = s->eng.host(s->id);
+    h.cfg.bc_kind = kind;
+    for (int v = 0; v < 8; ++v) h.cfg.bc_state[v] = r.bc_state[v];
     return s->eng.push();
 }
 
@@ -500,6 +590,69 @@ extern "C" int ppes_apply_bc(ppes_solver *s, int slot)
     int rc = api_prepare(s->eng, s->id, 0.0, 1);
     if (rc) return rc;
     return s->eng.launch_api(s->id, API_BC, slot, 0, 0);
+}
+
+// The error functionals of the smooth problems run once after a solve: the
+// state comes to the host and is summed there in the reference's serial
+// order (its loops disable OpenMP for reproducibility).
+static int error_functional(ppes_solver *s, int slot, int which, double T, double *err_l2, double *err_max)
+{
+    if (!s || !err_l2 || !err_max || slot < 0 || slot > s->eng.S) return PPES_BAD_ARG;
+    const HostInst &h = s->eng.host(s->id);
+    const int N = h.N, NN = h.NN, nx = h.cfg.nx, ny = h.cfg.ny;
+    std::vector<double> u((size_t)8 * NN * (nx + 2) * (ny + 2));
+    int rc = inst_state_copy(s->eng, s->id, slot, u.data(), false, false);
+    if (rc) return rc;
+    const double pi = 3.14159265358979323846, alpha = pi / 6.0, g = h.cfg.gamma;
+    double l2 = 0.0, mx = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int l = 0; l < N; ++l)
+                for (int k = 0; k < N; ++k) {
+                    const double *q = &u[8 * (k + (size_t)N * (l + (size_t)N * (i + (size_t)(nx + 2) * j)))];
+                    const double x = h.xx[k + (size_t)N * (i - 1)], y = h.yy[l + (size_t)N * (j - 1)];
+                    const double w = (((0.25 * h.hx) * h.hy) * h.qw[k]) * h.qw[l];
+                    if (which == 0) {  // compute_error_smooth_alfven (problems.f90:659-697)
+                        const double x_para = x * cos(alpha) + y * sin(alpha);
+                        const double b_exact = 0.1 * sin(2 * pi * x_para);
+                        const double d = (q[4] * (-sin(alpha)) + q[5] * cos(alpha)) - b_exact;
+                        l2 = l2 + w * (d * d);
+                        mx = std::max(mx, std::fabs(d));
+                    } else {  // compute_error_smooth1D (problems.f90:699-742)
+                        const double rho = 1.0 + 0.2 * sin(x - T);
+                        double ex[8];
+                        ex[0] = rho;
+                        ex[1] = rho * 1.0;
+                        ex[2] = rho * 0.0;
+                        ex[3] = rho * 0.0;
+                        ex[4] = 0.5;
+                        ex[5] = 1.0;
+                        ex[6] = 1.5;
+                        ex[7] = 2.0 / (g - 1.0) + 0.5 * (rho * (1.0 * 1.0 + 0.0 * 0.0 + 0.0 * 0.0) +
+                                                          (0.5 * 0.5 + 1.0 * 1.0 + 1.5 * 1.5));
+                        double sq = 0.0, am = 0.0;
+                        for (int v = 0; v < 8; ++v) {
+                            const double d = q[v] - ex[v];
+                            sq = sq + d * d;
+                            am = std::max(am, std::fabs(d));
+                        }
+                        l2 = l2 + w * sq;
+                        mx = std::max(mx, am);
+                    }
+                }
+    *err_l2 = sqrt(l2);
+    *err_max = mx;
+    return PPES_OK;
+}
+
+extern "C" int ppes_error_smooth_alfven(ppes_solver *s, int slot, double *err_l2, double *err_max)
+{
+    return error_functional(s, slot, 0, 0.0, err_l2, err_max);
+}
+
+extern "C" int ppes_error_smooth1d(ppes_solver *s, int slot, double T, double *err_l2, double *err_max)
+{
+    return error_functional(s, slot, 1, T, err_l2, err_max);
 }
 
 extern "C" int ppes_post_process(ppes_solver *s, int slot, int apply_oedg, int apply_ldf, double oedg_dt,

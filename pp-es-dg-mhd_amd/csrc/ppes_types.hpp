@@ -42,7 +42,17 @@ constexpr int MAXN = 5;  // p <= 4 on the device
 constexpr int MAXS = 5;  // RK stages
 
 enum InstState { ST_IDLE = 0, ST_ACTIVE = 1, ST_DONE = 2, ST_FAILED = 3 };
-enum BcKind { BC_PERIODIC = 0, BC_EXTRAP = 1, BC_FIXED = 2, BC_HOST = 3 };
+enum BcKind { BC_PERIODIC = 0, BC_EXTRAP = 1, BC_FIXED = 2, BC_HOST = 3, BC_BRIOWU = 4, BC_CLOUD = 5, BC_JET = 6 };
+// Per-side rule of a ghost strip (sides: 0 left, 1 right, 2 bottom, 3 top).
+// Every shipped BC_handle is one rule per side; the first `nfix` cells along
+// a side may instead take the fixed state (the jet nozzle).
+enum SideRule {
+    SIDE_PERIODIC = 0,  // the element on the opposite side, same node
+    SIDE_TRACE = 1,     // the boundary trace of the adjacent element, constant along the normal
+    SIDE_COPY = 2,      // the adjacent element, same node
+    SIDE_MIRROR = 3,    // the adjacent element reflected; normal momentum and normal B change sign
+    SIDE_FIXED = 4,     // the fixed conservative state
+};
 enum StatusCode { S_OK = 0, S_NEG_AVG = 1, S_DT_COLLAPSE = 2, S_PP_FAIL = 3 };
 
 // drive-loop state of one instance (drive.f90:83-320), device resident
@@ -67,6 +77,7 @@ struct Inst {
     double went[MAXN * MAXN];        // (((0.25 hx) hy) w_k) w_l
     double drow[2][MAXN][MAXN];      // drow[0][o][a] = (D^o)(N-1, a), drow[1][o][a] = (D^o)(0, a)
     int bc_kind, pad0;
+    int side[4], nfix[4];            // SideRule per side; cells 1..nfix[side] take bc_state
     double bc_state[8];
     double *u[MAXS + 1];             // state slots: 0 = u^n, s = uRK(s)
     double *ut[MAXS + 1];            // residual slots 1..S

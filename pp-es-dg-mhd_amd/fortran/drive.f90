@@ -43,7 +43,8 @@ contains
 		use ppesdg_capi
 		use MHD_PDE_m, only: gamma
 		use DG_solver_m
-		use problems, only: BC_periodic, BC_YeeSjogreen, BC_rotor, rotor_ghost_state
+		use problems, only: BC_periodic, BC_YeeSjogreen, BC_rotor, rotor_ghost_state, BC_BrioWu, BC_cloud_shock, &
+			BC_jet1_cell_ext
 		use io, only: save_tecbin_entlog
 		real, intent(in) :: final_T, CFL
 		logical, intent(in) :: use_oedg, use_LDF, ent_offset
@@ -69,6 +70,12 @@ contains
 		else if (associated(BC_handle, BC_rotor)) then
 			kind = PPES_BC_FIXED
 			state = rotor_ghost_state()
+		else if (associated(BC_handle, BC_BrioWu)) then
+			kind = PPES_BC_BRIOWU
+		else if (associated(BC_handle, BC_cloud_shock)) then
+			kind = PPES_BC_CLOUD
+		else if (associated(BC_handle, BC_jet1_cell_ext)) then
+			kind = PPES_BC_JET
 		else
 			call solve_execute_host_bc(final_T, CFL, use_oedg, use_LDF)
 			return

@@ -10,6 +10,7 @@ module ppesdg_capi
 	integer(c_int), parameter :: PPES_OK = 0, PPES_NEG_AVG = 1, PPES_DT_COLLAPSE = 2, PPES_PP_FAIL = 3, &
 		PPES_BAD_ARG = 4, PPES_HIP_ERROR = 5, PPES_OOM = 6, PPES_UNSUPPORTED = 7
 	integer(c_int), parameter :: PPES_BC_PERIODIC = 0, PPES_BC_EXTRAP = 1, PPES_BC_FIXED = 2, PPES_BC_HOST = 3
+	integer(c_int), parameter :: PPES_BC_BRIOWU = 4, PPES_BC_CLOUD = 5, PPES_BC_JET = 6
 
 	type, bind(C) :: ppes_config
 		integer(c_int) :: nx, ny, px, py, rk_stages

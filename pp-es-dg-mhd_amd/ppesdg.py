@@ -18,11 +18,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libppesdg.so")
 
 OK, NEG_AVG, DT_COLLAPSE, PP_FAIL, BAD_ARG, HIP_ERROR, OOM, UNSUPPORTED = range(8)
-BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3}
+BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3, "briowu": 4, "cloud": 5, "jet": 6}
 KERNELS = {"residual": 0, "stage": 1, "oedg": 2, "limit": 3, "reduce": 4, "bc": 5}
 INST_STATE = {0: "idle", 1: "active", 2: "done", 3: "failed"}
 ARITH = {"exact": 0, "fast": 1}
-PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5}
+PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5, "alfven": 6, "khi": 7,
+        "briowu": 8, "wave1d": 9, "cloud": 10, "jet": 11, "yee": 12}
 
 _lib = None
 
@@ -69,6 +70,8 @@ def lib():
             "ppes_download_avg": [vp, dp],
             "ppes_init_problem": [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double],
             "ppes_apply_bc": [vp, ctypes.c_int],
+            "ppes_error_smooth_alfven": [vp, ctypes.c_int, dp, dp],
+            "ppes_error_smooth1d": [vp, ctypes.c_int, ctypes.c_double, dp, dp],
             "ppes_post_process": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.c_int, ip, ip, dp],
             "ppes_semidiscretize": [vp, ctypes.c_int, ctypes.c_int, dp],
@@ -227,6 +230,20 @@ class Solver:
 
     def apply_bc(self, slot):
         _check(lib().ppes_apply_bc(self.h, slot), "ppes_apply_bc")
+
+    def error_smooth_alfven(self, slot=0):
+        """compute_error_smooth_alfven (problems.f90:659-697) -> (l2, max)"""
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ppes_error_smooth_alfven(self.h, slot, ctypes.byref(l2), ctypes.byref(mx)),
+               "ppes_error_smooth_alfven")
+        return l2.value, mx.value
+
+    def error_smooth1d(self, T, slot=0):
+        """compute_error_smooth1D (problems.f90:699-742) -> (l2, max)"""
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ppes_error_smooth1d(self.h, slot, float(T), ctypes.byref(l2), ctypes.byref(mx)),
+               "ppes_error_smooth1d")
+        return l2.value, mx.value
 
     def post_process(self, slot, apply_oedg, apply_ldf, oedg_dt, comput_ent=True):
         neg, nneg, ent = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

@@ -72,6 +72,16 @@ STAGE_CASES = {
     "stages_blast_p2_ldf0": dict(mode="stages", prob="blast", nx=8, ny=7, p=2, nsteps=3, oedg=2, ldf=0, cfl=0.15, dumpsteps=1),
     "stages_blast_p4": dict(mode="stages", prob="blast", nx=5, ny=4, p=4, nsteps=2, oedg=2, ldf=1, cfl=0.15, dumpsteps=1),
     "stages_otunit_p3": dict(mode="stages", prob="otunit", nx=6, ny=6, p=3, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
+    # the rest of the reference's catalogue (main.f90:57-190) with its boundary fillers
+    "stages_briowu_p3_ldf0": dict(mode="stages", prob="briowu", nx=12, ny=2, p=3, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_cloud_p2_ldf0": dict(mode="stages", prob="cloud", nx=8, ny=8, p=2, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_jet_p2_ldf0": dict(mode="stages", prob="jet", nx=10, ny=6, p=2, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_yee_p2_ldf0": dict(mode="stages", prob="yee", nx=8, ny=8, p=2, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_alfven_p2_ldf0": dict(mode="stages", prob="alfven", nx=6, ny=5, p=2, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_wave1d_p3_ldf0": dict(mode="stages", prob="wave1d", nx=10, ny=2, p=3, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_khi_p2": dict(mode="stages", prob="khi", nx=6, ny=8, p=2, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
+    "stages_jet_p3": dict(mode="stages", prob="jet", nx=10, ny=4, p=3, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
+    "stages_cloud_p1": dict(mode="stages", prob="cloud", nx=10, ny=10, p=1, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
 }
 
 END_CASES = {
@@ -87,6 +97,13 @@ END_CASES = {
     "end_ot2pi_p2_16_rk1": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.1, rk=1),
     "end_ot2pi_p2_16_rk2": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.2, rk=2),
     "end_rotor_p3_16_rk5": dict(mode="solve", prob="rotor", nx=16, ny=16, p=3, tfinal=0.005, oedg=1, ldf=1, cfl=0.2, rk=5),
+    # the smooth problems with their error functionals (main.f90:71-94), CFL 0.6, LDF off as there
+    "end_alfven_p2_8": dict(mode="solve", prob="alfven", nx=8, ny=8, p=2, tfinal=0.05, oedg=1, ldf=0, cfl=0.6),
+    "end_wave1d_p3_16_rk5": dict(mode="solve", prob="wave1d", nx=16, ny=2, p=3, tfinal=0.1, oedg=1, ldf=0, cfl=0.6, rk=5),
+    # shock problems with their own boundary fillers
+    "end_cloud_p2_16": dict(mode="solve", prob="cloud", nx=16, ny=16, p=2, tfinal=0.002, oedg=1, ldf=1, cfl=0.6),
+    "end_jet_p2_20": dict(mode="solve", prob="jet", nx=20, ny=20, p=2, tfinal=1e-4, oedg=1, ldf=1, cfl=0.6),
+    "end_briowu_p2_32": dict(mode="solve", prob="briowu", nx=32, ny=2, p=2, tfinal=0.02, oedg=1, ldf=1, cfl=0.6),
 }
 
 LONG_CASES = {
@@ -120,7 +137,7 @@ def main():
         if a.only and a.only != name:
             continue
         if name.startswith("end_"):
-            keep = ["u_final", "avg_final", "summary", "plt_", "grid_"]
+            keep = ["u_final", "avg_final", "summary", "plt_", "grid_", "err"]
         else:
             keep = ["u_init", "avg_init", "ent_init", "ut_init", "dtlim_init", "s1_", "u_final", "avg_final", "summary"]
         data = run(args, keep)

@@ -21,6 +21,14 @@ def rotor_bc_state(g=1.4):
 
 
 SETUPS["rotor"] = ((0.0, 1.0, 0.0, 1.0), 1.4, "fixed", rotor_bc_state())
+# the rest of the reference's catalogue (main.f90:57-190), on its domains
+SETUPS["alfven"] = ((0.0, 1.0 / math.cos(math.pi / 6.0), 0.0, 1.0 / math.sin(math.pi / 6.0)), 5.0 / 3.0, "periodic", None)
+SETUPS["khi"] = ((0.0, 1.0, -1.0, 1.0), 1.4, "periodic", None)
+SETUPS["briowu"] = ((-1.0, 1.0, -0.5, 0.5), 5.0 / 3.0, "briowu", None)
+SETUPS["wave1d"] = ((0.0, TWO_PI, -0.5, 0.5), 5.0 / 3.0, "periodic", None)
+SETUPS["cloud"] = ((0.0, 1.0, 0.0, 1.0), 5.0 / 3.0, "cloud", None)
+SETUPS["jet"] = ((0.0, 0.5, 0.0, 1.5), 1.4, "jet", None)
+SETUPS["yee"] = ((-1.0, 1.0, -1.0, 1.0), 5.0 / 3.0, "extrap", None)
 
 
 def setup(prob):

@@ -168,8 +168,11 @@ def replay_first_step(name, inject):
     return g, out
 
 
-LDF0 = ["stages_ot2pi_p2_ldf0", "stages_rotor_p3_ldf0", "stages_blast_p2_ldf0"]
-LDF1 = ["stages_ot2pi_p2", "stages_rotor_p3", "stages_blast_p1", "stages_blast_p4", "stages_otunit_p3"]
+LDF0 = ["stages_ot2pi_p2_ldf0", "stages_rotor_p3_ldf0", "stages_blast_p2_ldf0",
+        "stages_briowu_p3_ldf0", "stages_cloud_p2_ldf0", "stages_jet_p2_ldf0", "stages_yee_p2_ldf0",
+        "stages_alfven_p2_ldf0", "stages_wave1d_p3_ldf0"]
+LDF1 = ["stages_ot2pi_p2", "stages_rotor_p3", "stages_blast_p1", "stages_blast_p4", "stages_otunit_p3",
+        "stages_khi_p2", "stages_jet_p3", "stages_cloud_p1"]
 
 
 @pytest.mark.parametrize("name", LDF0)
@@ -231,7 +234,8 @@ def test_multistep_final_state(name):
 
 
 END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12",
-       "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5"]
+       "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5",
+       "end_alfven_p2_8", "end_wave1d_p3_16_rk5", "end_cloud_p2_16", "end_jet_p2_20", "end_briowu_p2_32"]
 
 
 @pytest.mark.parametrize("name", END)
@@ -250,6 +254,38 @@ def test_end_to_end(name):
     # stage (drive.f90:243-268) sum to 1 + 1.1e-15, so t drifts by ~1e-15
     assert abs(summ["t"] - a["tfinal"]) <= (1e-14 if a.get("rk") == 5 else 1e-15) * a["tfinal"]
     assert rel_l1(interior(o.u)[0], g["rho_final"]) <= 1e-12
+    if "err" in g:
+        # compute_error_smooth_alfven / compute_error_smooth1D of the final state
+        e = o.error_smooth_alfven() if a["prob"] == "alfven" else o.error_smooth1d(a["tfinal"])
+        assert abs(e[0] - g["err"][0]) <= 1e-10 * g["err"][0]
+        assert abs(e[1] - g["err"][1]) <= 1e-10 * g["err"][1]
+
+
+def test_error_functionals_closed_form():
+    """Both functionals vanish on the exact solution and equal the closed
+    forms on a constant offset: the Alfven one sees only B_perp (offset e in
+    B_x gives |e sin(pi/6)| everywhere, L2 = that times sqrt(area)); the 1D
+    one sees all 8 variables (offset e in every variable: L2 = e sqrt(8 area))."""
+    s = setup("alfven")
+    o = O.Oracle(6, 5, 2, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    o.init_problem("alfven")
+    l2, mx = o.error_smooth_alfven()
+    assert l2 < 1e-15 and mx < 1e-15
+    u = np.array(o.u, copy=True)
+    u[4] += 1e-3
+    area = (s["domain"][1] - s["domain"][0]) * (s["domain"][3] - s["domain"][2])
+    l2, mx = o.error_smooth_alfven(u)
+    assert abs(mx - 0.5e-3) <= 1e-15
+    assert abs(l2 - 0.5e-3 * math.sqrt(area)) <= 1e-14
+    s = setup("wave1d")
+    o = O.Oracle(10, 2, 3, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    o.init_problem("wave1d")
+    assert max(o.error_smooth1d(0.0)) < 1e-14
+    u = np.array(o.u, copy=True) + 2e-3
+    area = (s["domain"][1] - s["domain"][0]) * (s["domain"][3] - s["domain"][2])
+    l2, mx = o.error_smooth1d(0.0, u)
+    assert abs(mx - 2e-3) <= 1e-14
+    assert abs(l2 - 2e-3 * math.sqrt(8 * area)) <= 1e-13
 
 
 def test_long_orszag_tang_fixture():
