@@ -119,9 +119,9 @@ int ppes_apply_bc(ppes_solver *s, int slot);
 int ppes_error_smooth_alfven(ppes_solver *s, int slot, double *err_l2, double *err_max);
 int ppes_error_smooth1d(ppes_solver *s, int slot, double T, double *err_l2, double *err_max);
 
-/* post_process (DG_solver.f90:128-661) on a state slot. ent_offset must be 0
- * (the reference's option is off in every shipped case). Returns
- * PPES_NEG_AVG / PPES_PP_FAIL as well as setting the flags. */
+/* post_process (DG_solver.f90:128-661) on a state slot, ent_offset
+ * (DG_solver.f90:195-225,605-633) included. Returns PPES_NEG_AVG /
+ * PPES_PP_FAIL as well as setting the flags. */
 int ppes_post_process(ppes_solver *s, int slot, int apply_oedg, int apply_ldf, double oedg_dt,
                       int ent_offset, int comput_ent, int *neg_avg, int *num_negcell, double *tot_ent);
 
@@ -142,6 +142,10 @@ int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int
  * (drive.f90:66-73); ppes_advance continues the step loop for nsteps steps
  * (all steps if nsteps <= 0) or until t_final (> 0). Stats are cumulative. */
 int ppes_solve_begin(ppes_solver *s, int use_ldf);
+/* the ent_offset argument of solve_execute (drive.f90:32) for the following
+ * ppes_solve_begin / ppes_advance; ppes_solve sets it from its argument. The
+ * start-up post-process runs without it, every stage with it (drive.f90:69,280). */
+int ppes_set_ent_offset(ppes_solver *s, int on);
 int ppes_advance(ppes_solver *s, double t_final, long long nsteps, double cfl, int oedg_mode, ppes_run_stats *stats);
 
 /* Per-kernel-class device time, measured with HIP events on the solver's
@@ -195,6 +199,8 @@ int ppes_batch_init_problem(ppes_batch *b, int id, int prob, const double *pert,
 int ppes_batch_upload_state(ppes_batch *b, int id, int slot, const double *u);
 int ppes_batch_download_state(ppes_batch *b, int id, int slot, double *u);
 int ppes_batch_download_avg(ppes_batch *b, int id, double *u_avg);
+/* the ent_offset switch of one instance, read by its next ppes_batch_start */
+int ppes_batch_set_ent_offset(ppes_batch *b, int id, int on);
 /* start-up of one instance (drive.f90:66-73) and activation: it runs until
  * t_final (> 0) or max_steps (> 0) steps */
 int ppes_batch_start(ppes_batch *b, int id, double t_final, long long max_steps, double cfl, int oedg_mode,

@@ -52,6 +52,11 @@ def lib():
         L.orc_post_process.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_post_process_ex.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_run_ex.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(Summary)]
         L.orc_semidiscretize.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(Summary)]
@@ -161,11 +166,11 @@ class Oracle:
         assert arr.flags.f_contiguous
         lib().orc_apply_bc(self.h, _dp(arr))
 
-    def post_process(self, arr, avg, apply_oedg, apply_ldf, oedg_dt, comput_ent=True):
+    def post_process(self, arr, avg, apply_oedg, apply_ldf, oedg_dt, comput_ent=True, ent_offset=False):
         neg, nneg, ent = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        rc = lib().orc_post_process(self.h, _dp(arr), _dp(avg), int(apply_oedg), int(apply_ldf),
-                                    float(oedg_dt), int(comput_ent), ctypes.byref(neg),
-                                    ctypes.byref(nneg), ctypes.byref(ent))
+        rc = lib().orc_post_process_ex(self.h, _dp(arr), _dp(avg), int(apply_oedg), int(apply_ldf),
+                                       float(oedg_dt), int(ent_offset), int(comput_ent), ctypes.byref(neg),
+                                       ctypes.byref(nneg), ctypes.byref(ent))
         return dict(rc=rc, neg_avg=bool(neg.value), num_negcell=nneg.value, tot_ent=ent.value)
 
     def semidiscretize(self, ut, arr):
@@ -187,8 +192,8 @@ class Oracle:
         lib().orc_error_smooth1d(self.h, _dp(u), float(T), ctypes.byref(l2), ctypes.byref(mx))
         return l2.value, mx.value
 
-    def run(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1, use_ldf=True):
+    def run(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1, use_ldf=True, ent_offset=False):
         s = Summary()
-        lib().orc_run(self.h, float(t_final), int(nsteps), float(cfl), int(oedg_mode), int(use_ldf),
-                      ctypes.byref(s))
+        lib().orc_run_ex(self.h, float(t_final), int(nsteps), float(cfl), int(oedg_mode), int(use_ldf),
+                         int(ent_offset), ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in Summary._fields_}

@@ -1026,6 +1026,14 @@ int orc_post_process(orc_solver *s, double *u, double *u_avg, int apply_oedg, in
                      double oedg_dt, int comput_ent, int *neg_avg_out, int *num_negcell_out,
                      double *tot_ent_out)
 {
+    return orc_post_process_ex(s, u, u_avg, apply_oedg, apply_ldf, oedg_dt, 0, comput_ent, neg_avg_out,
+                               num_negcell_out, tot_ent_out);
+}
+
+int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg, int apply_ldf,
+                        double oedg_dt, int ent_offset, int comput_ent, int *neg_avg_out, int *num_negcell_out,
+                        double *tot_ent_out)
+{
     int nx = s->nx, ny = s->ny, N = s->N;
     double g = s->gamma, hx = s->hx, hy = s->hy;
     int nxg = nx + 2, nyg = ny + 2;
@@ -1051,6 +1059,27 @@ int orc_post_process(orc_solver *s, double *u, double *u_avg, int apply_oedg, in
             if (den_avg < PP_EPS || prs_avg < PP_EPS) { cflag[8] = 2.0; continue; }
             cflag[8] = 1.0;
             for (int v = 0; v < 8; ++v) cflag[v] = avg[v];
+            /* cell total entropy before LDF for ent_offset (:195-225): NaN
+             * unless every node has density and pressure >= PP_eps */
+            cflag[12] = NAN;
+            if (ent_offset) {
+                cflag[13] = ent_macro(den_avg, prs_avg, g);
+                int ok = 1;
+                for (int l = 0; l < N; ++l)
+                    for (int k = 0; k < N; ++k) {
+                        const double *q = ue + 8 * (k + N * l);
+                        ok = ok && q[0] >= PP_EPS && prs_macro(q, g) >= PP_EPS;
+                    }
+                if (ok) {
+                    double ce = 0.0;
+                    for (int l = 0; l < N; ++l)
+                        for (int k = 0; k < N; ++k) {
+                            const double *q = ue + 8 * (k + N * l);
+                            ce = ce + (0.25 * hx * hy * s->qw[k] * s->qw[l]) * ent_macro(q[0], prs_macro(q, g), g);
+                        }
+                    cflag[12] = ce;
+                }
+            }
             if (apply_ldf) {
                 double mv1[MAXN * MAXN], mv2[MAXN * MAXN], t1[MAXN * MAXN], c1[MAXN * MAXN], c2[MAXN * MAXN];
                 for (int l = 0; l < N; ++l)
@@ -1216,6 +1245,23 @@ int orc_post_process(orc_solver *s, double *u, double *u_avg, int apply_oedg, in
                     } else
                         break;
                     it_count = it_count + 1;
+                }
+            }
+            /* convex limiting toward the mean when the cell entropy rose
+             * above its pre-LDF value (:605-633) */
+            if (ent_offset && !isnan(cslot[12])) {
+                const double ent_old = cslot[12], ent_avg = cslot[13];
+                double ent_new = 0.0;
+                for (int l = 0; l < N; ++l)
+                    for (int k = 0; k < N; ++k) {
+                        const double *q = ue + 8 * (k + N * l);
+                        ent_new = ent_new + (0.25 * hx * hy * s->qw[k] * s->qw[l]) * ent_macro(q[0], dmax(prs_macro(q, g), PP_EPS), g);
+                    }
+                if (ent_new > ent_old) {
+                    const double theta = dmax(dmin((ent_old - ent_avg) / (ent_new - ent_avg), 1.0), 0.0);
+                    for (int l = 0; l < N; ++l)
+                        for (int k = 0; k < N; ++k)
+                            for (int v = 0; v < 8; ++v) ue[v + 8 * (k + N * l)] = avg[v] + theta * (ue[v + 8 * (k + N * l)] - avg[v]);
                 }
             }
             cslot[9] = neg_node;
@@ -1498,6 +1544,12 @@ static double interior_mass(orc_solver *s, const double *u)
 int orc_run(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_mode, int use_ldf,
             orc_summary *out)
 {
+    return orc_run_ex(s, t_final, nsteps, cfl, oedg_mode, use_ldf, 0, out);
+}
+
+int orc_run_ex(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_mode, int use_ldf, int ent_offset,
+               orc_summary *out)
+{
     int S = s->rk, neg_avg, num_negcell, rc;
     double t = 0.0, dt = 0.0, dt_lim, tot_ent, tRK[5];
     double cfl_rk = (S <= 3) ? 1.0 : 1.508;
@@ -1542,9 +1594,11 @@ int orc_run(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_mode
                 double *us = orc_urk(s, st);
                 orc_apply_bc(s, us);
                 if (oedg_mode == 2)
-                    rc = orc_post_process(s, us, s->uavg, 1, use_ldf, beta * dt, rk_final, &neg_avg, &num_negcell, &tot_ent);
+                    rc = orc_post_process_ex(s, us, s->uavg, 1, use_ldf, beta * dt, ent_offset, rk_final, &neg_avg,
+                                             &num_negcell, &tot_ent);
                 else
-                    rc = orc_post_process(s, us, s->uavg, oedg_mode == 1 && rk_final, use_ldf, dt, rk_final, &neg_avg, &num_negcell, &tot_ent);
+                    rc = orc_post_process_ex(s, us, s->uavg, oedg_mode == 1 && rk_final, use_ldf, dt, ent_offset,
+                                             rk_final, &neg_avg, &num_negcell, &tot_ent);
                 if (neg_avg) { try_again = 1; break; }
                 if (rc == ORC_PP_FAIL) { out->status = ORC_PP_FAIL; goto done; }
                 negcells += num_negcell;

@@ -65,6 +65,10 @@ void orc_error_smooth1d(const orc_solver *s, const double *u, double T, double *
 int orc_post_process(orc_solver *s, double *u, double *u_avg, int apply_oedg, int apply_ldf,
                      double oedg_dt, int comput_ent, int *neg_avg, int *num_negcell,
                      double *tot_ent);
+/* post_process with the ent_offset switch (DG_solver.f90:195-225,605-633) */
+int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg, int apply_ldf,
+                        double oedg_dt, int ent_offset, int comput_ent, int *neg_avg, int *num_negcell,
+                        double *tot_ent);
 void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim, const double *u);
 
 typedef struct orc_summary {
@@ -77,6 +81,9 @@ typedef struct orc_summary {
  * stage (oedg_dt = dt), 2 every stage (oedg_dt = beta_s dt). */
 int orc_run(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_mode, int use_ldf,
             orc_summary *out);
+/* the same with ent_offset passed to every stage's post_process (drive.f90:280) */
+int orc_run_ex(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_mode, int use_ldf, int ent_offset,
+               orc_summary *out);
 
 /* log_avg_divisor (DG_solver.f90:1653-1681), exported for its closed-form test */
 double orc_log_avg_divisor(double LL, double RR);

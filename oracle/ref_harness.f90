@@ -127,7 +127,7 @@ program ref_harness
 	implicit none
 
 	character(len=64) :: mode = 'stages', prob = 'ot2pi', bc = 'auto'
-	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3, plt = 0
+	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3, plt = 0, entoff = 0
 	real :: cfl = 0.2, tfinal = -1.0, oedg_dt_call = 0.01
 	real :: left, right, bottom, top, gam
 	real :: err2(2)
@@ -155,7 +155,7 @@ program ref_harness
 		call run_stages()
 	case ('solve')
 		if (tfinal <= 0.0) error stop 'harness: mode=solve needs tfinal'
-		call solve_execute(tfinal, cfl, oedg >= 1, ldf == 1, .false.)
+		call solve_execute(tfinal, cfl, oedg >= 1, ldf == 1, entoff == 1)
 		call dump_real('u_final', u_con, [8, p_+1, p_+1, nx_+2, ny_+2])
 		call dump_real('avg_final', u_avg, [8, nx_+2, ny_+2])
 		! the reference's Tecplot writer on the final averages (main.f90:67); its
@@ -200,6 +200,7 @@ contains
 			case ('p'); read(val, *) p_
 			case ('rk'); read(val, *) rk_
 			case ('plt'); read(val, *) plt
+			case ('entoff'); read(val, *) entoff
 			case ('nsteps'); read(val, *) nsteps
 			case ('dumpsteps'); read(val, *) dumpsteps
 			case ('oedg'); read(val, *) oedg
@@ -405,10 +406,10 @@ contains
 					call BC_handle(uRK_con(:,:,:,:,:,s), xx, yy, nx, ny, px, py)
 					if (oedg == 2) then
 						do_oedg = .true.
-						call post_process(uRK_con(:,:,:,:,:,s), u_avg, neg_avg, num_negcell, apply_oedg=.true., apply_LDF=(ldf==1), oedg_dt=(oedg_beta*dt), ent_offset=.false., comput_ent=rk_final, tot_ent=tot_ent)
+						call post_process(uRK_con(:,:,:,:,:,s), u_avg, neg_avg, num_negcell, apply_oedg=.true., apply_LDF=(ldf==1), oedg_dt=(oedg_beta*dt), ent_offset=(entoff==1), comput_ent=rk_final, tot_ent=tot_ent)
 					else
 						do_oedg = (oedg == 1) .and. rk_final
-						call post_process(uRK_con(:,:,:,:,:,s), u_avg, neg_avg, num_negcell, apply_oedg=do_oedg, apply_LDF=(ldf==1), oedg_dt=dt, ent_offset=.false., comput_ent=rk_final, tot_ent=tot_ent)
+						call post_process(uRK_con(:,:,:,:,:,s), u_avg, neg_avg, num_negcell, apply_oedg=do_oedg, apply_LDF=(ldf==1), oedg_dt=dt, ent_offset=(entoff==1), comput_ent=rk_final, tot_ent=tot_ent)
 					end if
 					if (step <= dumpsteps) then
 						write(tag, '(A,I0,A,I0,A,I0)') 's', step, '_', s, '_t', try_num

@@ -350,6 +350,7 @@ int Engine::add(const ppes_config &cfg, int *id)
     ok = ok && alloc(&r.part2, 24 * (size_t)cdiv(r.nblk_slot, BLOCK));
     ok = ok && alloc(&r.entpart, (size_t)std::max(r.nblk_el, r.nblk_slot));
     ok = ok && alloc(&r.ghost0, 8 * (size_t)(2 * nx + 2 * ny) * NN);
+    ok = ok && alloc(&r.cent, 2 * (size_t)(nx + 2) * (ny + 2));
     hosts[slot] = h;
     if (!ok) {
         remove(slot);
@@ -404,11 +405,14 @@ void Engine::free_group(Group &g)
 }
 
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs,
-// each with its own L2. Kernels that read neighbouring elements (the
-// residual's face traces, the OEDG face indicators) want neighbours on one
-// XCD: reorder a block map so that XCD x runs one contiguous run of logical
-// blocks, in order (dispatch slot w -> XCD w % 8, its (w / 8)-th block).
-constexpr int N_XCD = 8;
+// each with its own L2. The residual reads the face traces of its four
+// neighbour elements and wants them on its own XCD: reorder its block map so
+// that XCD x runs one contiguous run of logical blocks, in order (dispatch
+// slot w -> XCD w % 8, its (w / 8)-th block).
+#ifndef PPES_N_XCD
+#define PPES_N_XCD 8
+#endif
+constexpr int N_XCD = PPES_N_XCD;
 
 static void xcd_order(std::vector<Blk> &v)
 {
@@ -426,6 +430,8 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
 {
     g.p = p;
     g.members = members;
+    g.eo = false;
+    for (int id : members) g.eo = g.eo || hosts[id]->ent_offset;
     std::vector<Blk> el, sl, fc, bc, r1, r2, rs;
     for (int id : members) {
         const Inst &r = h_recs[id];
@@ -441,8 +447,8 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
         for (long long b = 0; b < cdiv(r.nblk_slot, BLOCK); ++b) r1.push_back({id, (int)b});
         r2.push_back({id, 0});
     }
+    // (the OEDG face map keeps dispatch order: measured slower when reordered)
     xcd_order(rs);
-    xcd_order(fc);
     auto up = [&](Blk **d, const std::vector<Blk> &h) -> int {
         ENG_TRY(hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(Blk)));
         if (!h.empty()) ENG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(Blk), hipMemcpyHostToDevice));
@@ -561,7 +567,11 @@ int Engine::launch_step(bool all_kernels)
     for (int st = 1; st <= S; ++st) {
         for (auto &g : groups) {
             Timed tm(this, KT_STAGE);
-            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+            if (g.eo) {
+                ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, true>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+            } else {
+                ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+            }
         }
         if (any_oedg_stage[st]) {
             for (auto &g : groups) {
@@ -576,7 +586,11 @@ int Engine::launch_step(bool all_kernels)
                 }
                 {
                     Timed tm(this, KT_LIMIT);
-                    ENG_DISPATCH(g.p, LAUNCH(k_oedg_limit<P>, g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                    if (g.eo) {
+                        ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, true>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                    } else {
+                        ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, false>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                    }
                 }
             }
         }
@@ -675,7 +689,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
         LAUNCH1(k_reset_flags, n, d_recs, (const int *)d_ids, n, 1 | 2 | 4);
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp)
-            ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, 0,
+            ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, 0,
                                      (int)(F_GHOST_MEM | F_FORCE_NO_OEDG | F_ENT)));
         for (auto &g : tmp) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, 0, (int)(F_FORCE_NO_OEDG | F_ENT));
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
@@ -712,19 +726,28 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         break;
     case API_COMBINE:
-        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
+        ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
         break;
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
         RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, slot, ut_slot, 0);
         break;
     case API_POST: {
-        ENG_DISPATCH(g.p, LAUNCH(k_stage_pp<P>, stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
+        const bool eo = h_recs[id].ctl.ent_offset != 0;
+        if (eo) {
+            ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, true>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
+        } else {
+            ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
+        }
         if (flags & F_FORCE_OEDG) {
             LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, slot, flags, 0);
             LAUNCH(k_reduce_global, 1, d_recs, (const Blk *)g.d_red2, slot, flags, 1);
             ENG_DISPATCH(g.p, LAUNCH(k_oedg_faces<P>, g.n_face, d_recs, (const Blk *)g.d_face, slot, flags));
-            ENG_DISPATCH(g.p, LAUNCH(k_oedg_limit<P>, g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
+            if (eo) {
+                ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, true>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
+            } else {
+                ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, false>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
+            }
         }
         if (flags & F_ENT) LAUNCH(k_reduce_ent, 1, d_recs, (const int *)d_ids, slot, flags);
         break;

@@ -23,6 +23,7 @@ struct HostInst {
     std::vector<double> qx, qw, D, xx, yy;
     std::vector<double *> allocs;
     size_t bytes = 0;
+    int ent_offset = 0;  // the solve's ent_offset switch (drive.f90:32,280)
 };
 
 struct Group {
@@ -32,6 +33,7 @@ struct Group {
     Blk *d_res = nullptr;  // element map of the residual kernel (its own workgroup size)
     int *d_list = nullptr;
     long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
+    bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
 };
 
 enum KTimerId { KT_RESIDUAL = 0, KT_STAGE = 1, KT_OEDG = 2, KT_LIMIT = 3, KT_REDUCE = 4, KT_BC = 5, KT_COUNT = 6 };

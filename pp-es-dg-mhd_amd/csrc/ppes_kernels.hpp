@@ -536,6 +536,72 @@ __device__ __forceinline__ double entropy_partial(const double q[8], bool valid,
     return s;
 }
 
+// ent_offset, first half (DG_solver.f90:195-225): the cell's total entropy
+// before LDF, summed in the reference's node order, NaN unless every node has
+// density and pressure >= PP_eps; with the entropy of the cell mean. Lane
+// n == 0 of each valid element writes them to sce[el]. All lanes call.
+template <int P>
+__device__ __forceinline__ void pre_ldf_entropy(const double q[8], const double avg[8], bool valid, const Inst &I,
+                                                double (*sred)[2], double (*sce)[2], int t, int el, int n)
+{
+    constexpr int NN = (P + 1) * (P + 1), EPB = BLOCK / NN;
+    const double g = I.g;
+    double c = 0.0, bad = 0.0;
+    if (valid) {
+        const double prs = prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g);
+        if (q[0] >= PP_EPS && prs >= PP_EPS) c = I.went[n] * ent_macro(q[0], prs, g);
+        else bad = 1.0;
+    }
+    sred[t][0] = c;
+    sred[t][1] = bad;
+    __syncthreads();
+    if (valid && el < EPB && n == 0) {
+        double ce = 0.0;
+        bool anybad = false;
+        for (int m = 0; m < NN; ++m) {
+            ce = ce + sred[el * NN + m][0];
+            anybad = anybad || sred[el * NN + m][1] != 0.0;
+        }
+        sce[el][0] = anybad ? __builtin_nan("") : ce;
+        sce[el][1] = ent_macro(avg[0], prs_macro(avg[0], avg[1], avg[2], avg[3], avg[4], avg[5], avg[6], avg[7], g), g);
+    }
+    __syncthreads();
+}
+
+// ent_offset, second half (DG_solver.f90:605-633): when the limited cell's
+// entropy exceeds its value before LDF (ent_old; NaN: skip), scale the cell
+// toward its mean by theta = (ent_old - ent_avg) / (ent_new - ent_avg)
+// clamped to [0, 1]. All lanes call; sth is per-element LDS.
+template <int P>
+__device__ __forceinline__ void entropy_offset(double q[8], const double avg[8], bool valid, double ent_old,
+                                               double ent_avg, const Inst &I, double (*sred)[2], double *sth, int t,
+                                               int el, int n)
+{
+    constexpr int NN = (P + 1) * (P + 1), EPB = BLOCK / NN;
+    const double g = I.g;
+    const bool on = valid && !__builtin_isnan(ent_old);
+    sred[t][0] = on ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
+                    : 0.0;
+    __syncthreads();
+    if (el < EPB && n == 0) {
+        double th = -1.0;  // no change
+        if (on) {
+            double ce = 0.0;
+            for (int m = 0; m < NN; ++m) ce = ce + sred[el * NN + m][0];
+            if (ce > ent_old) th = dmax(dmin((ent_old - ent_avg) / (ce - ent_avg), 1.0), 0.0);
+        }
+        sth[el] = th;
+    }
+    __syncthreads();
+    if (on) {
+        const double th = sth[el];
+        if (th >= 0.0)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) q[v] = avg[v] + th * (q[v] - avg[v]);
+    }
+    __syncthreads();  // sred and sth are reused by the caller
+}
+
 // --------------------------------------------------------------------------
 // Where ghost node (ex, ey, k, l) takes its value from under the instance's
 // side rules: the BC_handle fillers of problems.f90:312-654 each read only
@@ -590,7 +656,7 @@ __device__ __forceinline__ double ghost_value(int mode, int v, double x)
 // exactly what BC_handle would copy (problems.f90:312-348,386-423,597-654),
 // or read from memory (F_GHOST_MEM).
 
-template <int P>
+template <int P, bool EO>
 __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
 {
     using S = Shape<P>;
@@ -611,6 +677,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     __shared__ int scnt[EPB];
     __shared__ double sent[EPB];
     __shared__ double sem[3][EPB][8];
+    // ent_offset records (the EO variant only: the default variant keeps its registers and LDS)
+    __shared__ double sce[EO ? EPB : 1][2];
+    __shared__ double sth[EO ? EPB : 1];
 
     if ((int)blockIdx.x >= nmap) return;
     const Blk bm = map[blockIdx.x];
@@ -693,6 +762,11 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     }
     if (el < EPB && n == 0) sok[el] = (valid && ok) ? 1 : 0;
 
+    // ---- ent_offset: the cell entropy before LDF (workgroup-uniform switch)
+    const bool eo = EO && I.ctl.ent_offset != 0;
+    if constexpr (EO)
+        if (eo) pre_ldf_entropy<P>(q, avg, valid && ok, I, sred, sce, t, el, n);
+
     // ---- LDF projection of (Bx, By) as one dense nodal operator (LDS-staged)
     if (use_ldf) {
         if (valid && ok) {
@@ -721,6 +795,11 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
         if (valid && ok)
 #pragma unroll
             for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
+        if constexpr (EO)
+            if (eo && interior && ok && n == 0) {
+                gout(I.cent)[eg * 2] = sce[el][0];
+                gout(I.cent)[eg * 2 + 1] = sce[el][1];
+            }
         // per-workgroup (sum, min, max) over interior elements: per element
         // and variable by the element's own lanes, then across the
         // workgroup's elements in a fixed order
@@ -769,6 +848,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
     }
+    if constexpr (EO)
+        if (eo) entropy_offset<P>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
     if (stage_ent(I, st, flags)) {
         const double s = entropy_partial<P>(q, lim, lim, I, sred, sent, scnt, t, el, n);
         if (t == 0) gout(I.entpart)[bm.local] = s;
@@ -954,7 +1035,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
 
 // OEDG path, limiter loop (DG_solver.f90:428-656): OEDG damping toward the
 // cell mean, the positivity limiter, the cell entropy, the final state.
-template <int P>
+template <int P, bool EO>
 __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *map, int st, int flags)
 {
     using S = Shape<P>;
@@ -962,6 +1043,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     __shared__ double sred[BLOCK][2];
     __shared__ double sent[EPB];
     __shared__ int scnt[EPB];
+    __shared__ double sth[EO ? EPB : 1];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -1020,6 +1102,11 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
     }
+    if constexpr (EO)
+        if (I.ctl.ent_offset) {
+            const gdbl *ce = gin(I.cent) + ge * 2;
+            entropy_offset<P>(q, avg, valid, valid ? ce[0] : 0.0, valid ? ce[1] : 0.0, I, sred, sth, t, el, n);
+        }
     if (stage_ent(I, st, flags)) {
         const double s = entropy_partial<P>(q, valid, valid, I, sred, sent, scnt, t, el, n);
         if (t == 0) gout(I.entpart)[bm.local] = s;

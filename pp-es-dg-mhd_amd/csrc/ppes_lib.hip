@@ -347,7 +347,7 @@ static void fill_stats(const Inst &r, ppes_run_stats *out)
 
 // Make instance `id` live for one reference-API call outside the drive loop:
 // per-call flags cleared, dt (the OEDG dt) and the LDF switch set.
-static int api_prepare(Engine &e, int id, double dt, int use_ldf)
+static int api_prepare(Engine &e, int id, double dt, int use_ldf, int ent_offset = 0)
 {
     int rc = e.pull();
     if (rc) return rc;
@@ -355,6 +355,7 @@ static int api_prepare(Engine &e, int id, double dt, int use_ldf)
     c.state = ST_ACTIVE;
     c.dt = dt;
     c.use_ldf = use_ldf;
+    c.ent_offset = ent_offset;
     c.oedg_mode = 1;
     c.retry_residual = 0;
     c.neg_avg = 0;
@@ -393,7 +394,10 @@ static int begin_instance(Engine &e, int id, double t_final, long long max_steps
         if ((rc = e.push())) return rc;
         return st;
     }
-    return PPES_OK;
+    // the start-up post_process runs without ent_offset (drive.f90:69), the
+    // stages with the solve's switch (drive.f90:280)
+    c2.ent_offset = e.host(id).ent_offset;
+    return e.push();
 }
 
 static int timer_of(int kernel)
@@ -475,7 +479,7 @@ extern "C" long long ppes_instance_bytes(const ppes_config *cfg)
     const long long nblk_el = cdiv(nx * ny, E), nblk_slot = cdiv((nx + 2) * (ny + 2), E);
     const long long doubles = 8 * vs * (S + 1) + 8 * vsi * S + 8 * (nx + 2) * (ny + 2) + 8 * (nx + 1) * ny +
                               8 * nx * (ny + 1) + 4 * NN * NN + 24 * nblk_slot + 24 * cdiv(nblk_slot, BLOCK) +
-                              std::max(nblk_el, nblk_slot) + 8 * (2 * nx + 2 * ny) * NN;
+                              std::max(nblk_el, nblk_slot) + 8 * (2 * nx + 2 * ny) * NN + 2 * (nx + 2) * (ny + 2);
     return 8 * doubles;
 }
 
@@ -659,9 +663,8 @@ extern "C" int ppes_post_process(ppes_solver *s, int slot, int apply_oedg, int a
                                  int ent_offset, int comput_ent, int *neg_avg, int *num_negcell, double *tot_ent)
 {
     if (!s || slot < 0 || slot > s->eng.S) return PPES_BAD_ARG;
-    if (ent_offset) return PPES_UNSUPPORTED;
     Engine &e = s->eng;
-    int rc = api_prepare(e, s->id, oedg_dt, apply_ldf);
+    int rc = api_prepare(e, s->id, oedg_dt, apply_ldf, ent_offset ? 1 : 0);
     if (rc) return rc;
     const int flags = F_GHOST_MEM | (apply_oedg ? F_FORCE_OEDG : F_FORCE_NO_OEDG) | (comput_ent ? F_ENT : F_NO_ENT);
     if ((rc = e.launch_api(s->id, API_POST, slot, flags, 0))) return rc;
@@ -693,6 +696,13 @@ extern "C" int ppes_rk_combine(ppes_solver *s, int stage, double dt)
     int rc = api_prepare(s->eng, s->id, dt, 1);
     if (rc) return rc;
     return s->eng.launch_api(s->id, API_COMBINE, stage, 0, 0);
+}
+
+extern "C" int ppes_set_ent_offset(ppes_solver *s, int on)
+{
+    if (!s) return PPES_BAD_ARG;
+    s->eng.host(s->id).ent_offset = on ? 1 : 0;
+    return PPES_OK;
 }
 
 extern "C" int ppes_solve_begin(ppes_solver *s, int use_ldf)
@@ -740,9 +750,10 @@ extern "C" int ppes_solve(ppes_solver *s, double t_final, long long nsteps, doub
                           int ent_offset, ppes_run_stats *stats)
 {
     if (!s || !(cfl > 0) || oedg_mode < 0 || oedg_mode > 2) return PPES_BAD_ARG;
-    if (ent_offset) return PPES_UNSUPPORTED;
     if (stats) memset(stats, 0, sizeof *stats);
-    int rc = ppes_solve_begin(s, use_ldf);
+    int rc = ppes_set_ent_offset(s, ent_offset);
+    if (rc) return rc;
+    rc = ppes_solve_begin(s, use_ldf);
     if (rc) {
         if (stats) stats->status = rc;
         return rc;
@@ -823,6 +834,13 @@ extern "C" int ppes_batch_download_state(ppes_batch *b, int id, int slot, double
 extern "C" int ppes_batch_download_avg(ppes_batch *b, int id, double *u_avg)
 {
     return b ? inst_download_avg(b->eng, id, u_avg) : PPES_BAD_ARG;
+}
+
+extern "C" int ppes_batch_set_ent_offset(ppes_batch *b, int id, int on)
+{
+    if (!b || !b->eng.valid(id)) return PPES_BAD_ARG;
+    b->eng.host(id).ent_offset = on ? 1 : 0;
+    return PPES_OK;
 }
 
 extern "C" int ppes_batch_start(ppes_batch *b, int id, double t_final, long long max_steps, double cfl, int oedg_mode,

@@ -63,7 +63,7 @@ struct Ctl {
     unsigned long long dtlim_bits;   // dt_lim of the last completed residual
     int neg_avg, pp_fail, num_negcell, try_num;
     int state, status, flag_last, retry_residual;
-    int oedg_mode, use_ldf, pad0, pad1;
+    int oedg_mode, use_ldf, ent_offset, pad1;
     long long steps, retries, negcells, stages, max_steps;
 };
 
@@ -82,6 +82,7 @@ struct Inst {
     double *u[MAXS + 1];             // state slots: 0 = u^n, s = uRK(s)
     double *ut[MAXS + 1];            // residual slots 1..S
     double *uavg, *sigx, *sigy, *ldf, *part, *part2, *entpart, *ghost0;
+    double *cent;                    // per slot element: cell entropy before LDF (NaN: a node inadmissible), entropy of the mean
     long long nblk_el, nblk_slot;    // workgroups of the element / slot grids
     Ctl ctl;
 };

@@ -100,11 +100,11 @@ contains
 		real, intent(in) :: oedg_dt
 		real, intent(out) :: tot_ent
 		integer(c_int) :: rc, neg, nneg
-		if (ent_offset) error stop 'post_process: ent_offset is not supported'
 		rc = ppes_set_gamma(ppes_handle, gamma)
 		rc = ppes_upload_state(ppes_handle, 1_c_int, u_con)
 		if (rc /= PPES_OK) error stop 'post_process: upload failed'
-		rc = ppes_post_process(ppes_handle, 1_c_int, merge(1, 0, apply_oedg), merge(1, 0, apply_LDF), oedg_dt, 0_c_int, &
+		rc = ppes_post_process(ppes_handle, 1_c_int, merge(1, 0, apply_oedg), merge(1, 0, apply_LDF), oedg_dt, &
+			merge(1, 0, ent_offset), &
 			merge(1, 0, comput_ent), neg, nneg, tot_ent)
 		if (rc == PPES_PP_FAIL) then
 			write(*, *) 're-enter limiting due to FP error in pressure'

@@ -59,7 +59,6 @@ contains
 
 		if (.not. associated(Init_handle)) error stop "drive: solve_execute: procedure pointer 'Init_handle' not initialized!"
 		if (.not. associated(BC_handle)) error stop "drive: solve_execute: procedure pointer 'BC_handle' not initialized!"
-		if (ent_offset) error stop 'drive: solve_execute: ent_offset is not supported'
 		if (rknstg /= 1 .and. rknstg /= 2 .and. rknstg /= 3 .and. rknstg /= 5) error stop 'unsupported yet'
 
 		state = 0.0
@@ -77,7 +76,7 @@ contains
 		else if (associated(BC_handle, BC_jet1_cell_ext)) then
 			kind = PPES_BC_JET
 		else
-			call solve_execute_host_bc(final_T, CFL, use_oedg, use_LDF)
+			call solve_execute_host_bc(final_T, CFL, use_oedg, use_LDF, ent_offset)
 			return
 		end if
 		oedg_mode = merge(1, 0, use_oedg)
@@ -87,6 +86,7 @@ contains
 		rc = ppes_set_bc(ppes_handle, kind, state)
 		rc = ppes_upload_state(ppes_handle, 0_c_int, u_con)
 		if (rc /= PPES_OK) error stop 'drive: solve_execute: upload failed'
+		rc = ppes_set_ent_offset(ppes_handle, merge(1, 0, ent_offset))
 		rc = ppes_solve_begin(ppes_handle, merge(1, 0, use_LDF))
 		if (rc == PPES_NEG_AVG) error stop 'Error: Initial value has negative cell average!'
 		if (rc /= PPES_OK) error stop 'drive: solve_execute: device start-up failed'
@@ -152,14 +152,14 @@ contains
 	! The loop with an arbitrary host BC_handle: every stage's ghost layer is
 	! filled on the host (download, BC_handle, upload) around the device
 	! post-process and residual (drive.f90:83-320).
-	subroutine solve_execute_host_bc(final_T, CFL, use_oedg, use_LDF)
+	subroutine solve_execute_host_bc(final_T, CFL, use_oedg, use_LDF, ent_offset)
 		use, intrinsic :: iso_c_binding
 		use utility, only: stdout, stderr
 		use ppesdg_capi
 		use MHD_PDE_m, only: gamma
 		use DG_solver_m
 		real, intent(in) :: final_T, CFL
-		logical, intent(in) :: use_oedg, use_LDF
+		logical, intent(in) :: use_oedg, use_LDF, ent_offset
 		integer(c_int) :: rc, neg, nneg, s, slot_next
 		real :: t, dt, dt_lim, tot_ent, t_RK(5), CFL_RK, beta
 		logical :: flag, try_again, rk_final
@@ -216,7 +216,7 @@ contains
 					call BC_handle(w, xx, yy, nx, ny, px, py)
 					rc = ppes_upload_state(ppes_handle, s, w)
 					rc = ppes_post_process(ppes_handle, s, merge(1, 0, use_oedg .and. rk_final), merge(1, 0, use_LDF), &
-						real(dt, c_double), 0_c_int, merge(1, 0, rk_final), neg, nneg, tot_ent)
+						real(dt, c_double), merge(1, 0, ent_offset), merge(1, 0, rk_final), neg, nneg, tot_ent)
 					if (rc == PPES_PP_FAIL) error stop 123456789
 					if (neg /= 0) then
 						write(stderr, '(A, g0, A, g0)') "RKstg ", s, ": Inadmissible average! Restart RK with reduced dt. cur_try_num=", try_num

@@ -121,6 +121,12 @@ module ppesdg_capi
 			real(c_double), value :: dt
 		end function ppes_rk_combine
 
+		integer(c_int) function ppes_set_ent_offset(s, on) bind(C, name='ppes_set_ent_offset')
+			import :: c_int, c_ptr
+			type(c_ptr), value :: s
+			integer(c_int), value :: on
+		end function ppes_set_ent_offset
+
 		integer(c_int) function ppes_solve_begin(s, use_ldf) bind(C, name='ppes_solve_begin')
 			import :: c_int, c_ptr
 			type(c_ptr), value :: s

@@ -71,6 +71,8 @@ def lib():
             "ppes_init_problem": [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double],
             "ppes_apply_bc": [vp, ctypes.c_int],
             "ppes_error_smooth_alfven": [vp, ctypes.c_int, dp, dp],
+            "ppes_set_ent_offset": [vp, ctypes.c_int],
+            "ppes_batch_set_ent_offset": [vp, ctypes.c_int, ctypes.c_int],
             "ppes_error_smooth1d": [vp, ctypes.c_int, ctypes.c_double, dp, dp],
             "ppes_post_process": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.c_int, ip, ip, dp],
@@ -245,9 +247,9 @@ class Solver:
                "ppes_error_smooth1d")
         return l2.value, mx.value
 
-    def post_process(self, slot, apply_oedg, apply_ldf, oedg_dt, comput_ent=True):
+    def post_process(self, slot, apply_oedg, apply_ldf, oedg_dt, comput_ent=True, ent_offset=False):
         neg, nneg, ent = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        rc = lib().ppes_post_process(self.h, slot, int(apply_oedg), int(apply_ldf), float(oedg_dt), 0,
+        rc = lib().ppes_post_process(self.h, slot, int(apply_oedg), int(apply_ldf), float(oedg_dt), int(ent_offset),
                                      int(comput_ent), ctypes.byref(neg), ctypes.byref(nneg), ctypes.byref(ent))
         _check(rc, "ppes_post_process", allow=(OK, NEG_AVG, PP_FAIL))
         return dict(rc=rc, neg_avg=bool(neg.value), num_negcell=nneg.value, tot_ent=ent.value)
@@ -260,14 +262,15 @@ class Solver:
     def rk_combine(self, stage, dt):
         _check(lib().ppes_rk_combine(self.h, stage, float(dt)), "ppes_rk_combine")
 
-    def solve(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1, use_ldf=True):
+    def solve(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1, use_ldf=True, ent_offset=False):
         st = RunStats()
-        rc = lib().ppes_solve(self.h, float(t_final), int(nsteps), float(cfl), int(oedg_mode), int(use_ldf), 0,
-                              ctypes.byref(st))
+        rc = lib().ppes_solve(self.h, float(t_final), int(nsteps), float(cfl), int(oedg_mode), int(use_ldf),
+                              int(ent_offset), ctypes.byref(st))
         _check(rc, "ppes_solve", allow=(OK, DT_COLLAPSE, PP_FAIL, NEG_AVG))
         return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
 
-    def solve_begin(self, use_ldf=True):
+    def solve_begin(self, use_ldf=True, ent_offset=False):
+        _check(lib().ppes_set_ent_offset(self.h, int(ent_offset)), "ppes_set_ent_offset")
         _check(lib().ppes_solve_begin(self.h, int(use_ldf)), "ppes_solve_begin")
 
     def advance(self, t_final=-1.0, nsteps=0, cfl=0.2, oedg_mode=1):
@@ -342,7 +345,8 @@ class Batch:
         _check(lib().ppes_batch_download_state(self.h, int(i), slot, _dp(a)), "ppes_batch_download_state")
         return a
 
-    def start(self, i, t_final=-1.0, max_steps=0, cfl=0.2, oedg_mode=1, use_ldf=True):
+    def start(self, i, t_final=-1.0, max_steps=0, cfl=0.2, oedg_mode=1, use_ldf=True, ent_offset=False):
+        _check(lib().ppes_batch_set_ent_offset(self.h, int(i), int(ent_offset)), "ppes_batch_set_ent_offset")
         rc = lib().ppes_batch_start(self.h, int(i), float(t_final), int(max_steps), float(cfl), int(oedg_mode),
                                     int(use_ldf))
         _check(rc, "ppes_batch_start")

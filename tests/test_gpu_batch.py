@@ -114,3 +114,28 @@ def test_batch_memory_accounting():
     b = ppesdg.Batch(3)
     i = b.add(16, 12, 3, (0.0, 1.0, 0.0, 1.0), 1.4)
     assert b.memory(i) == ppesdg.instance_bytes(16, 12, 3)
+
+
+def test_ent_offset_instance_alone_equals_in_mix():
+    """An instance with ent_offset on gives the same bits alone and in a batch
+    of instances without it."""
+    s = setup("rotor")
+    alone = ppesdg.Batch(3)
+    i = alone.add(12, 10, 2, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    alone.init_problem(i, "rotor")
+    alone.start(i, -1.0, 5, 0.6, 1, True, ent_offset=True)
+    alone.step(0)
+    ref = alone.download_state(i, 0)
+    mix = ppesdg.Batch(3)
+    ids = []
+    for k, p in enumerate([3, 2, 1]):
+        j = mix.add(12 + k, 10, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+        mix.init_problem(j, "rotor")
+        ids.append(j)
+    j = mix.add(12, 10, 2, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    mix.init_problem(j, "rotor")
+    for q in ids:
+        mix.start(q, -1.0, 5, 0.6, 1, True)
+    mix.start(j, -1.0, 5, 0.6, 1, True, ent_offset=True)
+    mix.step(0)
+    assert np.array_equal(mix.download_state(j, 0), ref)

@@ -234,3 +234,58 @@ def test_solve_matches_reference_fixture(name):
     assert st["status"] == 0 and st["steps"] == ref_steps
     assert err <= 1e-10
     assert rho.min() > 0
+
+
+# ------------------------------------------------------------------ ent_offset
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("oedg", [0, 1])
+def test_post_process_ent_offset_matches_oracle(p, oedg):
+    """ent_offset (DG_solver.f90:195-225,605-633) in one post_process call,
+    with and without the switch. In one call the offset moves these states by
+    round-off only (the LDF projection seldom raises a cell's entropy); its
+    effect is pinned through whole solves below."""
+    nx, ny = 8, 8
+    u = advanced_state("rotor", nx, ny, p, 2)
+    o, g = make_pair("rotor", nx, ny, p)
+    for eo in (False, True):
+        ref = np.array(u, copy=True)
+        avg_ref = np.zeros(o.shape_avg, order="F")
+        r_ref = o.post_process(ref, avg_ref, oedg, True, 3e-3, True, ent_offset=eo)
+        g.upload_state(0, u)
+        r = g.post_process(0, oedg, True, 3e-3, True, ent_offset=eo)
+        got = g.download_state(0)
+        assert normwise(strips(got), strips(ref)) <= 1e-12
+        assert r["num_negcell"] == r_ref["num_negcell"]
+        assert abs(r["tot_ent"] - r_ref["tot_ent"]) <= 1e-12 * abs(r_ref["tot_ent"])
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_solve_ent_offset_matches_oracle(p):
+    """Rotor, CFL 0.6, 4 steps: the offset moves the state by 6e-10 to 1e-8
+    normwise (oracle with vs without it), 10 to 100 times the tolerance."""
+    o, g = make_pair("rotor", 10, 9, p)
+    o.init_problem("rotor")
+    g.init_problem("rotor")
+    s_ref = o.run(-1.0, 4, 0.6, 1, True, ent_offset=True)
+    s = g.solve(-1.0, 4, 0.6, 1, True, ent_offset=True)
+    ref = np.array(interior(o.u), copy=True)
+    o.init_problem("rotor")
+    o.run(-1.0, 4, 0.6, 1, True, ent_offset=False)
+    assert normwise(interior(o.u), ref) >= 5e-10
+    assert s["steps"] == s_ref["steps"] and s["retries"] == s_ref["retries"]
+    assert normwise(interior(g.download_state(0)), ref) <= 5e-11
+    assert abs(s["tot_ent"] - s_ref["tot_ent"]) <= 1e-12 * abs(s_ref["tot_ent"])
+
+
+def test_solve_ent_offset_matches_reference_fixture():
+    import json
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "end_ot2pi_p2_16_entoff.npz"))
+    a = json.loads(str(z["args_json"]))
+    s = setup(a["prob"])
+    g = ppesdg.Solver(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    g.init_problem(a["prob"])
+    st = g.solve(a["tfinal"], 0, a["cfl"], a["oedg"], a["ldf"], ent_offset=True)
+    assert st["steps"] == int(z["steps"])
+    assert rel_l1(interior(g.download_state(0))[0], z["rho_final"]) <= 1e-10

@@ -152,10 +152,11 @@ def replay_first_step(name, inject):
             w[:, :, :, 1:-1, 1:-1] = (1.0 / 3.0) * un + (2.0 / 3.0) * (urk[2][:, :, :, 1:-1, 1:-1] + dt * ut[3])
         o.apply_bc(w)
         beta = {1: 1.0, 2: 0.25, 3: 2.0 / 3.0}[st]
+        eo = bool(a.get("entoff", 0))
         if a["oedg"] == 2:
-            r = o.post_process(w, avg, True, a["ldf"], beta * dt, st == 3)
+            r = o.post_process(w, avg, True, a["ldf"], beta * dt, st == 3, ent_offset=eo)
         else:
-            r = o.post_process(w, avg, a["oedg"] == 1 and st == 3, a["ldf"], dt, st == 3)
+            r = o.post_process(w, avg, a["oedg"] == 1 and st == 3, a["ldf"], dt, st == 3, ent_offset=eo)
         out[f"s1_{st}_t1_u"] = np.array(w)
         out[f"s1_{st}_t1_avg"] = np.array(avg)
         out[f"s1_{st}_t1_info"] = np.array([dt, float(r["neg_avg"]), float(r["num_negcell"]), r["tot_ent"]])
@@ -170,9 +171,10 @@ def replay_first_step(name, inject):
 
 LDF0 = ["stages_ot2pi_p2_ldf0", "stages_rotor_p3_ldf0", "stages_blast_p2_ldf0",
         "stages_briowu_p3_ldf0", "stages_cloud_p2_ldf0", "stages_jet_p2_ldf0", "stages_yee_p2_ldf0",
-        "stages_alfven_p2_ldf0", "stages_wave1d_p3_ldf0"]
+        "stages_alfven_p2_ldf0", "stages_wave1d_p3_ldf0",
+        "stages_blast_p2_entoff_ldf0", "stages_ot2pi_p3_entoff_ldf0"]
 LDF1 = ["stages_ot2pi_p2", "stages_rotor_p3", "stages_blast_p1", "stages_blast_p4", "stages_otunit_p3",
-        "stages_khi_p2", "stages_jet_p3", "stages_cloud_p1"]
+        "stages_khi_p2", "stages_jet_p3", "stages_cloud_p1", "stages_rotor_p2_entoff", "stages_khi_p3_entoff"]
 
 
 @pytest.mark.parametrize("name", LDF0)
@@ -222,7 +224,7 @@ def test_multistep_final_state(name):
     if inject:
         o.set_tables(ref_tables(a["p"]))
     o.init_problem(a["prob"])
-    summ = o.run(-1.0, a["nsteps"], a["cfl"], a["oedg"], a["ldf"])
+    summ = o.run(-1.0, a["nsteps"], a["cfl"], a["oedg"], a["ldf"], ent_offset=bool(a.get("entoff", 0)))
     ref = g["summary"]
     assert summ["steps"] == int(ref[1]) and summ["retries"] == int(ref[2])
     if inject:
@@ -235,7 +237,8 @@ def test_multistep_final_state(name):
 
 END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12",
        "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5",
-       "end_alfven_p2_8", "end_wave1d_p3_16_rk5", "end_cloud_p2_16", "end_jet_p2_20", "end_briowu_p2_32"]
+       "end_alfven_p2_8", "end_wave1d_p3_16_rk5", "end_cloud_p2_16", "end_jet_p2_20", "end_briowu_p2_32",
+       "end_ot2pi_p2_16_entoff"]
 
 
 @pytest.mark.parametrize("name", END)
@@ -245,7 +248,7 @@ def test_end_to_end(name):
     g, a, s = stage_case(name)
     o = O.Oracle(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"], rk=a.get("rk", 3))
     o.init_problem(a["prob"])
-    summ = o.run(a["tfinal"], 0, a["cfl"], a["oedg"], a["ldf"])
+    summ = o.run(a["tfinal"], 0, a["cfl"], a["oedg"], a["ldf"], ent_offset=bool(a.get("entoff", 0)))
     if "steps" in g:
         assert summ["steps"] == int(g["steps"])
     else:
