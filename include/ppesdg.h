@@ -57,6 +57,14 @@ extern "C" {
 #define PPES_PROB_JET 11      /* jet1_init_pri         problems.f90:249-257 */
 #define PPES_PROB_YEE 12      /* YeeSjogreenInitCon    problems.f90:259-308 (conservative, init_by_fun_con) */
 
+/* face fluxes of the residual: the reference's ESPP_HLL_flux_x (default) and
+ * its alternatives, which the reference selects by editing the call at
+ * DG_solver.f90:685-688,703-706 */
+#define PPES_FLUX_ESPP_HLL 0  /* ESPP_HLL_flux_x DG_solver.f90:894-1071 */
+#define PPES_FLUX_PP_HLL 1    /* PP_HLL_flux_x   DG_solver.f90:1073-1221 */
+#define PPES_FLUX_ESPP_LF 2   /* ESPP_LF_flux_x  DG_solver.f90:1223-1395 */
+#define PPES_FLUX_EC_LF 3     /* EC_LF_flux_x    DG_solver.f90:1397-1497 */
+
 typedef struct ppes_config {
     int nx, ny;            /* elements */
     int px, py;            /* polynomial degree; px == py required */
@@ -142,6 +150,8 @@ int ppes_solve(ppes_solver *s, double t_final, long long nsteps, double cfl, int
  * (drive.f90:66-73); ppes_advance continues the step loop for nsteps steps
  * (all steps if nsteps <= 0) or until t_final (> 0). Stats are cumulative. */
 int ppes_solve_begin(ppes_solver *s, int use_ldf);
+/* face flux of the residual (PPES_FLUX_*), for every later call */
+int ppes_set_flux(ppes_solver *s, int kind);
 /* the ent_offset argument of solve_execute (drive.f90:32) for the following
  * ppes_solve_begin / ppes_advance; ppes_solve sets it from its argument. The
  * start-up post-process runs without it, every stage with it (drive.f90:69,280). */
@@ -199,6 +209,8 @@ int ppes_batch_init_problem(ppes_batch *b, int id, int prob, const double *pert,
 int ppes_batch_upload_state(ppes_batch *b, int id, int slot, const double *u);
 int ppes_batch_download_state(ppes_batch *b, int id, int slot, double *u);
 int ppes_batch_download_avg(ppes_batch *b, int id, double *u_avg);
+/* face flux of one instance (PPES_FLUX_*) */
+int ppes_batch_set_flux(ppes_batch *b, int id, int kind);
 /* the ent_offset switch of one instance, read by its next ppes_batch_start */
 int ppes_batch_set_ent_offset(ppes_batch *b, int id, int on);
 /* start-up of one instance (drive.f90:66-73) and activation: it runs until

@@ -62,6 +62,8 @@ def lib():
                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(Summary)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_error_smooth_alfven.argtypes = [ctypes.c_void_p, dp, dp, dp]
+        L.orc_face_flux.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double, dp]
+        L.orc_set_flux.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_error_smooth1d.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, dp, dp]
         L.orc_log_avg_divisor.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_log_avg_divisor.restype = ctypes.c_double
@@ -97,6 +99,18 @@ def ldf_modal_operator(p):
 
 def log_avg_divisor(L, R):
     return lib().orc_log_avg_divisor(float(L), float(R))
+
+
+FLUX = {"espp_hll": 0, "pp_hll": 1, "espp_lf": 2, "ec_lf": 3}
+
+
+def face_flux(kind, wl, wr, gamma):
+    """One x-flux evaluation -> (nflux[8], nBn, sl, sr) as an array of 11."""
+    out = np.zeros(11)
+    a = np.ascontiguousarray(wl, dtype=np.float64)
+    b = np.ascontiguousarray(wr, dtype=np.float64)
+    lib().orc_face_flux(FLUX[kind], _dp(a), _dp(b), float(gamma), _dp(out))
+    return out
 
 
 def set_threads(n):
@@ -177,6 +191,9 @@ class Oracle:
         dtl = ctypes.c_double()
         lib().orc_semidiscretize(self.h, _dp(ut), ctypes.byref(dtl), _dp(arr))
         return dtl.value
+
+    def set_flux(self, kind):
+        lib().orc_set_flux(self.h, FLUX[kind])
 
     def error_smooth_alfven(self, arr=None):
         """compute_error_smooth_alfven (problems.f90:659-697) -> (l2, max)"""

@@ -30,7 +30,7 @@ typedef long double ldbl;
 struct orc_solver {
     int nx, ny, p, N, rk;
     double left, right, bottom, top, gamma, hx, hy;
-    int bc_kind;
+    int bc_kind, flux;
     double bc_state[8];
     /* 1D tables (px = py) */
     double qx[MAXN], qw[MAXN], D[MAXN * MAXN], Pmn[MAXN * MAXN], Pnm[MAXN * MAXN];
@@ -541,6 +541,138 @@ static void espp_hll_flux_x(const double *wl, const double *wr, double g, double
     *sl_out = sl;
     *sr_out = sr;
 }
+
+/* State quantities shared by the alternative x-fluxes (the common head of
+ * PP_HLL_flux_x / ESPP_LF_flux_x / EC_LF_flux_x, DG_solver.f90:1073-1497) */
+typedef struct {
+    double rho, mx, my, mz, bx, by, bz, e;
+    double inv_rho, vx, vy, vz, e_knt, b_sq, p_mag, p_fld, p_tot, b_dot_u, snc_sq, cfx;
+    double inv_theta, phi, sp_ent, ent_flux;
+    double f[8], v[8];
+} side_t;
+
+static void side_state(const double *w, double g, side_t *a)
+{
+    a->rho = w[0]; a->mx = w[1]; a->my = w[2]; a->mz = w[3]; a->bx = w[4]; a->by = w[5]; a->bz = w[6]; a->e = w[7];
+    a->inv_rho = 1.0 / a->rho;
+    a->vx = a->inv_rho * a->mx;
+    a->vy = a->inv_rho * a->my;
+    a->vz = a->inv_rho * a->mz;
+    a->e_knt = 0.5 * a->inv_rho * (a->mx * a->mx + a->my * a->my + a->mz * a->mz);
+    const double ey = a->bz * a->vx - a->bx * a->vz, ez = a->bx * a->vy - a->by * a->vx;
+    a->b_sq = a->bx * a->bx + a->by * a->by + a->bz * a->bz;
+    a->p_mag = 0.5 * a->b_sq;
+    a->p_fld = dmax((g - 1.0) * (a->e - (a->e_knt + a->p_mag)), PP_EPS);
+    a->p_tot = a->p_fld + a->p_mag;
+    a->b_dot_u = a->bx * a->vx + a->by * a->vy + a->bz * a->vz;
+    a->snc_sq = g * a->p_fld * a->inv_rho;
+    const double alfven_sq = a->b_sq * a->inv_rho, sum_sq = a->snc_sq + alfven_sq;
+    a->cfx = sqrt(0.5 * (sum_sq + sqrt(dmax(sum_sq * sum_sq - 4.0 * a->snc_sq * a->inv_rho * (a->bx * a->bx), 0.0))));
+    a->inv_theta = a->rho / a->p_fld;
+    a->phi = a->inv_theta * a->b_dot_u;
+    a->sp_ent = log(a->p_fld) - g * log(a->rho);
+    a->ent_flux = (-((a->rho * a->sp_ent) / (g - 1.0))) * a->vx;
+    a->f[0] = a->mx; a->f[1] = a->vx * a->mx - a->bx * a->bx + a->p_tot; a->f[2] = a->vx * a->my - a->by * a->bx;
+    a->f[3] = a->vx * a->mz - a->bz * a->bx; a->f[4] = 0.0; a->f[5] = -ez; a->f[6] = ey;
+    a->f[7] = a->vx * (a->e + a->p_tot) - a->bx * a->b_dot_u;
+    a->v[0] = (g - a->sp_ent) / (g - 1.0) - a->e_knt / a->p_fld;
+    a->v[1] = a->mx / a->p_fld; a->v[2] = a->my / a->p_fld; a->v[3] = a->mz / a->p_fld;
+    a->v[4] = a->inv_theta * a->bx; a->v[5] = a->inv_theta * a->by; a->v[6] = a->inv_theta * a->bz; a->v[7] = -a->inv_theta;
+}
+
+/* Roe-average velocity and field-jump term of the PP speed estimates */
+static void pp_speeds(const side_t *L, const side_t *R, double *vx_roe, double *db_aux)
+{
+    const double sqrt_rho_l = sqrt(L->rho), sqrt_rho_r = sqrt(R->rho);
+    const double sum_sqrt_rho = sqrt_rho_l + sqrt_rho_r;
+    *vx_roe = (L->vx * sqrt_rho_l + R->vx * sqrt_rho_r) / sum_sqrt_rho;
+    *db_aux = sqrt((L->bx - R->bx) * (L->bx - R->bx) + (L->by - R->by) * (L->by - R->by) + (L->bz - R->bz) * (L->bz - R->bz)) /
+              sum_sqrt_rho;
+}
+
+/* PP_HLL_flux_x (DG_solver.f90:1073-1221): HLL with the PP speed estimates only */
+static void pp_hll_flux_x(const double *wl, const double *wr, double g, double *nflux, double *nBn, double *sl_out,
+                          double *sr_out)
+{
+    side_t L, R;
+    side_state(wl, g, &L);
+    side_state(wr, g, &R);
+    double vx_roe, db_aux;
+    pp_speeds(&L, &R, &vx_roe, &db_aux);
+    const double sl_pp = dmin(L.vx, vx_roe) - L.cfx - db_aux;
+    const double sr_pp = dmax(R.vx, vx_roe) + R.cfx + db_aux;
+    const double sl = dmin(sl_pp, 0.0), sr = dmax(sr_pp, 0.0);
+    const double dslr = sr - sl, wei_l = sr / dslr, wei_r = -sl / dslr, c = (sl * sr) / dslr;
+    for (int i = 0; i < 8; ++i) nflux[i] = wei_l * L.f[i] + wei_r * R.f[i] + c * (wr[i] - wl[i]);
+    *nBn = wei_l * L.bx + wei_r * R.bx;
+    *sl_out = sl;
+    *sr_out = sr;
+}
+
+/* ESPP_LF_flux_x (DG_solver.f90:1223-1395): local Lax-Friedrichs with the
+ * larger of the PP and the (one-sided) semidiscrete ES speed */
+static void espp_lf_flux_x(const double *wl, const double *wr, double g, double *nflux, double *nBn, double *sl_out,
+                           double *sr_out)
+{
+    side_t L, R;
+    side_state(wl, g, &L);
+    side_state(wr, g, &R);
+    double vx_roe, db_aux;
+    pp_speeds(&L, &R, &vx_roe, &db_aux);
+    const double sl_pp = dmin(L.vx, vx_roe) - L.cfx - db_aux;
+    const double sr_pp = dmax(dmax(fabs(L.vx) + L.cfx, fabs(R.vx) + R.cfx), fabs(vx_roe) + dmax(L.cfx, R.cfx)) + db_aux;
+    double dot_a = 0.0, dot_b = 0.0;
+    for (int i = 0; i < 8; ++i) dot_a += (R.v[i] - L.v[i]) * (wr[i] - wl[i]);
+    for (int i = 0; i < 8; ++i) dot_b += (L.v[i] + R.v[i]) * (R.f[i] - L.f[i]);
+    const double a_form = dmax(dot_a, 0.0) + 1e-9;
+    const double bb_form = 2.0 * (R.ent_flux - L.ent_flux) - dot_b - (L.phi + R.phi) * (R.bx - L.bx);
+    const double sr_es = dmax(bb_form / a_form, 0.0);
+    const double sr = dmax(dmax(dmax(sr_es, -sl_pp), sr_pp), 0.0);
+    for (int i = 0; i < 8; ++i) nflux[i] = 0.5 * L.f[i] + 0.5 * R.f[i] - 0.5 * sr * (wr[i] - wl[i]);
+    *nBn = 0.5 * L.bx + 0.5 * R.bx;
+    *sl_out = -sr;
+    *sr_out = sr;
+}
+
+/* EC_LF_flux_x (DG_solver.f90:1397-1497): the EC flux plus Lax-Friedrichs
+ * dissipation at the larger one-sided fast speed */
+static void ec_lf_flux_x(const double *wl, const double *wr, double g, double *nflux, double *nBn, double *sl_out,
+                         double *sr_out)
+{
+    side_t L, R;
+    side_state(wl, g, &L);
+    side_state(wr, g, &R);
+    const double sr = dmax(fabs(L.vx) + L.cfx, fabs(R.vx) + R.cfx);
+    prim_t pl = {L.rho, L.vx, L.vy, L.vz, L.bx, L.by, L.bz, 0.5 * L.rho / L.p_fld};
+    prim_t pr = {R.rho, R.vx, R.vy, R.vz, R.bx, R.by, R.bz, 0.5 * R.rho / R.p_fld};
+    double ec[8];
+    ec_flux(&pl, &pr, g, 0, ec);
+    for (int i = 0; i < 8; ++i) nflux[i] = ec[i] - 0.5 * sr * (wr[i] - wl[i]);
+    *nBn = 0.5 * (L.bx + R.bx);
+    *sl_out = -sr;
+    *sr_out = sr;
+}
+
+/* the face flux of the solver's choice (the call site DG_solver.f90:685-688,703-706) */
+static void face_flux_x(int kind, const double *wl, const double *wr, double g, double *nflux, double *nBn,
+                        double *sl, double *sr)
+{
+    switch (kind) {
+    case ORC_FLUX_PP_HLL: pp_hll_flux_x(wl, wr, g, nflux, nBn, sl, sr); break;
+    case ORC_FLUX_ESPP_LF: espp_lf_flux_x(wl, wr, g, nflux, nBn, sl, sr); break;
+    case ORC_FLUX_EC_LF: ec_lf_flux_x(wl, wr, g, nflux, nBn, sl, sr); break;
+    default: espp_hll_flux_x(wl, wr, g, nflux, nBn, sl, sr); break;
+    }
+}
+
+int orc_face_flux(int kind, const double *wl, const double *wr, double g, double *out)
+{
+    if (kind < ORC_FLUX_ESPP_HLL || kind > ORC_FLUX_EC_LF) return ORC_BAD_ARG;
+    face_flux_x(kind, wl, wr, g, out, out + 8, out + 9, out + 10);
+    return ORC_OK;
+}
+
+void orc_set_flux(orc_solver *s, int kind) { s->flux = kind; }
 
 /* swap_xy (DG_solver.f90:1556-1568) */
 static inline void swap_xy(const double *a, double *b)
@@ -1303,7 +1435,7 @@ void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim_out, const dou
         for (int i = 1; i <= nx + 1; ++i)
             for (int l = 0; l < N; ++l) {
                 size_t f = (size_t)l + N * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1));
-                espp_hll_flux_x(u + UIDX(s, 0, N - 1, l, i - 1, j), u + UIDX(s, 0, 0, l, i, j), g,
+                face_flux_x(s->flux, u + UIDX(s, 0, N - 1, l, i - 1, j), u + UIDX(s, 0, 0, l, i, j), g,
                                 s->nflux_x + 8 * f, s->nB_x + f, s->sxl + f, s->sxr + f);
             }
     /* y-faces (:693-713) */
@@ -1315,7 +1447,7 @@ void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim_out, const dou
                 double t1[8], t2[8], tf[8];
                 swap_xy(u + UIDX(s, 0, k, N - 1, i, j - 1), t1);
                 swap_xy(u + UIDX(s, 0, k, 0, i, j), t2);
-                espp_hll_flux_x(t1, t2, g, tf, s->nB_y + f, s->syd + f, s->syu + f);
+                face_flux_x(s->flux, t1, t2, g, tf, s->nB_y + f, s->syd + f, s->syu + f);
                 swap_xy(tf, s->nflux_y + 8 * f);
             }
     /* element loop (:717-889); dt_lim reduced serially afterwards */

@@ -26,7 +26,14 @@ enum { ORC_PROB_OT2PI = 0, ORC_PROB_ROTOR = 1, ORC_PROB_BLAST = 2, ORC_PROB_OTUN
        ORC_PROB_BLASTREF = 4, ORC_PROB_OTPERT = 5, ORC_PROB_ALFVEN = 6, ORC_PROB_KHI = 7,
        ORC_PROB_BRIOWU = 8, ORC_PROB_WAVE1D = 9, ORC_PROB_CLOUD = 10, ORC_PROB_JET = 11, ORC_PROB_YEE = 12 };
 
+/* face fluxes: ESPP_HLL_flux_x (the reference's choice), and the alternatives
+ * PP_HLL_flux_x, ESPP_LF_flux_x, EC_LF_flux_x (DG_solver.f90:894-1497) */
+enum { ORC_FLUX_ESPP_HLL = 0, ORC_FLUX_PP_HLL = 1, ORC_FLUX_ESPP_LF = 2, ORC_FLUX_EC_LF = 3 };
+
 typedef struct orc_solver orc_solver;
+
+/* one x-flux evaluation: out = nflux[8], nBn, sl, sr */
+int orc_face_flux(int kind, const double *wl, const double *wr, double g, double *out);
 
 /* One-time tables for degree p (1..6), computed from the mathematics in
  * long double: GLL nodes x and weights w (N), differentiation matrix D,
@@ -41,6 +48,8 @@ orc_solver *orc_create(int nx, int ny, int p, int rk_stages, double left, double
                        double bottom, double top, double gamma, int bc_kind,
                        const double *bc_state);
 void orc_destroy(orc_solver *s);
+/* face flux of the residual (default ORC_FLUX_ESPP_HLL) */
+void orc_set_flux(orc_solver *s, int kind);
 
 /* replace the 1D tables by caller-provided ones (same layout as orc_tables) */
 void orc_set_tables(orc_solver *s, const double *x, const double *w, const double *D,

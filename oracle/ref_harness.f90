@@ -141,6 +141,10 @@ program ref_harness
 		call dump_tables()
 		stop
 	end if
+	if (trim(mode) == 'fluxes') then
+		call dump_fluxes()
+		stop
+	end if
 
 	call setup_problem()
 	if (rk_ /= 3 .and. trim(mode) /= 'solve') error stop 'harness: rk /= 3 needs mode=solve'
@@ -459,6 +463,29 @@ contains
 		call dump_real('c_avg', u_avg, [8, nx_+2, ny_+2])
 		call dump_real('c_info', [merge(1.0, 0.0, neg_avg), real(num_negcell), tot_ent], [3])
 	end subroutine run_calls
+
+	! The four x-fluxes (ESPP_HLL_flux_x, PP_HLL_flux_x, ESPP_LF_flux_x,
+	! EC_LF_flux_x, DG_solver.f90:894-1497) on the state pairs of file `init`
+	! (n pairs of (w_l(8), w_r(8)), gamma from the command line): for each
+	! pair and flux, nflux(8), nBn, sl, sr.
+	subroutine dump_fluxes()
+		real, allocatable :: w(:,:,:), res(:,:,:)
+		integer :: u, n, i
+		open(newunit=u, file=trim(init_path), access='stream', form='unformatted', status='old')
+		inquire(unit=u, size=n)
+		n = n / (8 * 16)
+		allocate(w(8, 2, n), res(11, 4, n))
+		read(u) w
+		close(u)
+		call set_gamma(gam)
+		do i = 1, n
+			call ESPP_HLL_flux_x(w(:,1,i), w(:,2,i), gamma, res(1:8,1,i), res(9,1,i), res(10,1,i), res(11,1,i))
+			call PP_HLL_flux_x(w(:,1,i), w(:,2,i), gamma, res(1:8,2,i), res(9,2,i), res(10,2,i), res(11,2,i))
+			call ESPP_LF_flux_x(w(:,1,i), w(:,2,i), gamma, res(1:8,3,i), res(9,3,i), res(10,3,i), res(11,3,i))
+			call EC_LF_flux_x(w(:,1,i), w(:,2,i), gamma, res(1:8,4,i), res(9,4,i), res(10,4,i), res(11,4,i))
+		end do
+		call dump_real('fluxes', res, [11, 4, n])
+	end subroutine dump_fluxes
 
 	! 1D tables for p = 1..6 and the LDF modal projector for p = 1..6
 	! (applied to every unit vector, giving the dense 2(p+1)^2 operator).

@@ -29,10 +29,25 @@ namespace ppes {
             ENG_TRY(hipGetLastError());                                                        \
         }                                                                                      \
     } while (0)
-#define RESIDUAL_DISPATCH(p, grid, ...)                                                                          \
+#define RESIDUAL_DISPATCH_F(p, FK, grid, ...)                                                                    \
     do {                                                                                                         \
-        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, true>), grid, resid_tb(P), __VA_ARGS__)); } \
-        else { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, false>), grid, resid_tb(P), __VA_ARGS__)); }            \
+        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, true, resid_tb(P), FK>), grid, resid_tb(P), __VA_ARGS__)); } \
+        else { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, false, resid_tb(P), FK>), grid, resid_tb(P), __VA_ARGS__)); } \
+    } while (0)
+// the residual over a group: one launch per face flux kind present
+#define RESIDUAL_DISPATCH(g, ...)                                                                                \
+    do {                                                                                                         \
+        for (int fk_ = 0; fk_ < 4; ++fk_) {                                                                      \
+            const long long n_ = (g).res_off[fk_ + 1] - (g).res_off[fk_];                                        \
+            if (n_ <= 0) continue;                                                                               \
+            const Blk *m_ = (g).d_res + (g).res_off[fk_];                                                        \
+            switch (fk_) {                                                                                       \
+            case FLUX_PP_HLL: RESIDUAL_DISPATCH_F((g).p, FLUX_PP_HLL, n_, d_recs, m_, __VA_ARGS__); break;      \
+            case FLUX_ESPP_LF: RESIDUAL_DISPATCH_F((g).p, FLUX_ESPP_LF, n_, d_recs, m_, __VA_ARGS__); break;    \
+            case FLUX_EC_LF: RESIDUAL_DISPATCH_F((g).p, FLUX_EC_LF, n_, d_recs, m_, __VA_ARGS__); break;        \
+            default: RESIDUAL_DISPATCH_F((g).p, FLUX_ESPP_HLL, n_, d_recs, m_, __VA_ARGS__); break;             \
+            }                                                                                                    \
+        }                                                                                                        \
     } while (0)
 
 #define ENG_DISPATCH(p, CALL)                     \
@@ -432,13 +447,13 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
     g.members = members;
     g.eo = false;
     for (int id : members) g.eo = g.eo || hosts[id]->ent_offset;
-    std::vector<Blk> el, sl, fc, bc, r1, r2, rs;
+    std::vector<Blk> el, sl, fc, bc, r1, r2, rs, rsf[4];
     for (int id : members) {
         const Inst &r = h_recs[id];
         const int NN = (p + 1) * (p + 1);
         for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
         const long long nres = cdiv((long long)r.nx * r.ny, resid_tb(p) / NN);
-        for (long long b = 0; b < nres; ++b) rs.push_back({id, (int)b});
+        for (long long b = 0; b < nres; ++b) rsf[hosts[id]->flux].push_back({id, (int)b});
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
         const long long nf = ((long long)(r.nx + 1) * r.ny + (long long)r.nx * (r.ny + 1)) * 8;
         for (long long b = 0; b < cdiv(nf, BLOCK); ++b) fc.push_back({id, (int)b});
@@ -448,7 +463,12 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
         r2.push_back({id, 0});
     }
     // (the OEDG face map keeps dispatch order: measured slower when reordered)
-    xcd_order(rs);
+    g.res_off[0] = 0;
+    for (int f = 0; f < 4; ++f) {
+        xcd_order(rsf[f]);
+        rs.insert(rs.end(), rsf[f].begin(), rsf[f].end());
+        g.res_off[f + 1] = (long long)rs.size();
+    }
     auto up = [&](Blk **d, const std::vector<Blk> &h) -> int {
         ENG_TRY(hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(Blk)));
         if (!h.empty()) ENG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(Blk), hipMemcpyHostToDevice));
@@ -470,8 +490,9 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int p)
 }
 
 // The plan (block maps per degree, the instance list and the stages that
-// apply OEDG) depends only on which instances are active and their OEDG
-// modes. It is kept while that set is unchanged, and also while no instance
+// apply OEDG) depends only on which instances are active, their OEDG modes,
+// face fluxes and ent_offset switches. It is kept while that set is
+// unchanged, and also while no instance
 // is active, so that a solver advanced a few steps at a time (or a batch
 // whose membership returns to a previous set) reuses both the plan and the
 // captured graph.
@@ -480,7 +501,8 @@ int Engine::build_plan()
     if (plan_ok) return PPES_OK;
     std::vector<long long> key;
     for (size_t i = 0; i < hosts.size(); ++i)
-        if (hosts[i] && h_recs[i].ctl.state == ST_ACTIVE) key.push_back((long long)i * 16 + h_recs[i].ctl.oedg_mode);
+        if (hosts[i] && h_recs[i].ctl.state == ST_ACTIVE)
+            key.push_back((long long)i * 64 + h_recs[i].ctl.oedg_mode * 16 + hosts[i]->flux * 2 + hosts[i]->ent_offset);
     if (key.empty()) {
         n_all = 0;
         plan_ok = true;
@@ -561,7 +583,7 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 1);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, 0, 1, 1);
+            RESIDUAL_DISPATCH(g, 0, 1, 1);
         }
     }
     for (int st = 1; st <= S; ++st) {
@@ -603,7 +625,7 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 0);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, st, (st % S) + 1, 0);
+            RESIDUAL_DISPATCH(g, st, (st % S) + 1, 0);
         }
     }
     LAUNCH1(k_step_end, n_all, d_recs, (const int *)d_all, n_all);
@@ -695,7 +717,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
-        for (auto &g : tmp) RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, 0, 1, 0);
+        for (auto &g : tmp) RESIDUAL_DISPATCH(g, 0, 1, 0);
     }
     (void)hipStreamSynchronize(stream);
     for (auto &g : tmp) free_group(g);
@@ -703,8 +725,20 @@ int Engine::launch_startup(const std::vector<int> &ids)
     return rc;
 }
 
+int Engine::set_flux(int id, int kind)
+{
+    if (!valid(id) || kind < FLUX_ESPP_HLL || kind > FLUX_EC_LF) return PPES_BAD_ARG;
+    if (hosts[id]->flux == kind) return PPES_OK;
+    hosts[id]->flux = kind;
+    // the instance's residual maps change: drop its one-instance maps; the plan key holds the kind
+    if (id < (int)api_groups.size() && api_groups[id].d_el) free_group(api_groups[id]);
+    invalidate_plan();
+    return PPES_OK;
+}
+
 // single reference-API calls on one instance (its one-instance block maps
-// are built on first use and kept until the instance is removed)
+// are built on first use and kept until the instance is removed or its face
+// flux changes)
 int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
 {
     if ((int)api_groups.size() < (int)hosts.size()) api_groups.resize(hosts.size());
@@ -730,7 +764,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         break;
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
-        RESIDUAL_DISPATCH(g.p, g.n_res, d_recs, (const Blk *)g.d_res, slot, ut_slot, 0);
+        RESIDUAL_DISPATCH(g, slot, ut_slot, 0);
         break;
     case API_POST: {
         const bool eo = h_recs[id].ctl.ent_offset != 0;

@@ -24,13 +24,15 @@ struct HostInst {
     std::vector<double *> allocs;
     size_t bytes = 0;
     int ent_offset = 0;  // the solve's ent_offset switch (drive.f90:32,280)
+    int flux = 0;        // face flux (FluxKind)
 };
 
 struct Group {
     int p = 0;
     std::vector<int> members;
     Blk *d_el = nullptr, *d_slot = nullptr, *d_face = nullptr, *d_bc = nullptr, *d_red1 = nullptr, *d_red2 = nullptr;
-    Blk *d_res = nullptr;  // element map of the residual kernel (its own workgroup size)
+    Blk *d_res = nullptr;  // element map of the residual kernel (its own workgroup size), by face flux kind
+    long long res_off[5] = {0, 0, 0, 0, 0};  // blocks of flux kind f: [res_off[f], res_off[f + 1])
     int *d_list = nullptr;
     long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
     bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
@@ -68,6 +70,7 @@ public:
     long long stage_blocks(int, long long nmap) const { return nmap; }  // one element group per workgroup
     void set_arith(int a) { arith = a; graph_ok = false; }
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
+    int set_flux(int id, int kind);                                      // face flux of one instance (FluxKind)
 
     int active_count() const;
     void invalidate_plan() { plan_ok = false; }  // re-check the active set before the next launch

@@ -88,8 +88,12 @@ __device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long 
 // Build-time shape of the fast residual: waves per SIMD asked of the register
 // allocator, early trace loads, one-round pair buffer.
 #ifndef PPES_FAST_WAVES
-#define PPES_FAST_WAVES 3
+#define PPES_FAST_WAVES 3  // P4
 #endif
+#ifndef PPES_FAST_WAVES_LOW
+#define PPES_FAST_WAVES_LOW 1  // P1-P3
+#endif
+constexpr int resid_waves(int p, bool fast) { return !fast ? 1 : (p == 4 ? PPES_FAST_WAVES : PPES_FAST_WAVES_LOW); }
 #ifndef PPES_FAST_PREFETCH
 #define PPES_FAST_PREFETCH 0
 #endif
@@ -97,8 +101,8 @@ __device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long 
 #define PPES_FAST_ROUND 1
 #endif
 
-template <int P, bool FAST, int TB = resid_tb(P)>
-__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST && P == 4 ? PPES_FAST_WAVES : 1))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
+template <int P, bool FAST, int TB = resid_tb(P), int FLUX = FLUX_ESPP_HLL>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_waves(P, FAST)))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
     using S = Shape<P>;
     constexpr int N = S::N, NN = S::NN, EPB = TB / NN, REC = S::REC;
@@ -351,7 +355,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(FAST &&
             double wl[8], wr[8], tf[8], nB, sl, sr;
             swap_xy_if(yf, tl[r], wl);
             swap_xy_if(yf, tr[r], wr);
-            if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
+            if constexpr (FLUX == FLUX_PP_HLL) pp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
+            else if constexpr (FLUX == FLUX_ESPP_LF) espp_lf_flux_x(wl, wr, g, tf, nB, sl, sr);
+            else if constexpr (FLUX == FLUX_EC_LF) ec_lf_flux_x(wl, wr, g, tf, nB, sl, sr);
+            else if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
             else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
             double *o = &sbuf[el][f * S::FREC];
             swap_xy_if(yf, tf, o);

@@ -698,6 +698,12 @@ extern "C" int ppes_rk_combine(ppes_solver *s, int stage, double dt)
     return s->eng.launch_api(s->id, API_COMBINE, stage, 0, 0);
 }
 
+extern "C" int ppes_set_flux(ppes_solver *s, int kind)
+{
+    if (!s) return PPES_BAD_ARG;
+    return s->eng.set_flux(s->id, kind);
+}
+
 extern "C" int ppes_set_ent_offset(ppes_solver *s, int on)
 {
     if (!s) return PPES_BAD_ARG;
@@ -834,6 +840,12 @@ extern "C" int ppes_batch_download_state(ppes_batch *b, int id, int slot, double
 extern "C" int ppes_batch_download_avg(ppes_batch *b, int id, double *u_avg)
 {
     return b ? inst_download_avg(b->eng, id, u_avg) : PPES_BAD_ARG;
+}
+
+extern "C" int ppes_batch_set_flux(ppes_batch *b, int id, int kind)
+{
+    if (!b) return PPES_BAD_ARG;
+    return b->eng.set_flux(id, kind);
 }
 
 extern "C" int ppes_batch_set_ent_offset(ppes_batch *b, int id, int on)

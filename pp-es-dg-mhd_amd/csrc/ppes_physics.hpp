@@ -178,6 +178,127 @@ __device__ __forceinline__ void espp_hll_flux_x(const double wl[8], const double
 }
 
 // ---------------------------------------------------------------------------
+// The alternative face fluxes of the reference (DG_solver.f90:1073-1497),
+// selectable per instance. They share the state quantities below (the common
+// head of the three routines, in the same order).
+
+enum FluxKind { FLUX_ESPP_HLL = 0, FLUX_PP_HLL = 1, FLUX_ESPP_LF = 2, FLUX_EC_LF = 3 };
+
+struct Side {
+    double rho, bx, vx, vy, vz, p_fld, cfx, phi, ent_flux;
+    double f[8], v[8];
+};
+
+template <bool ENTV>
+__device__ __forceinline__ void side_state(const double w[8], double g, Side &a)
+{
+    const double rho = w[0], mx = w[1], my = w[2], mz = w[3], bx = w[4], by = w[5], bz = w[6], e = w[7];
+    const double inv_rho = 1.0 / rho;
+    const double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
+    const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
+    const double ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
+    const double b_sq = bx * bx + by * by + bz * bz;
+    const double p_mag = 0.5 * b_sq;
+    const double p_fld = dmax((g - 1.0) * (e - (e_knt + p_mag)), PP_EPS);
+    const double p_tot = p_fld + p_mag;
+    const double b_dot_u = bx * vx + by * vy + bz * vz;
+    const double snc_sq = g * p_fld * inv_rho;
+    const double sum_sq = snc_sq + b_sq * inv_rho;
+    a.rho = rho; a.bx = bx; a.vx = vx; a.vy = vy; a.vz = vz; a.p_fld = p_fld;
+    a.cfx = sqrt(0.5 * (sum_sq + sqrt(dmax(sum_sq * sum_sq - 4.0 * snc_sq * inv_rho * (bx * bx), 0.0))));
+    a.f[0] = mx; a.f[1] = vx * mx - bx * bx + p_tot; a.f[2] = vx * my - by * bx;
+    a.f[3] = vx * mz - bz * bx; a.f[4] = 0.0; a.f[5] = -ez; a.f[6] = ey;
+    a.f[7] = vx * (e + p_tot) - bx * b_dot_u;
+    if constexpr (ENTV) {
+        const double inv_theta = rho / p_fld;
+        a.phi = inv_theta * b_dot_u;
+        const double sp_ent = log(p_fld) - g * log(rho);
+        a.ent_flux = (-((rho * sp_ent) / (g - 1.0))) * vx;
+        a.v[0] = (g - sp_ent) / (g - 1.0) - e_knt / p_fld;
+        a.v[1] = mx / p_fld; a.v[2] = my / p_fld; a.v[3] = mz / p_fld;
+        a.v[4] = inv_theta * bx; a.v[5] = inv_theta * by; a.v[6] = inv_theta * bz; a.v[7] = -inv_theta;
+    }
+}
+
+// Roe-average x-velocity and the field-jump term of the PP speed estimates
+__device__ __forceinline__ void pp_speed_terms(const double wl[8], const double wr[8], const Side &L, const Side &R,
+                                               double &vx_roe, double &db_aux)
+{
+    const double sqrt_rho_l = sqrt(L.rho), sqrt_rho_r = sqrt(R.rho);
+    const double sum_sqrt_rho = sqrt_rho_l + sqrt_rho_r;
+    vx_roe = (L.vx * sqrt_rho_l + R.vx * sqrt_rho_r) / sum_sqrt_rho;
+    const double dbx = wl[4] - wr[4], dby = wl[5] - wr[5], dbz = wl[6] - wr[6];
+    db_aux = sqrt(dbx * dbx + dby * dby + dbz * dbz) / sum_sqrt_rho;
+}
+
+// PP_HLL_flux_x (DG_solver.f90:1073-1221): HLL with the PP speeds only
+__device__ __forceinline__ void pp_hll_flux_x(const double wl[8], const double wr[8], double g, double nflux[8],
+                                              double &nBn, double &sl_out, double &sr_out)
+{
+    Side L, R;
+    side_state<false>(wl, g, L);
+    side_state<false>(wr, g, R);
+    double vx_roe, db_aux;
+    pp_speed_terms(wl, wr, L, R, vx_roe, db_aux);
+    const double sl = dmin(dmin(L.vx, vx_roe) - L.cfx - db_aux, 0.0);
+    const double sr = dmax(dmax(R.vx, vx_roe) + R.cfx + db_aux, 0.0);
+    const double dslr = sr - sl, wei_l = sr / dslr, wei_r = -sl / dslr, c = (sl * sr) / dslr;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) nflux[i] = wei_l * L.f[i] + wei_r * R.f[i] + c * (wr[i] - wl[i]);
+    nBn = wei_l * L.bx + wei_r * R.bx;
+    sl_out = sl;
+    sr_out = sr;
+}
+
+// ESPP_LF_flux_x (DG_solver.f90:1223-1395): Lax-Friedrichs at the larger of
+// the PP speed and the semidiscrete ES speed
+__device__ __forceinline__ void espp_lf_flux_x(const double wl[8], const double wr[8], double g, double nflux[8],
+                                               double &nBn, double &sl_out, double &sr_out)
+{
+    Side L, R;
+    side_state<true>(wl, g, L);
+    side_state<true>(wr, g, R);
+    double vx_roe, db_aux;
+    pp_speed_terms(wl, wr, L, R, vx_roe, db_aux);
+    const double sl_pp = dmin(L.vx, vx_roe) - L.cfx - db_aux;
+    const double sr_pp = dmax(dmax(fabs(L.vx) + L.cfx, fabs(R.vx) + R.cfx), fabs(vx_roe) + dmax(L.cfx, R.cfx)) + db_aux;
+    double dot_a = 0.0, dot_b = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dot_a += (R.v[i] - L.v[i]) * (wr[i] - wl[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dot_b += (L.v[i] + R.v[i]) * (R.f[i] - L.f[i]);
+    const double a_form = dmax(dot_a, 0.0) + 1e-9;
+    const double bb_form = 2.0 * (R.ent_flux - L.ent_flux) - dot_b - (L.phi + R.phi) * (wr[4] - wl[4]);
+    const double sr_es = dmax(bb_form / a_form, 0.0);
+    const double sr = dmax(dmax(dmax(sr_es, -sl_pp), sr_pp), 0.0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) nflux[i] = 0.5 * L.f[i] + 0.5 * R.f[i] - 0.5 * sr * (wr[i] - wl[i]);
+    nBn = 0.5 * L.bx + 0.5 * R.bx;
+    sl_out = -sr;
+    sr_out = sr;
+}
+
+// EC_LF_flux_x (DG_solver.f90:1397-1497): the EC flux plus Lax-Friedrichs
+// dissipation at the larger one-sided fast speed
+__device__ __forceinline__ void ec_lf_flux_x(const double wl[8], const double wr[8], double g, double nflux[8],
+                                             double &nBn, double &sl_out, double &sr_out)
+{
+    Side L, R;
+    side_state<false>(wl, g, L);
+    side_state<false>(wr, g, R);
+    const double sr = dmax(fabs(L.vx) + L.cfx, fabs(R.vx) + R.cfx);
+    const Prim pl{L.rho, L.vx, L.vy, L.vz, wl[4], wl[5], wl[6], 0.5 * L.rho / L.p_fld};
+    const Prim pr{R.rho, R.vx, R.vy, R.vz, wr[4], wr[5], wr[6], 0.5 * R.rho / R.p_fld};
+    double ec[8];
+    ec_flux<0>(pl, pr, g, ec);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) nflux[i] = ec[i] - 0.5 * sr * (wr[i] - wl[i]);
+    nBn = 0.5 * (wl[4] + wr[4]);
+    sl_out = -sr;
+    sr_out = sr;
+}
+
+// ---------------------------------------------------------------------------
 // Shared-reciprocal forms of the two flux functions (the library's fast
 // arithmetic mode): the same mathematics with one division per distinct
 // denominator -- (zeta - 1) / (zeta + 1) as (L - R) / (L + R), x / p as
@@ -276,28 +397,30 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     fr[0] = mx_r; fr[1] = vx_r * mx_r - bx_r * bx_r + p_tot_r; fr[2] = vx_r * my_r - by_r * bx_r;
     fr[3] = vx_r * mz_r - bz_r * bx_r; fr[4] = 0.0; fr[5] = -ez_r; fr[6] = ey_r;
     fr[7] = vx_r * (e_r + p_tot_r) - bx_r * b_dot_u_r;
-    // entropy variables (v[4..6] = B / theta, v[7] = -1 / theta)
-    double vl[8], vr[8];
-    vl[0] = (g - sp_ent_l) * inv_gm1 - e_knt_l * inv_p_l;
-    vl[1] = mx_l * inv_p_l; vl[2] = my_l * inv_p_l; vl[3] = mz_l * inv_p_l;
-    vl[4] = inv_theta_l * bx_l; vl[5] = inv_theta_l * by_l; vl[6] = inv_theta_l * bz_l; vl[7] = -inv_theta_l;
-    vr[0] = (g - sp_ent_r) * inv_gm1 - e_knt_r * inv_p_r;
-    vr[1] = mx_r * inv_p_r; vr[2] = my_r * inv_p_r; vr[3] = mz_r * inv_p_r;
-    vr[4] = inv_theta_r * bx_r; vr[5] = inv_theta_r * by_r; vr[6] = inv_theta_r * bz_r; vr[7] = -inv_theta_r;
     const double sqrt_rho_l = sqrt(rho_l), sqrt_rho_r = sqrt(rho_r);
     const double inv_ssr = 1.0 / (sqrt_rho_l + sqrt_rho_r);
     const double vx_roe = (vx_l * sqrt_rho_l + vx_r * sqrt_rho_r) * inv_ssr;
     const double db_aux = sqrt((bx_l - bx_r) * (bx_l - bx_r) + (by_l - by_r) * (by_l - by_r) + (bz_l - bz_r) * (bz_l - bz_r)) * inv_ssr;
     const double sl_pp = dmin(vx_l, vx_roe) - cfx_l - db_aux;
     const double sr_pp = dmax(vx_r, vx_roe) + cfx_r + db_aux;
+    // the three dot products with the entropy variables v (v[4..6] = B / theta,
+    // v[7] = -1 / theta), each v component formed where it is consumed so that
+    // the two vectors never occupy registers whole
     double dot_a = 0.0, dot_bl = 0.0, dot_br = 0.0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    auto term = [&](int i, double v_l, double v_r) {
         const double df = fr[i] - fl[i];
-        dot_a += (vr[i] - vl[i]) * (wr[i] - wl[i]);
-        dot_bl += vl[i] * df;
-        dot_br += vr[i] * df;
-    }
+        dot_a += (v_r - v_l) * (wr[i] - wl[i]);
+        dot_bl += v_l * df;
+        dot_br += v_r * df;
+    };
+    term(0, (g - sp_ent_l) * inv_gm1 - e_knt_l * inv_p_l, (g - sp_ent_r) * inv_gm1 - e_knt_r * inv_p_r);
+    term(1, mx_l * inv_p_l, mx_r * inv_p_r);
+    term(2, my_l * inv_p_l, my_r * inv_p_r);
+    term(3, mz_l * inv_p_l, mz_r * inv_p_r);
+    term(4, inv_theta_l * bx_l, inv_theta_r * bx_r);
+    term(5, inv_theta_l * by_l, inv_theta_r * by_r);
+    term(6, inv_theta_l * bz_l, inv_theta_r * bz_r);
+    term(7, -inv_theta_l, -inv_theta_r);
     const double inv_a = 1.0 / (dmax(dot_a, 0.0) + 1e-8);
     const double dflux = ent_flux_r - ent_flux_l;
     const double bl_form = dflux - dot_bl - (inv_theta_l * b_dot_u_l) * (bx_r - bx_l);

@@ -22,6 +22,7 @@ BC = {"periodic": 0, "extrap": 1, "fixed": 2, "host": 3, "briowu": 4, "cloud": 5
 KERNELS = {"residual": 0, "stage": 1, "oedg": 2, "limit": 3, "reduce": 4, "bc": 5}
 INST_STATE = {0: "idle", 1: "active", 2: "done", 3: "failed"}
 ARITH = {"exact": 0, "fast": 1}
+FLUX = {"espp_hll": 0, "pp_hll": 1, "espp_lf": 2, "ec_lf": 3}
 PROB = {"ot2pi": 0, "rotor": 1, "blast": 2, "otunit": 3, "blastref": 4, "otpert": 5, "alfven": 6, "khi": 7,
         "briowu": 8, "wave1d": 9, "cloud": 10, "jet": 11, "yee": 12}
 
@@ -72,6 +73,8 @@ def lib():
             "ppes_apply_bc": [vp, ctypes.c_int],
             "ppes_error_smooth_alfven": [vp, ctypes.c_int, dp, dp],
             "ppes_set_ent_offset": [vp, ctypes.c_int],
+            "ppes_set_flux": [vp, ctypes.c_int],
+            "ppes_batch_set_flux": [vp, ctypes.c_int, ctypes.c_int],
             "ppes_batch_set_ent_offset": [vp, ctypes.c_int, ctypes.c_int],
             "ppes_error_smooth1d": [vp, ctypes.c_int, ctypes.c_double, dp, dp],
             "ppes_post_process": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -286,6 +289,11 @@ class Solver:
         """"exact" (the reference's rounding order) or "fast" (default)"""
         _check(lib().ppes_set_arith(self.h, ARITH[mode]), "ppes_set_arith")
 
+    def set_flux(self, kind):
+        """face flux: espp_hll (default), pp_hll, espp_lf or ec_lf (ESPP_HLL_flux_x,
+        PP_HLL_flux_x, ESPP_LF_flux_x, EC_LF_flux_x of DG_solver.f90:894-1497)"""
+        _check(lib().ppes_set_flux(self.h, FLUX[kind]), "ppes_set_flux")
+
     def kernel_time(self, kernel):
         """(total ms, launches) of one kernel class while timing is on."""
         ms, n = ctypes.c_double(), ctypes.c_longlong()
@@ -375,6 +383,10 @@ class Batch:
 
     def set_arith(self, mode):
         _check(lib().ppes_batch_set_arith(self.h, ARITH[mode]), "ppes_batch_set_arith")
+
+    def set_flux(self, i, kind):
+        """face flux of instance i (see Solver.set_flux)"""
+        _check(lib().ppes_batch_set_flux(self.h, int(i), FLUX[kind]), "ppes_batch_set_flux")
 
     def kernel_time(self, kernel):
         ms, n = ctypes.c_double(), ctypes.c_longlong()

@@ -118,6 +118,42 @@ LONG_CASES = {
 }
 
 
+def flux_pairs(seed=5, n=400):
+    """Admissible state pairs (conservative) for the face-flux fixture:
+    random states of every scale of the shipped problems, equal pairs,
+    near-vacuum and strongly magnetised ones, mirrored pairs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        pr = []
+        for _ in range(2):
+            rho = float(np.exp(rng.uniform(-6, 3)))
+            v = rng.normal(0, [1.0, 1.0, 0.3]) * float(np.exp(rng.uniform(-3, 3)))
+            b = rng.normal(0, 1.0, 3) * float(np.exp(rng.uniform(-3, 3)))
+            p = float(np.exp(rng.uniform(-8, 7)))
+            pr.append([rho, rho * v[0], rho * v[1], rho * v[2], b[0], b[1], b[2],
+                       p / (5.0 / 3.0 - 1.0) + 0.5 * rho * float(v @ v) + 0.5 * float(b @ b)])
+        if q % 10 == 0:
+            pr[1] = list(pr[0])                     # equal states
+        if q % 10 == 1:
+            pr[1] = list(pr[0])
+            pr[1][0] *= 1.0 + 1e-7                  # log-mean series branch
+        if q % 10 == 2:
+            pr[0][7] = 0.5 * (pr[0][1] ** 2 + pr[0][2] ** 2 + pr[0][3] ** 2) / pr[0][0] + \
+                0.5 * (pr[0][4] ** 2 + pr[0][5] ** 2 + pr[0][6] ** 2) + 1e-13  # pressure at the floor
+        out.append(pr)
+    return np.array(out, dtype=np.float64)  # (n, 2, 8)
+
+
+def run_fluxes():
+    w = flux_pairs()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "pairs.bin")
+        np.ascontiguousarray(w).tofile(path)   # Fortran (8, 2, n) = C (n, 2, 8)
+        data = run({"mode": "fluxes", "init": path, "gamma": 5.0 / 3.0})
+    return {"pairs": w, "gamma": np.array(5.0 / 3.0), "fluxes": data["fluxes"], "_stdout": data["_stdout"]}
+
+
 def steps_from_stdout(txt):
     for line in txt.splitlines():
         if line.strip().startswith("# time steps"):
@@ -135,6 +171,9 @@ def main():
     # tables of gauss.f90 and the LDF projector of LDF.f90, p = 1..6
     if not a.only or a.only == "tables":
         save("tables", {"mode": "tables"}, run({"mode": "tables"}))
+    # the four face fluxes of DG_solver.f90 on fixed state pairs
+    if not a.only or a.only == "fluxes":
+        save("fluxes", {"mode": "fluxes", "gamma": 5.0 / 3.0}, run_fluxes())
     cases = dict(STAGE_CASES)
     cases.update(END_CASES)
     if a.long:
