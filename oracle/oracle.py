@@ -34,6 +34,9 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [ctypes.c_int, dp]
+        L.orc_create2.restype = ctypes.c_void_p
+        L.orc_create2.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double] * 5 + [ctypes.c_int, dp]
+        L.orc_ldf_modal_operator2.argtypes = [ctypes.c_int, ctypes.c_int, dp]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         for name in ("orc_u", "orc_uavg"):
             getattr(L, name).restype = dp
@@ -118,17 +121,20 @@ def set_threads(n):
 
 
 class Oracle:
-    """One solver instance of the restated reference (px = py = p)."""
+    """One solver instance of the restated reference: degree p in x and py in
+    y (default py = p; py = 0 is the reference's 1D set-up)."""
 
-    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
-        self.nx, self.ny, self.p, self.N, self.rk = nx, ny, p, p + 1, rk
+    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3, py=None):
+        py = p if py is None else py
+        self.nx, self.ny, self.p, self.py, self.rk = nx, ny, p, py, rk
+        self.N, self.NX, self.NY = p + 1, p + 1, py + 1
         st = np.zeros(8) if bc_state is None else np.asarray(bc_state, dtype=np.float64)
-        self.h = lib().orc_create(nx, ny, p, rk, *[float(v) for v in domain], float(gamma), BC[bc], _dp(st))
+        self.h = lib().orc_create2(nx, ny, p, py, rk, *[float(v) for v in domain], float(gamma), BC[bc], _dp(st))
         if not self.h:
-            raise ValueError("orc_create rejected the configuration")
-        N = self.N
-        self.shape_u = (8, N, N, nx + 2, ny + 2)
-        self.shape_ut = (8, N, N, nx, ny)
+            raise ValueError("orc_create2 rejected the configuration")
+        NX, NY = self.NX, self.NY
+        self.shape_u = (8, NX, NY, nx + 2, ny + 2)
+        self.shape_ut = (8, NX, NY, nx, ny)
         self.shape_avg = (8, nx + 2, ny + 2)
 
     def __del__(self):
@@ -156,11 +162,11 @@ class Oracle:
 
     @property
     def xx(self):
-        return self._view(lib().orc_xx(self.h), (self.N, self.nx))
+        return self._view(lib().orc_xx(self.h), (self.NX, self.nx))
 
     @property
     def yy(self):
-        return self._view(lib().orc_yy(self.h), (self.N, self.ny))
+        return self._view(lib().orc_yy(self.h), (self.NY, self.ny))
 
     def set_tables(self, t):
         f = lambda a: np.asfortranarray(a, dtype=np.float64)

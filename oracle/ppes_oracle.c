@@ -28,15 +28,16 @@
 typedef long double ldbl;
 
 struct orc_solver {
-    int nx, ny, p, N, rk;
+    int nx, ny, px, py, NX, NY, rk; /* degrees and node counts in x and y (py may be 0: N_y = 1) */
     double left, right, bottom, top, gamma, hx, hy;
     int bc_kind, flux;
     double bc_state[8];
-    /* 1D tables (px = py) */
-    double qx[MAXN], qw[MAXN], D[MAXN * MAXN], Pmn[MAXN * MAXN], Pnm[MAXN * MAXN];
+    /* 1D tables in x and in y (DG_solver.f90:82-89) */
+    double qx_x[MAXN], qw_x[MAXN], Dx[MAXN * MAXN], Pmn_x[MAXN * MAXN], Pnm_x[MAXN * MAXN];
+    double qx_y[MAXN], qw_y[MAXN], Dy[MAXN * MAXN], Pmn_y[MAXN * MAXN], Pnm_y[MAXN * MAXN];
     /* LDF projector pieces (LDF.f90:98-103) */
-    double lmx[MAXN], linvmx[MAXN], lD[MAXN * MAXN], lU[MAXN * MAXN], sqrtm2d[MAXN * MAXN],
-        invm2d[MAXN * MAXN], Ldag[MAXN * MAXN];
+    double lmx[MAXN], linvmx[MAXN], lmy[MAXN], linvmy[MAXN], lDx[MAXN * MAXN], lDy[MAXN * MAXN],
+        lUx[MAXN * MAXN], lUy[MAXN * MAXN], sqrtm2d[MAXN * MAXN], invm2d[MAXN * MAXN], Ldag[MAXN * MAXN];
     double *xx, *yy;
     double *u, *urk, *ut, *uavg;
     /* face data (DG_solver.f90:27-31) */
@@ -131,7 +132,11 @@ static void tables_ld(int p, ldbl *x, ldbl *w, ldbl *D, ldbl *Pmn, ldbl *Pnm)
 
 int orc_tables(int p, double *x, double *w, double *D, double *Pmn, double *Pnm)
 {
-    if (p < 1 || p > 6) return ORC_BAD_ARG;
+    if (p < 0 || p > 6) return ORC_BAD_ARG;
+    if (p == 0) {  /* gauss_lobatto(1): x = 0, w = 2 (gauss.f90:35-37); D = 0, P_0 = 1 */
+        x[0] = 0.0; w[0] = 2.0; D[0] = 0.0; Pmn[0] = 1.0; Pnm[0] = 1.0;
+        return ORC_OK;
+    }
     int N = p + 1;
     ldbl lx[MAXN], lw[MAXN], lD[MAXN * MAXN], lPmn[MAXN * MAXN], lPnm[MAXN * MAXN];
     tables_ld(p, lx, lw, lD, lPmn, lPnm);
@@ -184,27 +189,23 @@ static void jacobi_eig_ld(int n, ldbl *A, ldbl *V, ldbl *ev)
             }
 }
 
-/* LDF projector set-up (LDF.f90:114-217): modal mass m_j = 1/(j+1/2), modal
- * derivative D_ij = 2i+1 for odd j-i > 0 (gauss.f90:436-475), and the left
- * singular vectors / values of G = sqrt(m) D sqrt(m)^-1, obtained here from
- * the eigen-decomposition of G G^T (no LAPACK). */
-static void ldf_setup(orc_solver *s)
+/* Legendre modal pieces of one direction (gauss_legendre_matrix,
+ * gauss.f90:436-475): mass m_j = 1/(j+1/2), derivative D_ij = 2i+1 for odd
+ * j-i > 0, and the left singular vectors / values of G = sqrt(m) D
+ * sqrt(m)^-1 sorted descending, obtained from the eigen-decomposition of
+ * G G^T (no LAPACK); the smallest value is exactly 0 (constants). */
+static void ldf_dir(int p, double *m, double *invm, double *Dm, double *U, double *L)
 {
-    int N = s->N, p = s->p;
+    int N = p + 1;
     for (int j = 0; j < N; ++j) {
-        s->lmx[j] = 1.0 / (j + 0.5);
-        s->linvmx[j] = j + 0.5;
-        for (int i = 0; i < N; ++i) s->lD[i + N * j] = (i < j && ((j - i) & 1)) ? 2.0 * i + 1.0 : 0.0;
+        m[j] = 1.0 / (j + 0.5);
+        invm[j] = j + 0.5;
+        for (int i = 0; i < N; ++i) Dm[i + N * j] = (i < j && ((j - i) & 1)) ? 2.0 * i + 1.0 : 0.0;
     }
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
-            s->sqrtm2d[i + N * j] = sqrt(s->lmx[i] * s->lmx[j]);
-            s->invm2d[i + N * j] = s->linvmx[i] * s->linvmx[j];
-        }
     ldbl G[MAXN * MAXN], S[MAXN * MAXN], V[MAXN * MAXN], ev[MAXN];
     for (int j = 0; j < N; ++j)
         for (int i = 0; i < N; ++i)
-            G[i + N * j] = sqrtl(1.0L / (i + 0.5L)) * (ldbl)s->lD[i + N * j] / sqrtl(1.0L / (j + 0.5L));
+            G[i + N * j] = sqrtl(1.0L / (i + 0.5L)) * (ldbl)Dm[i + N * j] / sqrtl(1.0L / (j + 0.5L));
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j) {
             ldbl a = 0.0L;
@@ -212,137 +213,171 @@ static void ldf_setup(orc_solver *s)
             S[i + N * j] = a;
         }
     jacobi_eig_ld(N, S, V, ev);
-    double L[MAXN];
     for (int i = 0; i < N; ++i) {
         L[i] = (double)sqrtl(ev[i] > 0 ? ev[i] : 0.0L);
-        for (int k = 0; k < N; ++k) s->lU[k + N * i] = (double)V[k + N * i];
+        for (int k = 0; k < N; ++k) U[k + N * i] = (double)V[k + N * i];
     }
     L[N - 1] = 0.0;
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i)
-            s->Ldag[i + N * j] = (i < p || j < p) ? 1.0 / (L[i] * L[i] + L[j] * L[j]) : 0.0;
 }
 
-/* LDF_projector%apply (LDF.f90:219-250), combined version at :247-249 */
+/* LDF projector set-up (LDF.f90:114-217) for degrees (px, py) */
+static void ldf_setup(orc_solver *s)
+{
+    int NX = s->NX, NY = s->NY, px = s->px, py = s->py;
+    double Lx[MAXN], Ly[MAXN];
+    ldf_dir(px, s->lmx, s->linvmx, s->lDx, s->lUx, Lx);
+    ldf_dir(py, s->lmy, s->linvmy, s->lDy, s->lUy, Ly);
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
+            s->sqrtm2d[i + NX * j] = sqrt(s->lmx[i] * s->lmy[j]);
+            s->invm2d[i + NX * j] = s->linvmx[i] * s->linvmy[j];
+        }
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i)
+            s->Ldag[i + NX * j] = (i < px || j < py) ? 1.0 / (Lx[i] * Lx[i] + Ly[j] * Ly[j]) : 0.0;
+}
+
+/* LDF_projector%apply (LDF.f90:219-250), combined version at :247-249;
+ * uu1, uu2: (NX, NY) modal coefficients */
 static void ldf_apply(const orc_solver *s, double *uu1, double *uu2)
 {
-    int N = s->N;
-    const double *Dx = s->lD, *U = s->lU;
+    int NX = s->NX, NY = s->NY;
+    const double *Dx = s->lDx, *Dy = s->lDy, *Ux = s->lUx, *Uy = s->lUy;
     double A[MAXN * MAXN], T[MAXN * MAXN], C[MAXN * MAXN], udiv[MAXN * MAXN];
     /* A = sqrtm2d * (Dx uu1 + uu2 Dy^T) */
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
             double a = 0.0, b = 0.0;
-            for (int k = 0; k < N; ++k) a += Dx[i + N * k] * uu1[k + N * j];
-            for (int l = 0; l < N; ++l) b += uu2[i + N * l] * Dx[j + N * l];
-            A[i + N * j] = s->sqrtm2d[i + N * j] * (a + b);
+            for (int k = 0; k < NX; ++k) a += Dx[i + NX * k] * uu1[k + NX * j];
+            for (int l = 0; l < NY; ++l) b += uu2[i + NX * l] * Dy[j + NY * l];
+            A[i + NX * j] = s->sqrtm2d[i + NX * j] * (a + b);
         }
-    /* T = U^T A; C = T U */
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
+    /* T = Ux^T A; C = Ldag * (T Uy) */
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
             double a = 0.0;
-            for (int k = 0; k < N; ++k) a += U[k + N * i] * A[k + N * j];
-            T[i + N * j] = a;
+            for (int k = 0; k < NX; ++k) a += Ux[k + NX * i] * A[k + NX * j];
+            T[i + NX * j] = a;
         }
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
             double a = 0.0;
-            for (int l = 0; l < N; ++l) a += T[i + N * l] * U[l + N * j];
-            C[i + N * j] = s->Ldag[i + N * j] * a;
+            for (int l = 0; l < NY; ++l) a += T[i + NX * l] * Uy[l + NY * j];
+            C[i + NX * j] = s->Ldag[i + NX * j] * a;
         }
-    /* T = U C; udiv = sqrtm2d * (T U^T) */
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
+    /* T = Ux C; udiv = sqrtm2d * (T Uy^T) */
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
             double a = 0.0;
-            for (int k = 0; k < N; ++k) a += U[i + N * k] * C[k + N * j];
-            T[i + N * j] = a;
+            for (int k = 0; k < NX; ++k) a += Ux[i + NX * k] * C[k + NX * j];
+            T[i + NX * j] = a;
         }
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
             double a = 0.0;
-            for (int l = 0; l < N; ++l) a += T[i + N * l] * U[j + N * l];
-            udiv[i + N * j] = s->sqrtm2d[i + N * j] * a;
+            for (int l = 0; l < NY; ++l) a += T[i + NX * l] * Uy[j + NY * l];
+            udiv[i + NX * j] = s->sqrtm2d[i + NX * j] * a;
         }
     double n1[MAXN * MAXN], n2[MAXN * MAXN];
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
             double a = 0.0, b = 0.0;
-            for (int k = 0; k < N; ++k) a += Dx[k + N * i] * udiv[k + N * j];
-            for (int l = 0; l < N; ++l) b += udiv[i + N * l] * Dx[l + N * j];
-            n1[i + N * j] = uu1[i + N * j] - s->invm2d[i + N * j] * a;
-            n2[i + N * j] = uu2[i + N * j] - s->invm2d[i + N * j] * b;
+            for (int k = 0; k < NX; ++k) a += Dx[k + NX * i] * udiv[k + NX * j];
+            for (int l = 0; l < NY; ++l) b += udiv[i + NX * l] * Dy[l + NY * j];
+            n1[i + NX * j] = uu1[i + NX * j] - s->invm2d[i + NX * j] * a;
+            n2[i + NX * j] = uu2[i + NX * j] - s->invm2d[i + NX * j] * b;
         }
-    memcpy(uu1, n1, sizeof(double) * N * N);
-    memcpy(uu2, n2, sizeof(double) * N * N);
+    memcpy(uu1, n1, sizeof(double) * NX * NY);
+    memcpy(uu2, n2, sizeof(double) * NX * NY);
 }
 
-int orc_ldf_modal_operator(int p, double *op)
+int orc_ldf_modal_operator2(int px, int py, double *op)
 {
-    if (p < 1 || p > 6) return ORC_BAD_ARG;
+    if (px < 0 || px > 6 || py < 0 || py > 6) return ORC_BAD_ARG;
     orc_solver s;
     memset(&s, 0, sizeof s);
-    s.p = p; s.N = p + 1;
+    s.px = px; s.py = py; s.NX = px + 1; s.NY = py + 1;
     ldf_setup(&s);
-    int N = s.N, M = 2 * N * N;
+    int NN = s.NX * s.NY, M = 2 * NN;
     for (int j = 0; j < M; ++j) {
         double uu[2 * MAXN * MAXN];
         memset(uu, 0, sizeof uu);
         uu[j] = 1.0;
-        ldf_apply(&s, uu, uu + N * N);
+        ldf_apply(&s, uu, uu + NN);
         for (int i = 0; i < M; ++i) op[i + (size_t)M * j] = uu[i];
     }
     return ORC_OK;
 }
 
+int orc_ldf_modal_operator(int p, double *op)
+{
+    if (p < 1 || p > 6) return ORC_BAD_ARG;
+    return orc_ldf_modal_operator2(p, p, op);
+}
+
 /* ----------------------------------------------------------------- solver */
 
-#define UIDX(s_, v, k, l, i, j) ((v) + 8 * ((size_t)(k) + (s_)->N * ((size_t)(l) + (s_)->N * ((size_t)(i) + ((s_)->nx + 2) * (size_t)(j)))))
-#define UTIDX(s_, v, k, l, i, j) ((v) + 8 * ((size_t)(k) + (s_)->N * ((size_t)(l) + (s_)->N * ((size_t)((i) - 1) + (s_)->nx * (size_t)((j) - 1)))))
+#define UIDX(s_, v, k, l, i, j) ((v) + 8 * ((size_t)(k) + (s_)->NX * ((size_t)(l) + (s_)->NY * ((size_t)(i) + ((s_)->nx + 2) * (size_t)(j)))))
+#define UTIDX(s_, v, k, l, i, j) ((v) + 8 * ((size_t)(k) + (s_)->NX * ((size_t)(l) + (s_)->NY * ((size_t)((i) - 1) + (s_)->nx * (size_t)((j) - 1)))))
 #define AIDX(s_, v, i, j) ((v) + 8 * ((size_t)(i) + ((s_)->nx + 2) * (size_t)(j)))
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+
+static void geometry(orc_solver *s)
+{
+    int NX = s->NX, NY = s->NY;
+    for (int i = 1; i <= s->nx; ++i)
+        for (int k = 0; k < NX; ++k) s->xx[k + NX * (i - 1)] = (0.5 * s->hx) * s->qx_x[k] + (s->left + (i - 0.5) * s->hx);
+    for (int j = 1; j <= s->ny; ++j)
+        for (int k = 0; k < NY; ++k) s->yy[k + NY * (j - 1)] = (0.5 * s->hy) * s->qx_y[k] + (s->bottom + (j - 0.5) * s->hy);
+}
+
+orc_solver *orc_create2(int nx, int ny, int px, int py, int rk_stages, double left, double right,
+                        double bottom, double top, double gamma, int bc_kind, const double *bc_state)
+{
+    if (nx < 1 || ny < 1 || px < 0 || px > 6 || py < 0 || py > 6 || px + py < 1) return NULL;
+    if (rk_stages != 1 && rk_stages != 2 && rk_stages != 3 && rk_stages != 5) return NULL;
+    orc_solver *s = (orc_solver *)calloc(1, sizeof *s);
+    s->nx = nx; s->ny = ny; s->px = px; s->py = py; s->NX = px + 1; s->NY = py + 1; s->rk = rk_stages;
+    s->left = left; s->right = right; s->bottom = bottom; s->top = top; s->gamma = gamma;
+    s->bc_kind = bc_kind;
+    if (bc_state) memcpy(s->bc_state, bc_state, sizeof s->bc_state);
+    int NX = s->NX, NY = s->NY;
+    orc_tables(px, s->qx_x, s->qw_x, s->Dx, s->Pmn_x, s->Pnm_x);
+    orc_tables(py, s->qx_y, s->qw_y, s->Dy, s->Pmn_y, s->Pnm_y);
+    ldf_setup(s);
+    /* geometry (DG_solver.f90:93-106) */
+    s->hx = (right - left) / nx;
+    s->hy = (top - bottom) / ny;
+    s->xx = (double *)xcalloc((size_t)NX * nx, sizeof(double));
+    s->yy = (double *)xcalloc((size_t)NY * ny, sizeof(double));
+    geometry(s);
+    s->ustride = (size_t)8 * NX * NY * (nx + 2) * (ny + 2);
+    s->utstride = (size_t)8 * NX * NY * nx * ny;
+    s->u = (double *)xcalloc(s->ustride, sizeof(double));
+    s->urk = (double *)xcalloc(s->ustride * rk_stages, sizeof(double));
+    s->ut = (double *)xcalloc(s->utstride * rk_stages, sizeof(double));
+    s->uavg = (double *)xcalloc((size_t)8 * (nx + 2) * (ny + 2), sizeof(double));
+    s->nflux_x = (double *)xcalloc((size_t)8 * NY * (nx + 1) * ny, sizeof(double));
+    s->nflux_y = (double *)xcalloc((size_t)8 * NX * nx * (ny + 1), sizeof(double));
+    s->nB_x = (double *)xcalloc((size_t)NY * (nx + 1) * ny, sizeof(double));
+    s->sxl = (double *)xcalloc((size_t)NY * (nx + 1) * ny, sizeof(double));
+    s->sxr = (double *)xcalloc((size_t)NY * (nx + 1) * ny, sizeof(double));
+    s->nB_y = (double *)xcalloc((size_t)NX * nx * (ny + 1), sizeof(double));
+    s->syd = (double *)xcalloc((size_t)NX * nx * (ny + 1), sizeof(double));
+    s->syu = (double *)xcalloc((size_t)NX * nx * (ny + 1), sizeof(double));
+    s->sig_x = (double *)xcalloc((size_t)8 * (nx + 1) * ny, sizeof(double));
+    s->sig_y = (double *)xcalloc((size_t)8 * nx * (ny + 1), sizeof(double));
+    s->cell_tmp = (double *)xcalloc((size_t)16 * (nx + 2) * (ny + 2), sizeof(double));
+    return s;
+}
 
 orc_solver *orc_create(int nx, int ny, int p, int rk_stages, double left, double right,
                        double bottom, double top, double gamma, int bc_kind,
                        const double *bc_state)
 {
-    if (nx < 1 || ny < 1 || p < 1 || p > 6) return NULL;
-    if (rk_stages != 1 && rk_stages != 2 && rk_stages != 3 && rk_stages != 5) return NULL;
-    orc_solver *s = (orc_solver *)calloc(1, sizeof *s);
-    s->nx = nx; s->ny = ny; s->p = p; s->N = p + 1; s->rk = rk_stages;
-    s->left = left; s->right = right; s->bottom = bottom; s->top = top; s->gamma = gamma;
-    s->bc_kind = bc_kind;
-    if (bc_state) memcpy(s->bc_state, bc_state, sizeof s->bc_state);
-    int N = s->N;
-    orc_tables(p, s->qx, s->qw, s->D, s->Pmn, s->Pnm);
-    ldf_setup(s);
-    /* geometry (DG_solver.f90:93-106) */
-    s->hx = (right - left) / nx;
-    s->hy = (top - bottom) / ny;
-    s->xx = (double *)xcalloc((size_t)N * nx, sizeof(double));
-    s->yy = (double *)xcalloc((size_t)N * ny, sizeof(double));
-    for (int i = 1; i <= nx; ++i)
-        for (int k = 0; k < N; ++k) s->xx[k + N * (i - 1)] = (0.5 * s->hx) * s->qx[k] + (left + (i - 0.5) * s->hx);
-    for (int j = 1; j <= ny; ++j)
-        for (int k = 0; k < N; ++k) s->yy[k + N * (j - 1)] = (0.5 * s->hy) * s->qx[k] + (bottom + (j - 0.5) * s->hy);
-    s->ustride = (size_t)8 * N * N * (nx + 2) * (ny + 2);
-    s->utstride = (size_t)8 * N * N * nx * ny;
-    s->u = (double *)xcalloc(s->ustride, sizeof(double));
-    s->urk = (double *)xcalloc(s->ustride * rk_stages, sizeof(double));
-    s->ut = (double *)xcalloc(s->utstride * rk_stages, sizeof(double));
-    s->uavg = (double *)xcalloc((size_t)8 * (nx + 2) * (ny + 2), sizeof(double));
-    s->nflux_x = (double *)xcalloc((size_t)8 * N * (nx + 1) * ny, sizeof(double));
-    s->nflux_y = (double *)xcalloc((size_t)8 * N * nx * (ny + 1), sizeof(double));
-    s->nB_x = (double *)xcalloc((size_t)N * (nx + 1) * ny, sizeof(double));
-    s->sxl = (double *)xcalloc((size_t)N * (nx + 1) * ny, sizeof(double));
-    s->sxr = (double *)xcalloc((size_t)N * (nx + 1) * ny, sizeof(double));
-    s->nB_y = (double *)xcalloc((size_t)N * nx * (ny + 1), sizeof(double));
-    s->syd = (double *)xcalloc((size_t)N * nx * (ny + 1), sizeof(double));
-    s->syu = (double *)xcalloc((size_t)N * nx * (ny + 1), sizeof(double));
-    s->sig_x = (double *)xcalloc((size_t)8 * (nx + 1) * ny, sizeof(double));
-    s->sig_y = (double *)xcalloc((size_t)8 * nx * (ny + 1), sizeof(double));
-    s->cell_tmp = (double *)xcalloc((size_t)16 * (nx + 2) * (ny + 2), sizeof(double));
-    return s;
+    if (p < 1) return NULL;
+    return orc_create2(nx, ny, p, p, rk_stages, left, right, bottom, top, gamma, bc_kind, bc_state);
 }
 
 void orc_destroy(orc_solver *s)
@@ -358,16 +393,22 @@ void orc_destroy(orc_solver *s)
 void orc_set_tables(orc_solver *s, const double *x, const double *w, const double *D,
                     const double *Pmn, const double *Pnm)
 {
-    int N = s->N;
-    memcpy(s->qx, x, sizeof(double) * N);
-    memcpy(s->qw, w, sizeof(double) * N);
-    memcpy(s->D, D, sizeof(double) * N * N);
-    memcpy(s->Pmn, Pmn, sizeof(double) * N * N);
-    memcpy(s->Pnm, Pnm, sizeof(double) * N * N);
-    for (int i = 1; i <= s->nx; ++i)
-        for (int k = 0; k < N; ++k) s->xx[k + N * (i - 1)] = (0.5 * s->hx) * s->qx[k] + (s->left + (i - 0.5) * s->hx);
-    for (int j = 1; j <= s->ny; ++j)
-        for (int k = 0; k < N; ++k) s->yy[k + N * (j - 1)] = (0.5 * s->hy) * s->qx[k] + (s->bottom + (j - 0.5) * s->hy);
+    /* the tables of degree px; also those of y when py = px (a y degree of 0
+     * keeps its exact one-node tables) */
+    int N = s->NX;
+    memcpy(s->qx_x, x, sizeof(double) * N);
+    memcpy(s->qw_x, w, sizeof(double) * N);
+    memcpy(s->Dx, D, sizeof(double) * N * N);
+    memcpy(s->Pmn_x, Pmn, sizeof(double) * N * N);
+    memcpy(s->Pnm_x, Pnm, sizeof(double) * N * N);
+    if (s->py == s->px) {
+        memcpy(s->qx_y, x, sizeof(double) * N);
+        memcpy(s->qw_y, w, sizeof(double) * N);
+        memcpy(s->Dy, D, sizeof(double) * N * N);
+        memcpy(s->Pmn_y, Pmn, sizeof(double) * N * N);
+        memcpy(s->Pnm_y, Pnm, sizeof(double) * N * N);
+    }
+    geometry(s);
 }
 
 double *orc_u(orc_solver *s) { return s->u; }
@@ -701,16 +742,16 @@ static void jet_nozzle_state(double g, double *uc)
  * BC_BrioWu (:350-384), BC_cloud_shock (:425-472), BC_jet1_cell_ext (:474-595). */
 void orc_apply_bc(orc_solver *s, double *u)
 {
-    int nx = s->nx, ny = s->ny, N = s->N;
-    size_t elem = (size_t)8 * N * N;
+    int nx = s->nx, ny = s->ny, NX = s->NX, NY = s->NY;
+    size_t elem = (size_t)8 * NX * NY;
     if (s->bc_kind == ORC_BC_BRIOWU) {
         /* x: trace extrapolation, y: periodic */
         for (int j = 1; j <= ny; ++j)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) {
                         u[UIDX(s, v, k, l, 0, j)] = u[UIDX(s, v, 0, l, 1, j)];
-                        u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, N - 1, l, nx, j)];
+                        u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, NX - 1, l, nx, j)];
                     }
         for (int i = 1; i <= nx; ++i) {
             memcpy(u + UIDX(s, 0, 0, 0, i, 0), u + UIDX(s, 0, 0, 0, i, ny), elem * sizeof(double));
@@ -723,8 +764,8 @@ void orc_apply_bc(orc_solver *s, double *u)
         cloud_inflow_state(s->gamma, inflow);
         for (int j = 1; j <= ny; ++j) {
             memcpy(u + UIDX(s, 0, 0, 0, 0, j), u + UIDX(s, 0, 0, 0, 1, j), elem * sizeof(double));
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) u[UIDX(s, v, k, l, nx + 1, j)] = inflow[v];
         }
         for (int i = 1; i <= nx; ++i) {
@@ -737,10 +778,10 @@ void orc_apply_bc(orc_solver *s, double *u)
         double nozzle[8];
         jet_nozzle_state(s->gamma, nozzle);
         for (int j = 1; j <= ny; ++j)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
                     /* mirror symmetry at x = left: normal momentum and normal B change sign */
-                    const int km = N - 1 - k;
+                    const int km = NX - 1 - k;
                     u[UIDX(s, 0, k, l, 0, j)] = u[UIDX(s, 0, km, l, 1, j)];
                     u[UIDX(s, 1, k, l, 0, j)] = -u[UIDX(s, 1, km, l, 1, j)];
                     u[UIDX(s, 2, k, l, 0, j)] = u[UIDX(s, 2, km, l, 1, j)];
@@ -752,8 +793,8 @@ void orc_apply_bc(orc_solver *s, double *u)
                     for (int v = 0; v < 8; ++v) u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, k, l, nx, j)];
                 }
         for (int i = 1; i <= nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) {
                         /* nozzle inflow over the first nx/10 bottom cells */
                         u[UIDX(s, v, k, l, i, 0)] = (i <= nx / 10) ? nozzle[v] : u[UIDX(s, v, k, l, i, 1)];
@@ -772,30 +813,30 @@ void orc_apply_bc(orc_solver *s, double *u)
         }
     } else if (s->bc_kind == ORC_BC_EXTRAP) {
         for (int j = 1; j <= ny; ++j)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) {
                         u[UIDX(s, v, k, l, 0, j)] = u[UIDX(s, v, 0, l, 1, j)];
-                        u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, N - 1, l, nx, j)];
+                        u[UIDX(s, v, k, l, nx + 1, j)] = u[UIDX(s, v, NX - 1, l, nx, j)];
                     }
         for (int i = 1; i <= nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) {
                         u[UIDX(s, v, k, l, i, 0)] = u[UIDX(s, v, k, 0, i, 1)];
-                        u[UIDX(s, v, k, l, i, ny + 1)] = u[UIDX(s, v, k, N - 1, i, ny)];
+                        u[UIDX(s, v, k, l, i, ny + 1)] = u[UIDX(s, v, k, NY - 1, i, ny)];
                     }
     } else {
         for (int j = 1; j <= ny; ++j)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) {
                         u[UIDX(s, v, k, l, 0, j)] = s->bc_state[v];
                         u[UIDX(s, v, k, l, nx + 1, j)] = s->bc_state[v];
                     }
         for (int i = 1; i <= nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
                     for (int v = 0; v < 8; ++v) {
                         u[UIDX(s, v, k, l, i, 0)] = s->bc_state[v];
                         u[UIDX(s, v, k, l, i, ny + 1)] = s->bc_state[v];
@@ -952,13 +993,13 @@ static void con_yee(double x, double y, double *uc)
 /* init_by_fun_pri / init_by_fun_con (problems.f90:25-72) */
 int orc_init_problem(orc_solver *s, int prob, const double *pert, int nk, double amp)
 {
-    int N = s->N;
+    int NX = s->NX, NY = s->NY;
     if (prob < ORC_PROB_OT2PI || prob > ORC_PROB_YEE) return ORC_BAD_ARG;
     for (int j = 1; j <= s->ny; ++j)
         for (int i = 1; i <= s->nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
-                    double x = s->xx[k + N * (i - 1)], y = s->yy[l + N * (j - 1)], up[8], uc[8];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
+                    double x = s->xx[k + NX * (i - 1)], y = s->yy[l + NY * (j - 1)], up[8], uc[8];
                     switch (prob) {
                     case ORC_PROB_OT2PI: pri_ot2pi(x, y, s->gamma, up); break;
                     case ORC_PROB_ROTOR: pri_rotor(x, y, up); break;
@@ -986,18 +1027,18 @@ int orc_init_problem(orc_solver *s, int prob, const double *pert, int nk, double
 void orc_error_smooth_alfven(const orc_solver *s, const double *u, double *err_l2, double *err_max)
 {
     const double pi = 3.14159265358979323846, alpha = pi / 6.0;
-    int N = s->N;
+    int NX = s->NX, NY = s->NY;
     double l2 = 0.0, mx = 0.0;
     for (int j = 1; j <= s->ny; ++j)
         for (int i = 1; i <= s->nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
-                    const double x = s->xx[k + N * (i - 1)], y = s->yy[l + N * (j - 1)];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
+                    const double x = s->xx[k + NX * (i - 1)], y = s->yy[l + NY * (j - 1)];
                     const double x_para = x * cos(alpha) + y * sin(alpha);
                     const double B_perp = 0.1 * sin(2 * pi * x_para);
                     const double Bh = u[UIDX(s, 4, k, l, i, j)] * (-sin(alpha)) + u[UIDX(s, 5, k, l, i, j)] * cos(alpha);
                     const double d = Bh - B_perp;
-                    l2 = l2 + ((((0.25 * s->hx) * s->hy) * s->qw[k]) * s->qw[l]) * (d * d);
+                    l2 = l2 + ((((0.25 * s->hx) * s->hy) * s->qw_x[k]) * s->qw_y[l]) * (d * d);
                     mx = dmax(mx, fabs(d));
                 }
     *err_l2 = sqrt(l2);
@@ -1008,13 +1049,13 @@ void orc_error_smooth_alfven(const orc_solver *s, const double *u, double *err_l
  * variables against the exact density wave at time T, serial order */
 void orc_error_smooth1d(const orc_solver *s, const double *u, double T, double *err_l2, double *err_max)
 {
-    int N = s->N;
+    int NX = s->NX, NY = s->NY;
     double l2 = 0.0, mx = 0.0;
     for (int j = 1; j <= s->ny; ++j)
         for (int i = 1; i <= s->nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
-                    const double x = s->xx[k + N * (i - 1)];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
+                    const double x = s->xx[k + NX * (i - 1)];
                     double up[8] = {1.0 + 0.2 * sin(x - T), 1.0, 0.0, 0.0, 0.5, 1.0, 1.5, 2.0}, ex[8];
                     pri2con(up, ex, s->gamma);
                     double sq = 0.0, am = 0.0;
@@ -1023,7 +1064,7 @@ void orc_error_smooth1d(const orc_solver *s, const double *u, double T, double *
                         sq = sq + d * d;
                         am = dmax(am, fabs(d));
                     }
-                    l2 = l2 + ((((0.25 * s->hx) * s->hy) * s->qw[k]) * s->qw[l]) * sq;
+                    l2 = l2 + ((((0.25 * s->hx) * s->hy) * s->qw_x[k]) * s->qw_y[l]) * sq;
                     mx = dmax(mx, am);
                 }
     *err_l2 = sqrt(l2);
@@ -1036,54 +1077,55 @@ void orc_error_smooth1d(const orc_solver *s, const double *u, double T, double *
  * (DG_solver.f90:296-343); ux/uy 1D tables; out[8] */
 static void oedg_face_x(const orc_solver *s, const double *uL, const double *uR, double *out)
 {
-    int N = s->N, px = s->p, py = s->p;
+    int NX = s->NX, NY = s->NY, px = s->px, py = s->py;
     double hx = s->hx, hy = s->hy;
     int mxp = px > py ? px : py;
     double Dx2[MAXN * MAXN], Dy2[MAXN * MAXN];
-    for (int i = 0; i < N * N; ++i) { Dx2[i] = (2.0 / hx) * s->D[i]; Dy2[i] = (2.0 / hy) * s->D[i]; }
+    for (int q = 0; q < NX * NX; ++q) Dx2[q] = (2.0 / hx) * s->Dx[q];
+    for (int q = 0; q < NY * NY; ++q) Dy2[q] = (2.0 / hy) * s->Dy[q];
     for (int v = 0; v < 8; ++v) {
         double ll[MAXN * MAXN], rr[MAXN * MAXN], tmp[MAXN * MAXN], jump[MAXN], jt[MAXN];
-        for (int l = 0; l < N; ++l)
-            for (int k = 0; k < N; ++k) { ll[k + N * l] = uL[v + 8 * (k + N * l)]; rr[k + N * l] = uR[v + 8 * (k + N * l)]; }
+        for (int l = 0; l < NY; ++l)
+            for (int k = 0; k < NX; ++k) { ll[k + NX * l] = uL[v + 8 * (k + NX * l)]; rr[k + NX * l] = uR[v + 8 * (k + NX * l)]; }
         double total = 0.0;
         int fac_kk = 1;
         double hx_pow_kk = 1.0;
         for (int kk = 0; kk <= px; ++kk) {
-            for (int l = 0; l < N; ++l) jump[l] = rr[0 + N * l] - ll[(N - 1) + N * l];
+            for (int l = 0; l < NY; ++l) jump[l] = rr[0 + NX * l] - ll[(NX - 1) + NX * l];
             int fac_ll = 1;
             double hx_pow = hx_pow_kk;
             for (int lq = 0; lq <= mxp - kk; ++lq) {
                 int mm = kk + lq;
                 double jl1 = 0.0;
-                for (int l = 0; l < N; ++l) jl1 += (0.5 * s->qw[l]) * fabs(jump[l]);
+                for (int l = 0; l < NY; ++l) jl1 += (0.5 * s->qw_y[l]) * fabs(jump[l]);
                 double coef = ((double)(2 * mm + 1) * hx_pow) / (double)(2 * (2 * px - 1 > 1 ? 2 * px - 1 : 1) * fac_kk * fac_ll);
                 total = total + coef * jl1;
                 if (lq < py) {
-                    for (int a = 0; a < N; ++a) {
+                    for (int a = 0; a < NY; ++a) {
                         double acc = 0.0;
-                        for (int b = 0; b < N; ++b) acc += Dy2[a + N * b] * jump[b];
+                        for (int b = 0; b < NY; ++b) acc += Dy2[a + NY * b] * jump[b];
                         jt[a] = acc;
                     }
-                    memcpy(jump, jt, sizeof(double) * N);
+                    memcpy(jump, jt, sizeof(double) * NY);
                     fac_ll = fac_ll * (lq + 1);
                     hx_pow = hx_pow * hx;
                 }
             }
             if (kk < px) {
-                for (int l = 0; l < N; ++l)
-                    for (int a = 0; a < N; ++a) {
+                for (int l = 0; l < NY; ++l)
+                    for (int a = 0; a < NX; ++a) {
                         double acc = 0.0;
-                        for (int b = 0; b < N; ++b) acc += Dx2[a + N * b] * ll[b + N * l];
-                        tmp[a + N * l] = acc;
+                        for (int b = 0; b < NX; ++b) acc += Dx2[a + NX * b] * ll[b + NX * l];
+                        tmp[a + NX * l] = acc;
                     }
-                memcpy(ll, tmp, sizeof(double) * N * N);
-                for (int l = 0; l < N; ++l)
-                    for (int a = 0; a < N; ++a) {
+                memcpy(ll, tmp, sizeof(double) * NX * NY);
+                for (int l = 0; l < NY; ++l)
+                    for (int a = 0; a < NX; ++a) {
                         double acc = 0.0;
-                        for (int b = 0; b < N; ++b) acc += Dx2[a + N * b] * rr[b + N * l];
-                        tmp[a + N * l] = acc;
+                        for (int b = 0; b < NX; ++b) acc += Dx2[a + NX * b] * rr[b + NX * l];
+                        tmp[a + NX * l] = acc;
                     }
-                memcpy(rr, tmp, sizeof(double) * N * N);
+                memcpy(rr, tmp, sizeof(double) * NX * NY);
                 fac_kk = fac_kk * (kk + 1);
                 hx_pow_kk = hx_pow_kk * hx;
             }
@@ -1096,54 +1138,55 @@ static void oedg_face_x(const orc_solver *s, const double *uL, const double *uR,
  * (DG_solver.f90:360-407) */
 static void oedg_face_y(const orc_solver *s, const double *uB, const double *uT, double *out)
 {
-    int N = s->N, px = s->p, py = s->p;
+    int NX = s->NX, NY = s->NY, px = s->px, py = s->py;
     double hx = s->hx, hy = s->hy;
     int mxp = px > py ? px : py;
     double Dx2[MAXN * MAXN], Dy2[MAXN * MAXN];
-    for (int i = 0; i < N * N; ++i) { Dx2[i] = (2.0 / hx) * s->D[i]; Dy2[i] = (2.0 / hy) * s->D[i]; }
+    for (int q = 0; q < NX * NX; ++q) Dx2[q] = (2.0 / hx) * s->Dx[q];
+    for (int q = 0; q < NY * NY; ++q) Dy2[q] = (2.0 / hy) * s->Dy[q];
     for (int v = 0; v < 8; ++v) {
         double dd[MAXN * MAXN], uu[MAXN * MAXN], tmp[MAXN * MAXN], jump[MAXN], jt[MAXN];
-        for (int l = 0; l < N; ++l)
-            for (int k = 0; k < N; ++k) { dd[k + N * l] = uB[v + 8 * (k + N * l)]; uu[k + N * l] = uT[v + 8 * (k + N * l)]; }
+        for (int l = 0; l < NY; ++l)
+            for (int k = 0; k < NX; ++k) { dd[k + NX * l] = uB[v + 8 * (k + NX * l)]; uu[k + NX * l] = uT[v + 8 * (k + NX * l)]; }
         double total = 0.0;
         int fac_ll = 1;
         double hy_pow_ll = 1.0;
         for (int lq = 0; lq <= py; ++lq) {
-            for (int k = 0; k < N; ++k) jump[k] = uu[k + N * 0] - dd[k + N * (N - 1)];
+            for (int k = 0; k < NX; ++k) jump[k] = uu[k + NX * 0] - dd[k + NX * (NY - 1)];
             int fac_kk = 1;
             double hy_pow = hy_pow_ll;
             for (int kk = 0; kk <= mxp - lq; ++kk) {
                 int mm = kk + lq;
                 double jl1 = 0.0;
-                for (int k = 0; k < N; ++k) jl1 += (0.5 * s->qw[k]) * fabs(jump[k]);
+                for (int k = 0; k < NX; ++k) jl1 += (0.5 * s->qw_x[k]) * fabs(jump[k]);
                 double coef = ((double)(2 * mm + 1) * hy_pow) / (double)(2 * (2 * py - 1 > 1 ? 2 * py - 1 : 1) * fac_kk * fac_ll);
                 total = total + coef * jl1;
                 if (kk < px) {
-                    for (int a = 0; a < N; ++a) {
+                    for (int a = 0; a < NX; ++a) {
                         double acc = 0.0;
-                        for (int b = 0; b < N; ++b) acc += Dx2[a + N * b] * jump[b];
+                        for (int b = 0; b < NX; ++b) acc += Dx2[a + NX * b] * jump[b];
                         jt[a] = acc;
                     }
-                    memcpy(jump, jt, sizeof(double) * N);
+                    memcpy(jump, jt, sizeof(double) * NX);
                     fac_kk = fac_kk * (kk + 1);
                     hy_pow = hy_pow * hy;
                 }
             }
             if (lq < py) {
-                for (int a = 0; a < N; ++a)
-                    for (int k = 0; k < N; ++k) {
+                for (int a = 0; a < NY; ++a)
+                    for (int k = 0; k < NX; ++k) {
                         double acc = 0.0;
-                        for (int b = 0; b < N; ++b) acc += dd[k + N * b] * Dy2[a + N * b];
-                        tmp[k + N * a] = acc;
+                        for (int b = 0; b < NY; ++b) acc += dd[k + NX * b] * Dy2[a + NY * b];
+                        tmp[k + NX * a] = acc;
                     }
-                memcpy(dd, tmp, sizeof(double) * N * N);
-                for (int a = 0; a < N; ++a)
-                    for (int k = 0; k < N; ++k) {
+                memcpy(dd, tmp, sizeof(double) * NX * NY);
+                for (int a = 0; a < NY; ++a)
+                    for (int k = 0; k < NX; ++k) {
                         double acc = 0.0;
-                        for (int b = 0; b < N; ++b) acc += uu[k + N * b] * Dy2[a + N * b];
-                        tmp[k + N * a] = acc;
+                        for (int b = 0; b < NY; ++b) acc += uu[k + NX * b] * Dy2[a + NY * b];
+                        tmp[k + NX * a] = acc;
                     }
-                memcpy(uu, tmp, sizeof(double) * N * N);
+                memcpy(uu, tmp, sizeof(double) * NX * NY);
                 fac_ll = fac_ll * (lq + 1);
                 hy_pow_ll = hy_pow_ll * hy;
             }
@@ -1166,7 +1209,7 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
                         double oedg_dt, int ent_offset, int comput_ent, int *neg_avg_out, int *num_negcell_out,
                         double *tot_ent_out)
 {
-    int nx = s->nx, ny = s->ny, N = s->N;
+    int nx = s->nx, ny = s->ny, NX = s->NX, NY = s->NY;
     double g = s->gamma, hx = s->hx, hy = s->hy;
     int nxg = nx + 2, nyg = ny + 2;
     double u_avg_global[8] = {0}, u_vari_global[8] = {0};
@@ -1183,9 +1226,9 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
             if (!((i >= 1 && i <= nx) || (j >= 1 && j <= ny))) continue;
             double *ue = u + UIDX(s, 0, 0, 0, i, j);
             double avg[8] = {0};
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
-                    for (int v = 0; v < 8; ++v) avg[v] = avg[v] + (0.25 * s->qw[k] * s->qw[l]) * ue[v + 8 * (k + N * l)];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
+                    for (int v = 0; v < 8; ++v) avg[v] = avg[v] + (0.25 * s->qw_x[k] * s->qw_y[l]) * ue[v + 8 * (k + NX * l)];
             double den_avg = avg[0];
             double prs_avg = prs_macro(avg, g);
             if (den_avg < PP_EPS || prs_avg < PP_EPS) { cflag[8] = 2.0; continue; }
@@ -1197,65 +1240,65 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
             if (ent_offset) {
                 cflag[13] = ent_macro(den_avg, prs_avg, g);
                 int ok = 1;
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) {
-                        const double *q = ue + 8 * (k + N * l);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) {
+                        const double *q = ue + 8 * (k + NX * l);
                         ok = ok && q[0] >= PP_EPS && prs_macro(q, g) >= PP_EPS;
                     }
                 if (ok) {
                     double ce = 0.0;
-                    for (int l = 0; l < N; ++l)
-                        for (int k = 0; k < N; ++k) {
-                            const double *q = ue + 8 * (k + N * l);
-                            ce = ce + (0.25 * hx * hy * s->qw[k] * s->qw[l]) * ent_macro(q[0], prs_macro(q, g), g);
+                    for (int l = 0; l < NY; ++l)
+                        for (int k = 0; k < NX; ++k) {
+                            const double *q = ue + 8 * (k + NX * l);
+                            ce = ce + (0.25 * hx * hy * s->qw_x[k] * s->qw_y[l]) * ent_macro(q[0], prs_macro(q, g), g);
                         }
                     cflag[12] = ce;
                 }
             }
             if (apply_ldf) {
                 double mv1[MAXN * MAXN], mv2[MAXN * MAXN], t1[MAXN * MAXN], c1[MAXN * MAXN], c2[MAXN * MAXN];
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) {
-                        mv1[k + N * l] = (0.5 * hy) * (ue[4 + 8 * (k + N * l)] - avg[4]);
-                        mv2[k + N * l] = (0.5 * hx) * (ue[5 + 8 * (k + N * l)] - avg[5]);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) {
+                        mv1[k + NX * l] = (0.5 * hy) * (ue[4 + 8 * (k + NX * l)] - avg[4]);
+                        mv2[k + NX * l] = (0.5 * hx) * (ue[5 + 8 * (k + NX * l)] - avg[5]);
                     }
                 /* modal = Pnm_x * V * Pnm_y^T (:247-248) */
                 for (int q = 0; q < 2; ++q) {
                     double *mv = q ? mv2 : mv1, *cc = q ? c2 : c1;
-                    for (int l = 0; l < N; ++l)
-                        for (int a = 0; a < N; ++a) {
+                    for (int l = 0; l < NY; ++l)
+                        for (int a = 0; a < NX; ++a) {
                             double acc = 0.0;
-                            for (int k = 0; k < N; ++k) acc += s->Pnm[a + N * k] * mv[k + N * l];
-                            t1[a + N * l] = acc;
+                            for (int k = 0; k < NX; ++k) acc += s->Pnm_x[a + NX * k] * mv[k + NX * l];
+                            t1[a + NX * l] = acc;
                         }
-                    for (int b = 0; b < N; ++b)
-                        for (int a = 0; a < N; ++a) {
+                    for (int b = 0; b < NY; ++b)
+                        for (int a = 0; a < NX; ++a) {
                             double acc = 0.0;
-                            for (int l = 0; l < N; ++l) acc += t1[a + N * l] * s->Pnm[b + N * l];
-                            cc[a + N * b] = acc;
+                            for (int l = 0; l < NY; ++l) acc += t1[a + NX * l] * s->Pnm_y[b + NY * l];
+                            cc[a + NX * b] = acc;
                         }
                 }
                 ldf_apply(s, c1, c2);
                 /* nodal = Pmn_x * C * Pmn_y^T (:254-255) */
                 for (int q = 0; q < 2; ++q) {
                     double *mv = q ? mv2 : mv1, *cc = q ? c2 : c1;
-                    for (int b = 0; b < N; ++b)
-                        for (int a = 0; a < N; ++a) {
+                    for (int b = 0; b < NY; ++b)
+                        for (int a = 0; a < NX; ++a) {
                             double acc = 0.0;
-                            for (int k = 0; k < N; ++k) acc += s->Pmn[a + N * k] * cc[k + N * b];
-                            t1[a + N * b] = acc;
+                            for (int k = 0; k < NX; ++k) acc += s->Pmn_x[a + NX * k] * cc[k + NX * b];
+                            t1[a + NX * b] = acc;
                         }
-                    for (int l = 0; l < N; ++l)
-                        for (int k = 0; k < N; ++k) {
+                    for (int l = 0; l < NY; ++l)
+                        for (int k = 0; k < NX; ++k) {
                             double acc = 0.0;
-                            for (int b = 0; b < N; ++b) acc += t1[k + N * b] * s->Pmn[l + N * b];
-                            mv[k + N * l] = acc;
+                            for (int b = 0; b < NY; ++b) acc += t1[k + NX * b] * s->Pmn_y[l + NY * b];
+                            mv[k + NX * l] = acc;
                         }
                 }
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) {
-                        ue[4 + 8 * (k + N * l)] = (2.0 / hy) * mv1[k + N * l] + avg[4];
-                        ue[5 + 8 * (k + N * l)] = (2.0 / hx) * mv2[k + N * l] + avg[5];
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) {
+                        ue[4 + 8 * (k + NX * l)] = (2.0 / hy) * mv1[k + NX * l] + avg[4];
+                        ue[5 + 8 * (k + NX * l)] = (2.0 / hx) * mv2[k + NX * l] + avg[5];
                     }
             }
             for (int v = 0; v < 8; ++v) u_avg[AIDX(s, v, i, j)] = avg[v];
@@ -1277,8 +1320,8 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
         /* global variation (:282-294); max is exact in any order */
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i)
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k)
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k)
                         for (int v = 0; v < 8; ++v)
                             u_vari_global[v] = dmax(u_vari_global[v], fabs(u[UIDX(s, v, k, l, i, j)] - u_avg_global[v]));
 #pragma omp parallel for collapse(2) schedule(static) num_threads(OMP_THREADS)
@@ -1301,9 +1344,9 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
             for (int v = 0; v < 8; ++v) avg[v] = u_avg[AIDX(s, v, i, j)];
             if (apply_oedg) {
                 double maxeig_x = 0.0, maxeig_y = 0.0;
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) {
-                        const double *q = ue + 8 * (k + N * l);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) {
+                        const double *q = ue + 8 * (k + NX * l);
                         double rho = dmax(q[0], PP_EPS), mx = q[1], my = q[2], mz = q[3];
                         double bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
                         double inv_rho = 1.0 / rho;
@@ -1335,45 +1378,45 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
                 }
                 double theta = exp(-oedg_dt * dmaxv);
                 for (int v = 0; v < 8; ++v)
-                    for (int l = 0; l < N; ++l)
-                        for (int k = 0; k < N; ++k) ue[v + 8 * (k + N * l)] = avg[v] + theta * (ue[v + 8 * (k + N * l)] - avg[v]);
+                    for (int l = 0; l < NY; ++l)
+                        for (int k = 0; k < NX; ++k) ue[v + 8 * (k + NX * l)] = avg[v] + theta * (ue[v + 8 * (k + NX * l)] - avg[v]);
             }
             /* Zhang-Shu PP limiter (:528-603) */
             int neg_node = 0;
             double den_avg = avg[0];
             double prs_avg = prs_macro(avg, g);
             double den_min = 1.79769313486231571e+308;
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) den_min = dmin(den_min, ue[0 + 8 * (k + N * l)]);
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) den_min = dmin(den_min, ue[0 + 8 * (k + NX * l)]);
             if (den_min < PP_EPS) {
                 neg_node = 1;
                 double theta = dmax(dmin((den_avg - PP_EPS) / (den_avg - den_min), 1.0), 0.0);
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) ue[8 * (k + N * l)] = avg[0] + theta * (ue[8 * (k + N * l)] - avg[0]);
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) ue[8 * (k + N * l)] = dmax(ue[8 * (k + N * l)], PP_EPS);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) ue[8 * (k + NX * l)] = avg[0] + theta * (ue[8 * (k + NX * l)] - avg[0]);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) ue[8 * (k + NX * l)] = dmax(ue[8 * (k + NX * l)], PP_EPS);
             }
             double prs_min = 1.79769313486231571e+308;
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) prs_min = dmin(prs_min, prs_macro(ue + 8 * (k + N * l), g));
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) prs_min = dmin(prs_min, prs_macro(ue + 8 * (k + NX * l), g));
             int fail = 0;
             if (prs_min < PP_EPS) {
                 neg_node = 1;
                 double theta = dmax(dmin((prs_avg - PP_EPS) / (prs_avg - prs_min), 1.0), 0.0);
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k)
-                        for (int v = 0; v < 8; ++v) ue[v + 8 * (k + N * l)] = avg[v] + theta * (ue[v + 8 * (k + N * l)] - avg[v]);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k)
+                        for (int v = 0; v < 8; ++v) ue[v + 8 * (k + NX * l)] = avg[v] + theta * (ue[v + 8 * (k + NX * l)] - avg[v]);
                 int it_count = 0;
                 for (;;) {
                     prs_min = 1.79769313486231571e+308;
-                    for (int l = 0; l < N; ++l)
-                        for (int k = 0; k < N; ++k) prs_min = dmin(prs_min, prs_macro(ue + 8 * (k + N * l), g));
+                    for (int l = 0; l < NY; ++l)
+                        for (int k = 0; k < NX; ++k) prs_min = dmin(prs_min, prs_macro(ue + 8 * (k + NX * l), g));
                     if (prs_min < 0.99 * PP_EPS) {
                         if (it_count > 1) { fail = 1; break; }
                         theta = 0.6 * theta;
-                        for (int l = 0; l < N; ++l)
-                            for (int k = 0; k < N; ++k)
-                                for (int v = 0; v < 8; ++v) ue[v + 8 * (k + N * l)] = avg[v] + theta * (ue[v + 8 * (k + N * l)] - avg[v]);
+                        for (int l = 0; l < NY; ++l)
+                            for (int k = 0; k < NX; ++k)
+                                for (int v = 0; v < 8; ++v) ue[v + 8 * (k + NX * l)] = avg[v] + theta * (ue[v + 8 * (k + NX * l)] - avg[v]);
                     } else
                         break;
                     it_count = it_count + 1;
@@ -1384,26 +1427,26 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
             if (ent_offset && !isnan(cslot[12])) {
                 const double ent_old = cslot[12], ent_avg = cslot[13];
                 double ent_new = 0.0;
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) {
-                        const double *q = ue + 8 * (k + N * l);
-                        ent_new = ent_new + (0.25 * hx * hy * s->qw[k] * s->qw[l]) * ent_macro(q[0], dmax(prs_macro(q, g), PP_EPS), g);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) {
+                        const double *q = ue + 8 * (k + NX * l);
+                        ent_new = ent_new + (0.25 * hx * hy * s->qw_x[k] * s->qw_y[l]) * ent_macro(q[0], dmax(prs_macro(q, g), PP_EPS), g);
                     }
                 if (ent_new > ent_old) {
                     const double theta = dmax(dmin((ent_old - ent_avg) / (ent_new - ent_avg), 1.0), 0.0);
-                    for (int l = 0; l < N; ++l)
-                        for (int k = 0; k < N; ++k)
-                            for (int v = 0; v < 8; ++v) ue[v + 8 * (k + N * l)] = avg[v] + theta * (ue[v + 8 * (k + N * l)] - avg[v]);
+                    for (int l = 0; l < NY; ++l)
+                        for (int k = 0; k < NX; ++k)
+                            for (int v = 0; v < 8; ++v) ue[v + 8 * (k + NX * l)] = avg[v] + theta * (ue[v + 8 * (k + NX * l)] - avg[v]);
                 }
             }
             cslot[9] = neg_node;
             cslot[10] = fail;
             if (comput_ent) {
                 double ce = 0.0;
-                for (int l = 0; l < N; ++l)
-                    for (int k = 0; k < N; ++k) {
-                        const double *q = ue + 8 * (k + N * l);
-                        ce = ce + (0.25 * hx * hy * s->qw[k] * s->qw[l]) * ent_macro(q[0], dmax(prs_macro(q, g), PP_EPS), g);
+                for (int l = 0; l < NY; ++l)
+                    for (int k = 0; k < NX; ++k) {
+                        const double *q = ue + 8 * (k + NX * l);
+                        ce = ce + (0.25 * hx * hy * s->qw_x[k] * s->qw_y[l]) * ent_macro(q[0], dmax(prs_macro(q, g), PP_EPS), g);
                     }
                 cslot[11] = ce;
             }
@@ -1427,25 +1470,25 @@ int orc_post_process_ex(orc_solver *s, double *u, double *u_avg, int apply_oedg,
 /* DG_semidiscretize (DG_solver.f90:663-892) */
 void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim_out, const double *u)
 {
-    int nx = s->nx, ny = s->ny, N = s->N, px = s->p, py = s->p;
+    int nx = s->nx, ny = s->ny, NX = s->NX, NY = s->NY, px = s->px, py = s->py;
     double g = s->gamma, hx = s->hx, hy = s->hy;
     /* x-faces (:680-692) */
 #pragma omp parallel for collapse(2) schedule(static) num_threads(OMP_THREADS)
     for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx + 1; ++i)
-            for (int l = 0; l < N; ++l) {
-                size_t f = (size_t)l + N * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1));
-                face_flux_x(s->flux, u + UIDX(s, 0, N - 1, l, i - 1, j), u + UIDX(s, 0, 0, l, i, j), g,
+            for (int l = 0; l < NY; ++l) {
+                size_t f = (size_t)l + NY * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1));
+                face_flux_x(s->flux, u + UIDX(s, 0, NX - 1, l, i - 1, j), u + UIDX(s, 0, 0, l, i, j), g,
                                 s->nflux_x + 8 * f, s->nB_x + f, s->sxl + f, s->sxr + f);
             }
     /* y-faces (:693-713) */
 #pragma omp parallel for collapse(2) schedule(static) num_threads(OMP_THREADS)
     for (int j = 1; j <= ny + 1; ++j)
         for (int i = 1; i <= nx; ++i)
-            for (int k = 0; k < N; ++k) {
-                size_t f = (size_t)k + N * ((size_t)(i - 1) + nx * (size_t)(j - 1));
+            for (int k = 0; k < NX; ++k) {
+                size_t f = (size_t)k + NX * ((size_t)(i - 1) + nx * (size_t)(j - 1));
                 double t1[8], t2[8], tf[8];
-                swap_xy(u + UIDX(s, 0, k, N - 1, i, j - 1), t1);
+                swap_xy(u + UIDX(s, 0, k, NY - 1, i, j - 1), t1);
                 swap_xy(u + UIDX(s, 0, k, 0, i, j), t2);
                 face_flux_x(s->flux, t1, t2, g, tf, s->nB_y + f, s->syd + f, s->syu + f);
                 swap_xy(tf, s->nflux_y + 8 * f);
@@ -1458,10 +1501,10 @@ void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim_out, const dou
             prim_t pr[MAXN * MAXN];
             double flux[2][MAXN * MAXN][8], gp[MAXN * MAXN][8], divB[MAXN * MAXN], utl[MAXN * MAXN][8];
             double max_sigma_x = 0.0, max_sigma_y = 0.0;
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
                     const double *q = u + UIDX(s, 0, k, l, i, j);
-                    int n = k + N * l;
+                    int n = k + NX * l;
                     double rho = q[0], mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
                     double inv_rho = 1.0 / rho;
                     double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
@@ -1503,76 +1546,76 @@ void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim_out, const dou
             /* dt_lim (:806) */
             double msxr = -1.79769313486231571e+308, msxl = 1.79769313486231571e+308;
             double msyu = -1.79769313486231571e+308, msyd = 1.79769313486231571e+308;
-            for (int l = 0; l < N; ++l) {
-                msxr = dmax(msxr, s->sxr[(size_t)l + N * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1))]);
-                msxl = dmin(msxl, s->sxl[(size_t)l + N * ((size_t)i + (nx + 1) * (size_t)(j - 1))]);
+            for (int l = 0; l < NY; ++l) {
+                msxr = dmax(msxr, s->sxr[(size_t)l + NY * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1))]);
+                msxl = dmin(msxl, s->sxl[(size_t)l + NY * ((size_t)i + (nx + 1) * (size_t)(j - 1))]);
             }
-            for (int k = 0; k < N; ++k) {
-                msyu = dmax(msyu, s->syu[(size_t)k + N * ((size_t)(i - 1) + nx * (size_t)(j - 1))]);
-                msyd = dmin(msyd, s->syd[(size_t)k + N * ((size_t)(i - 1) + nx * (size_t)j)]);
+            for (int k = 0; k < NX; ++k) {
+                msyu = dmax(msyu, s->syu[(size_t)k + NX * ((size_t)(i - 1) + nx * (size_t)(j - 1))]);
+                msyd = dmin(msyd, s->syd[(size_t)k + NX * ((size_t)(i - 1) + nx * (size_t)j)]);
             }
             double ax = dmax(dmax(max_sigma_x, msxr), -msxl);
             double ay = dmax(dmax(max_sigma_y, msyu), -msyd);
             cell[16 * ((size_t)i + (nx + 2) * (size_t)j)] = 1.0 / ((2 * px + 1) * ax / hx + (2 * py + 1) * ay / hy);
             /* divB (:813) */
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
                     double a = 0.0, b = 0.0;
-                    for (int r = 0; r < N; ++r) a += s->D[k + N * r] * pr[r + N * l].bx;
-                    for (int q = 0; q < N; ++q) b += pr[k + N * q].by * s->D[l + N * q];
-                    divB[k + N * l] = (2.0 / hx) * a + (2.0 / hy) * b;
+                    for (int r = 0; r < NX; ++r) a += s->Dx[k + NX * r] * pr[r + NX * l].bx;
+                    for (int q = 0; q < NY; ++q) b += pr[k + NX * q].by * s->Dy[l + NY * q];
+                    divB[k + NX * l] = (2.0 / hx) * a + (2.0 / hy) * b;
                 }
-            for (int n = 0; n < N * N; ++n)
+            for (int n = 0; n < NX * NY; ++n)
                 for (int v = 0; v < 8; ++v) utl[n][v] = 0.0;
             /* flux differencing (:828-851) */
             double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
-                    int n = k + N * l;
-                    double dxx = cx * s->D[k + N * k], dyy = cy * s->D[l + N * l];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
+                    int n = k + NX * l;
+                    double dxx = cx * s->Dx[k + NX * k], dyy = cy * s->Dy[l + NY * l];
                     for (int v = 0; v < 8; ++v)
                         utl[n][v] = utl[n][v] - dxx * flux[0][n][v] - dyy * flux[1][n][v] - divB[n] * gp[n][v];
                     for (int r = 0; r < k; ++r) {
                         double ec[8];
-                        int m = r + N * l;
+                        int m = r + NX * l;
                         ec_flux(&pr[n], &pr[m], g, 0, ec);
-                        double a = cx * s->D[k + N * r], b = cx * s->D[r + N * k];
+                        double a = cx * s->Dx[k + NX * r], b = cx * s->Dx[r + NX * k];
                         for (int v = 0; v < 8; ++v) utl[n][v] = utl[n][v] - a * ec[v];
                         for (int v = 0; v < 8; ++v) utl[m][v] = utl[m][v] - b * ec[v];
                     }
                     for (int q = 0; q < l; ++q) {
                         double ec[8];
-                        int m = k + N * q;
+                        int m = k + NX * q;
                         ec_flux(&pr[n], &pr[m], g, 1, ec);
-                        double a = cy * s->D[l + N * q], b = cy * s->D[q + N * l];
+                        double a = cy * s->Dy[l + NY * q], b = cy * s->Dy[q + NY * l];
                         for (int v = 0; v < 8; ++v) utl[n][v] = utl[n][v] - a * ec[v];
                         for (int v = 0; v < 8; ++v) utl[m][v] = utl[m][v] - b * ec[v];
                     }
                 }
             /* surface terms (:872-882) */
-            double sxa = (2.0 / hx) / s->qw[0], sxb = (2.0 / hx) / s->qw[N - 1];
-            double sya = (2.0 / hy) / s->qw[0], syb = (2.0 / hy) / s->qw[N - 1];
-            for (int l = 0; l < N; ++l) {
-                size_t fL = (size_t)l + N * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1));
-                size_t fR = (size_t)l + N * ((size_t)i + (nx + 1) * (size_t)(j - 1));
-                int n0 = 0 + N * l, n1 = (N - 1) + N * l;
+            double sxa = (2.0 / hx) / s->qw_x[0], sxb = (2.0 / hx) / s->qw_x[NX - 1];
+            double sya = (2.0 / hy) / s->qw_y[0], syb = (2.0 / hy) / s->qw_y[NY - 1];
+            for (int l = 0; l < NY; ++l) {
+                size_t fL = (size_t)l + NY * ((size_t)(i - 1) + (nx + 1) * (size_t)(j - 1));
+                size_t fR = (size_t)l + NY * ((size_t)i + (nx + 1) * (size_t)(j - 1));
+                int n0 = 0 + NX * l, n1 = (NX - 1) + NX * l;
                 for (int v = 0; v < 8; ++v)
                     utl[n0][v] = utl[n0][v] - sxa * (flux[0][n0][v] - s->nflux_x[8 * fL + v] + gp[n0][v] * (pr[n0].bx - s->nB_x[fL]));
                 for (int v = 0; v < 8; ++v)
                     utl[n1][v] = utl[n1][v] + sxb * (flux[0][n1][v] - s->nflux_x[8 * fR + v] + gp[n1][v] * (pr[n1].bx - s->nB_x[fR]));
             }
-            for (int k = 0; k < N; ++k) {
-                size_t fB = (size_t)k + N * ((size_t)(i - 1) + nx * (size_t)(j - 1));
-                size_t fT = (size_t)k + N * ((size_t)(i - 1) + nx * (size_t)j);
-                int n0 = k + N * 0, n1 = k + N * (N - 1);
+            for (int k = 0; k < NX; ++k) {
+                size_t fB = (size_t)k + NX * ((size_t)(i - 1) + nx * (size_t)(j - 1));
+                size_t fT = (size_t)k + NX * ((size_t)(i - 1) + nx * (size_t)j);
+                int n0 = k + NX * 0, n1 = k + NX * (NY - 1);
                 for (int v = 0; v < 8; ++v)
                     utl[n0][v] = utl[n0][v] - sya * (flux[1][n0][v] - s->nflux_y[8 * fB + v] + gp[n0][v] * (pr[n0].by - s->nB_y[fB]));
                 for (int v = 0; v < 8; ++v)
                     utl[n1][v] = utl[n1][v] + syb * (flux[1][n1][v] - s->nflux_y[8 * fT + v] + gp[n1][v] * (pr[n1].by - s->nB_y[fT]));
             }
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k)
-                    for (int v = 0; v < 8; ++v) ut[UTIDX(s, v, k, l, i, j)] = utl[k + N * l][v];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k)
+                    for (int v = 0; v < 8; ++v) ut[UTIDX(s, v, k, l, i, j)] = utl[k + NX * l][v];
         }
     double dt_lim = 1.79769313486231571e+308;
     for (int j = 1; j <= ny; ++j)
@@ -1585,7 +1628,7 @@ void orc_semidiscretize(orc_solver *s, double *ut, double *dt_lim_out, const dou
 static void rk_combine(orc_solver *s, int st, double dt)
 {
     int nx = s->nx, ny = s->ny;
-    size_t ne = (size_t)8 * s->N * s->N;
+    size_t ne = (size_t)8 * s->NX * s->NY;
     double *u = s->u;
 #pragma omp parallel for collapse(2) schedule(static) num_threads(OMP_THREADS)
     for (int j = 1; j <= ny; ++j)
@@ -1664,12 +1707,11 @@ static void rk_time(int rk, int st, double t, const double *tRK, double dt, doub
 
 static double interior_mass(orc_solver *s, const double *u)
 {
-    int N = s->N;
     double m = 0.0;
     for (int j = 1; j <= s->ny; ++j)
         for (int i = 1; i <= s->nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) m += (0.25 * s->hx * s->hy * s->qw[k] * s->qw[l]) * u[UIDX(s, 0, k, l, i, j)];
+            for (int l = 0; l < s->NY; ++l)
+                for (int k = 0; k < s->NX; ++k) m += (0.25 * s->hx * s->hy * s->qw_x[k] * s->qw_y[l]) * u[UIDX(s, 0, k, l, i, j)];
     return m;
 }
 
@@ -1741,7 +1783,7 @@ int orc_run_ex(orc_solver *s, double t_final, int nsteps, double cfl, int oedg_m
         /* commit the interior only (drive.f90:304-311): the ghost layer of
          * u_con keeps its initial values, which a retried residual reads */
         {
-            size_t ne = (size_t)8 * s->N * s->N;
+            size_t ne = (size_t)8 * s->NX * s->NY;
             for (int j = 1; j <= s->ny; ++j)
                 for (int i = 1; i <= s->nx; ++i)
                     memcpy(s->u + UIDX(s, 0, 0, 0, i, j), orc_urk(s, S) + UIDX(s, 0, 0, 0, i, j), ne * sizeof(double));

@@ -10,7 +10,8 @@
  *   u      (8, N, N, 0:nx+1, 0:ny+1)  state with one ghost layer
  *   ut     (8, N, N, nx, ny)          residual, interior only
  *   u_avg  (8, 0:nx+1, 0:ny+1)        cell averages
- * with N = p + 1 and px = py = p.
+ * with N = NX = px + 1 in x and NY = py + 1 in y (the second N of the
+ * layouts is NY).
  */
 #ifndef PPES_ORACLE_H
 #define PPES_ORACLE_H
@@ -47,11 +48,17 @@ int orc_ldf_modal_operator(int p, double *op);
 orc_solver *orc_create(int nx, int ny, int p, int rk_stages, double left, double right,
                        double bottom, double top, double gamma, int bc_kind,
                        const double *bc_state);
+/* degrees px in x and py in y (0..6 each; py = 0 is the reference's 1D set-up) */
+orc_solver *orc_create2(int nx, int ny, int px, int py, int rk_stages, double left, double right,
+                        double bottom, double top, double gamma, int bc_kind, const double *bc_state);
+/* the LDF projector of degrees (px, py) as a dense modal operator of size 2 NX NY */
+int orc_ldf_modal_operator2(int px, int py, double *op);
 void orc_destroy(orc_solver *s);
 /* face flux of the residual (default ORC_FLUX_ESPP_HLL) */
 void orc_set_flux(orc_solver *s, int kind);
 
-/* replace the 1D tables by caller-provided ones (same layout as orc_tables) */
+/* replace the 1D tables of degree px (and of y when py = px) by
+ * caller-provided ones (same layout as orc_tables) */
 void orc_set_tables(orc_solver *s, const double *x, const double *w, const double *D,
                     const double *Pmn, const double *Pnm);
 
@@ -59,8 +66,8 @@ double *orc_u(orc_solver *s);            /* u_con */
 double *orc_urk(orc_solver *s, int st);  /* uRK_con(..., st), st = 1..rk_stages */
 double *orc_ut(orc_solver *s, int st);   /* ut_con(..., st) */
 double *orc_uavg(orc_solver *s);
-const double *orc_xx(orc_solver *s);     /* (N, nx) */
-const double *orc_yy(orc_solver *s);     /* (N, ny) */
+const double *orc_xx(orc_solver *s);     /* (NX, nx) */
+const double *orc_yy(orc_solver *s);     /* (NY, ny) */
 
 /* Nodal point-sampling initial data (problems.f90:25-51) into u_con. For
  * ORC_PROB_OTPERT, pert holds nk rows of (kx, ky, phi, psi) and amp scales it. */

@@ -127,7 +127,7 @@ program ref_harness
 	implicit none
 
 	character(len=64) :: mode = 'stages', prob = 'ot2pi', bc = 'auto'
-	integer :: nx_ = 8, ny_ = -1, p_ = 2, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3, plt = 0, entoff = 0
+	integer :: nx_ = 8, ny_ = -1, p_ = 2, py_ = -1, nsteps = 1, oedg = 1, ldf = 1, dumpsteps = 1000000, rk_ = 3, plt = 0, entoff = 0
 	real :: cfl = 0.2, tfinal = -1.0, oedg_dt_call = 0.01
 	real :: left, right, bottom, top, gam
 	real :: err2(2)
@@ -148,11 +148,11 @@ program ref_harness
 
 	call setup_problem()
 	if (rk_ /= 3 .and. trim(mode) /= 'solve') error stop 'harness: rk /= 3 needs mode=solve'
-	call init_DG_solver(nx_, ny_, p_, p_, rk_, left, right, bottom, top)
+	call init_DG_solver(nx_, ny_, p_, py_, rk_, left, right, bottom, top)
 	call dump_real('xx', xx, [p_+1, nx_])
 	call dump_real('grid_x', grid_x, [nx_+1])
 	call dump_real('grid_y', grid_y, [ny_+1])
-	call dump_real('yy', yy, [p_+1, ny_])
+	call dump_real('yy', yy, [py_+1, ny_])
 
 	select case (trim(mode))
 	case ('stages')
@@ -160,7 +160,7 @@ program ref_harness
 	case ('solve')
 		if (tfinal <= 0.0) error stop 'harness: mode=solve needs tfinal'
 		call solve_execute(tfinal, cfl, oedg >= 1, ldf == 1, entoff == 1)
-		call dump_real('u_final', u_con, [8, p_+1, p_+1, nx_+2, ny_+2])
+		call dump_real('u_final', u_con, [8, p_+1, py_+1, nx_+2, ny_+2])
 		call dump_real('avg_final', u_avg, [8, nx_+2, ny_+2])
 		! the reference's Tecplot writer on the final averages (main.f90:67); its
 		! solve_execute has written entropy.plt (drive.f90:338-341)
@@ -202,6 +202,7 @@ contains
 			case ('nx'); read(val, *) nx_
 			case ('ny'); read(val, *) ny_
 			case ('p'); read(val, *) p_
+			case ('py'); read(val, *) py_
 			case ('rk'); read(val, *) rk_
 			case ('plt'); read(val, *) plt
 			case ('entoff'); read(val, *) entoff
@@ -220,6 +221,7 @@ contains
 			end select
 		end do
 		if (ny_ < 0) ny_ = nx_
+		if (py_ < 0) py_ = p_
 		have_init_file = len_trim(init_path) > 0
 	end subroutine parse_args
 
@@ -330,8 +332,8 @@ contains
 		integer, parameter :: max_try_num = 10
 		integer :: shp_u(5), shp_ut(5), shp_avg(3)
 
-		shp_u = [8, p_+1, p_+1, nx_+2, ny_+2]
-		shp_ut = [8, p_+1, p_+1, nx_, ny_]
+		shp_u = [8, p_+1, py_+1, nx_+2, ny_+2]
+		shp_ut = [8, p_+1, py_+1, nx_, ny_]
 		shp_avg = [8, nx_+2, ny_+2]
 
 		call Init_handle(u_con, xx, yy, nx, ny, px, py)
@@ -454,12 +456,12 @@ contains
 		logical :: neg_avg
 		call Init_handle(u_con, xx, yy, nx, ny, px, py)
 		call BC_handle(u_con, xx, yy, nx, ny, px, py)
-		call dump_real('c_u_in', u_con, [8, p_+1, p_+1, nx_+2, ny_+2])
+		call dump_real('c_u_in', u_con, [8, p_+1, py_+1, nx_+2, ny_+2])
 		call DG_semidiscretize(ut_con(:,:,:,:,:,1), dt_lim, u_con)
-		call dump_real('c_ut', ut_con(:,:,:,:,:,1), [8, p_+1, p_+1, nx_, ny_])
+		call dump_real('c_ut', ut_con(:,:,:,:,:,1), [8, p_+1, py_+1, nx_, ny_])
 		call dump_scalar('c_dtlim', dt_lim)
 		call post_process(u_con, u_avg, neg_avg, num_negcell, apply_oedg=(oedg>=1), apply_LDF=(ldf==1), oedg_dt=oedg_dt_call, ent_offset=.false., comput_ent=.true., tot_ent=tot_ent)
-		call dump_real('c_u_pp', u_con, [8, p_+1, p_+1, nx_+2, ny_+2])
+		call dump_real('c_u_pp', u_con, [8, p_+1, py_+1, nx_+2, ny_+2])
 		call dump_real('c_avg', u_avg, [8, nx_+2, ny_+2])
 		call dump_real('c_info', [merge(1.0, 0.0, neg_avg), real(num_negcell), tot_ent], [3])
 	end subroutine run_calls

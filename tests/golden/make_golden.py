@@ -82,6 +82,10 @@ STAGE_CASES = {
     "stages_khi_p2": dict(mode="stages", prob="khi", nx=6, ny=8, p=2, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
     "stages_jet_p3": dict(mode="stages", prob="jet", nx=10, ny=4, p=3, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
     "stages_cloud_p1": dict(mode="stages", prob="cloud", nx=10, ny=10, p=1, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
+    # the reference's 1D set-ups: degree 0 in y (main.f90:60-81)
+    "stages_briowu_p3_py0_ldf0": dict(mode="stages", prob="briowu", nx=12, ny=1, p=3, py=0, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_wave1d_p3_py0_ldf0": dict(mode="stages", prob="wave1d", nx=10, ny=1, p=3, py=0, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_briowu_p2_py0": dict(mode="stages", prob="briowu", nx=16, ny=1, p=2, py=0, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
     # ent_offset on (convex limiting against the cell entropy before LDF, DG_solver.f90:195-225,605-633)
     "stages_blast_p2_entoff_ldf0": dict(mode="stages", prob="blast", nx=8, ny=7, p=2, nsteps=3, oedg=2, ldf=0, cfl=0.15, dumpsteps=1, entoff=1),
     "stages_ot2pi_p3_entoff_ldf0": dict(mode="stages", prob="ot2pi", nx=6, ny=6, p=3, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1, entoff=1),
@@ -110,6 +114,10 @@ END_CASES = {
     "end_jet_p2_20": dict(mode="solve", prob="jet", nx=20, ny=20, p=2, tfinal=1e-4, oedg=1, ldf=1, cfl=0.6),
     "end_briowu_p2_32": dict(mode="solve", prob="briowu", nx=32, ny=2, p=2, tfinal=0.02, oedg=1, ldf=1, cfl=0.6),
     "end_ot2pi_p2_16_entoff": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.1, oedg=1, ldf=1, cfl=0.6, entoff=1),
+    # main.f90 cases 0 and 1 in full: Brio-Wu and the smooth density wave, 480 x 1 elements, P3 in x, P0 in
+    # y, SSP-RK(5,4), CFL 0.6 (the second with its error functional)
+    "end_briowu_main0": dict(mode="solve", prob="briowu", nx=480, ny=1, p=3, py=0, tfinal=0.2, oedg=1, ldf=1, cfl=0.6, rk=5),
+    "end_wave1d_main1": dict(mode="solve", prob="wave1d", nx=480, ny=1, p=3, py=0, tfinal=1.3, oedg=1, ldf=0, cfl=0.6, rk=5),
 }
 
 LONG_CASES = {

@@ -124,7 +124,7 @@ def replay_first_step(name, inject):
     formula of drive.f90:175-212 (numpy, same IEEE operations)."""
     g, a, s = stage_case(name)
     nx, ny, p = a["nx"], a["ny"], a["p"]
-    o = O.Oracle(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    o = O.Oracle(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"], py=a.get("py"))
     if inject:
         o.set_tables(ref_tables(p))
     o.init_problem(a["prob"])
@@ -172,9 +172,11 @@ def replay_first_step(name, inject):
 LDF0 = ["stages_ot2pi_p2_ldf0", "stages_rotor_p3_ldf0", "stages_blast_p2_ldf0",
         "stages_briowu_p3_ldf0", "stages_cloud_p2_ldf0", "stages_jet_p2_ldf0", "stages_yee_p2_ldf0",
         "stages_alfven_p2_ldf0", "stages_wave1d_p3_ldf0",
-        "stages_blast_p2_entoff_ldf0", "stages_ot2pi_p3_entoff_ldf0"]
+        "stages_blast_p2_entoff_ldf0", "stages_ot2pi_p3_entoff_ldf0",
+        "stages_briowu_p3_py0_ldf0", "stages_wave1d_p3_py0_ldf0"]
 LDF1 = ["stages_ot2pi_p2", "stages_rotor_p3", "stages_blast_p1", "stages_blast_p4", "stages_otunit_p3",
-        "stages_khi_p2", "stages_jet_p3", "stages_cloud_p1", "stages_rotor_p2_entoff", "stages_khi_p3_entoff"]
+        "stages_khi_p2", "stages_jet_p3", "stages_cloud_p1", "stages_rotor_p2_entoff", "stages_khi_p3_entoff",
+        "stages_briowu_p2_py0"]
 
 
 @pytest.mark.parametrize("name", LDF0)
@@ -219,7 +221,7 @@ def test_stages_with_ldf(name):
 @pytest.mark.parametrize("name", LDF0 + LDF1)
 def test_multistep_final_state(name):
     g, a, s = stage_case(name)
-    o = O.Oracle(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"])
+    o = O.Oracle(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"], py=a.get("py"))
     inject = name in LDF0
     if inject:
         o.set_tables(ref_tables(a["p"]))
@@ -238,7 +240,29 @@ def test_multistep_final_state(name):
 END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12",
        "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5",
        "end_alfven_p2_8", "end_wave1d_p3_16_rk5", "end_cloud_p2_16", "end_jet_p2_20", "end_briowu_p2_32",
-       "end_ot2pi_p2_16_entoff"]
+       "end_ot2pi_p2_16_entoff", "end_briowu_main0", "end_wave1d_main1"]
+
+
+# The Brio-Wu strip (480 x 1 elements, P3 x P0) carries round-off noise on its
+# constant states that 1409 RK5 steps amplify: the last-bit differences of the
+# P3 tables (long-double here, closed forms in the reference) become 1e-8 in
+# density. With the reference's own tables the run is bit-identical
+# (test_main_cases_bit_exact_with_reference_tables).
+END_TOL = {"end_briowu_main0": 5e-8}
+
+
+@pytest.mark.parametrize("name", ["end_briowu_main0", "end_wave1d_main1"])
+def test_main_cases_bit_exact_with_reference_tables(name):
+    """main.f90 cases 0 and 1 (degree 0 in y, SSP-RK(5,4)) to t_final, with
+    the reference's 1D tables injected: every density node equals the
+    reference's bit for bit."""
+    g, a, s = stage_case(name)
+    o = O.Oracle(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"], rk=a["rk"], py=a["py"])
+    o.set_tables(ref_tables(a["p"]))
+    o.init_problem(a["prob"])
+    summ = o.run(a["tfinal"], 0, a["cfl"], a["oedg"], a["ldf"])
+    assert summ["steps"] == int(g["steps"])
+    assert np.array_equal(interior(o.u)[0], g["rho_final"])
 
 
 @pytest.mark.parametrize("name", END)
@@ -246,7 +270,8 @@ def test_end_to_end(name):
     """The reference's solve_execute to t_final (RK3, and RK1 / RK2 / RK5 in
     the *_rk* cases)."""
     g, a, s = stage_case(name)
-    o = O.Oracle(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"], rk=a.get("rk", 3))
+    o = O.Oracle(a["nx"], a["ny"], a["p"], s["domain"], s["gamma"], s["bc"], s["bc_state"], rk=a.get("rk", 3),
+                 py=a.get("py"))
     o.init_problem(a["prob"])
     summ = o.run(a["tfinal"], 0, a["cfl"], a["oedg"], a["ldf"], ent_offset=bool(a.get("entoff", 0)))
     if "steps" in g:
@@ -256,12 +281,14 @@ def test_end_to_end(name):
     # t advances through t_RK (drive.f90:312); the RK5 weights of the final
     # stage (drive.f90:243-268) sum to 1 + 1.1e-15, so t drifts by ~1e-15
     assert abs(summ["t"] - a["tfinal"]) <= (1e-14 if a.get("rk") == 5 else 1e-15) * a["tfinal"]
-    assert rel_l1(interior(o.u)[0], g["rho_final"]) <= 1e-12
+    assert rel_l1(interior(o.u)[0], g["rho_final"]) <= END_TOL.get(name, 1e-12)
     if "err" in g:
         # compute_error_smooth_alfven / compute_error_smooth1D of the final state
+        # relative to the error, plus round-off of the state (O(1) values):
+        # the 480-cell P3 wave's error at t = 1.3 is only 4e-11
         e = o.error_smooth_alfven() if a["prob"] == "alfven" else o.error_smooth1d(a["tfinal"])
-        assert abs(e[0] - g["err"][0]) <= 1e-10 * g["err"][0]
-        assert abs(e[1] - g["err"][1]) <= 1e-10 * g["err"][1]
+        assert abs(e[0] - g["err"][0]) <= 1e-10 * g["err"][0] + 1e-13
+        assert abs(e[1] - g["err"][1]) <= 1e-10 * g["err"][1] + 1e-13
 
 
 def test_error_functionals_closed_form():
