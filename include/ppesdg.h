@@ -9,7 +9,9 @@
  *   state     (8, N, N, 0:nx+1, 0:ny+1)   u_con / uRK_con(...,s)   DG_solver.f90:6-14
  *   residual  (8, N, N, nx, ny)           ut_con(...,s)
  *   averages  (8, 0:nx+1, 0:ny+1)         u_avg
- * with N = p + 1 (px = py = p, 1 <= p <= 4). On the device the state is kept
+ * with N = px + 1 in x and py + 1 in y (the second N of each layout is
+ * py + 1). Degrees: px = py in 1..4, or py = 0 with px in 1..4 (the
+ * reference's 1D set-up, main.f90:60-81). On the device the state is kept
  * as structure-of-arrays planes; the library converts at the boundary.
  *
  * State slots: 0 = u_con (the committed solution), 1..rk_stages = uRK_con(s).
@@ -67,7 +69,7 @@ extern "C" {
 
 typedef struct ppes_config {
     int nx, ny;            /* elements */
-    int px, py;            /* polynomial degree; px == py required */
+    int px, py;            /* degrees: px = py in 1..4, or py = 0 with px in 1..4 */
     int rk_stages;         /* 1, 2, 3 or 5 (SSP-RK(1,1),(2,2),(3,3),(5,4)) */
     double left, right, bottom, top;
     double gamma;
@@ -229,7 +231,8 @@ int ppes_batch_set_arith(ppes_batch *b, int mode);
 int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long long *count);
 
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
- * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major). */
+ * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major),
+ * N = p + 1, p in 0..6 (p = 0: the one-node rule x = 0, w = 2). */
 int ppes_tables(int p, double *x, double *w, double *D, double *Pmn, double *Pnm);
 /* The LDF projection (LDF.f90:114-250 with the Piola maps of
  * DG_solver.f90:237-264) as one dense nodal operator on (Bx - avg, By - avg):

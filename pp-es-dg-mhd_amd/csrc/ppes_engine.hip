@@ -29,10 +29,10 @@ namespace ppes {
             ENG_TRY(hipGetLastError());                                                        \
         }                                                                                      \
     } while (0)
-#define RESIDUAL_DISPATCH_F(p, FK, grid, ...)                                                                    \
+#define RESIDUAL_DISPATCH_F(sid, FK, grid, ...)                                                                  \
     do {                                                                                                         \
-        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, true, resid_tb(P), FK>), grid, resid_tb(P), __VA_ARGS__)); } \
-        else { ENG_DISPATCH(p, LAUNCH_TB((k_residual<P, false, resid_tb(P), FK>), grid, resid_tb(P), __VA_ARGS__)); } \
+        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); } \
+        else { ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, false, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); } \
     } while (0)
 // the residual over a group: one launch per face flux kind present
 #define RESIDUAL_DISPATCH(g, ...)                                                                                \
@@ -42,37 +42,37 @@ namespace ppes {
             if (n_ <= 0) continue;                                                                               \
             const Blk *m_ = (g).d_res + (g).res_off[fk_];                                                        \
             switch (fk_) {                                                                                       \
-            case FLUX_PP_HLL: RESIDUAL_DISPATCH_F((g).p, FLUX_PP_HLL, n_, d_recs, m_, __VA_ARGS__); break;      \
-            case FLUX_ESPP_LF: RESIDUAL_DISPATCH_F((g).p, FLUX_ESPP_LF, n_, d_recs, m_, __VA_ARGS__); break;    \
-            case FLUX_EC_LF: RESIDUAL_DISPATCH_F((g).p, FLUX_EC_LF, n_, d_recs, m_, __VA_ARGS__); break;        \
-            default: RESIDUAL_DISPATCH_F((g).p, FLUX_ESPP_HLL, n_, d_recs, m_, __VA_ARGS__); break;             \
+            case FLUX_PP_HLL: RESIDUAL_DISPATCH_F((g).sid, FLUX_PP_HLL, n_, d_recs, m_, __VA_ARGS__); break;      \
+            case FLUX_ESPP_LF: RESIDUAL_DISPATCH_F((g).sid, FLUX_ESPP_LF, n_, d_recs, m_, __VA_ARGS__); break;    \
+            case FLUX_EC_LF: RESIDUAL_DISPATCH_F((g).sid, FLUX_EC_LF, n_, d_recs, m_, __VA_ARGS__); break;        \
+            default: RESIDUAL_DISPATCH_F((g).sid, FLUX_ESPP_HLL, n_, d_recs, m_, __VA_ARGS__); break;             \
             }                                                                                                    \
         }                                                                                                        \
     } while (0)
 
-#define ENG_DISPATCH(p, CALL)                     \
-    switch (p) {                                  \
-    case 1: { constexpr int P = 1; CALL; } break; \
-    case 2: { constexpr int P = 2; CALL; } break; \
-    case 3: { constexpr int P = 3; CALL; } break; \
-    case 4: { constexpr int P = 4; CALL; } break; \
-    default: return PPES_UNSUPPORTED;             \
+// one case per shape id, with PX, PY and the node count NN_ as constants
+// (CALL is substituted in place: it expands to launch syntax whose commas
+// must not pass through another macro)
+#define ENG_CASE_(SID) \
+    case SID: {        \
+        constexpr int PX = sid_px(SID), PY = sid_py(SID), NN_ = (PX + 1) * (PY + 1); (void)NN_;
+#define ENG_DISPATCH(sid, CALL)                      \
+    switch (sid) {                                   \
+        ENG_CASE_(1) CALL; } break;                  \
+        ENG_CASE_(2) CALL; } break;                  \
+        ENG_CASE_(3) CALL; } break;                  \
+        ENG_CASE_(4) CALL; } break;                  \
+        ENG_CASE_(5) CALL; } break;                  \
+        ENG_CASE_(6) CALL; } break;                  \
+        ENG_CASE_(7) CALL; } break;                  \
+        ENG_CASE_(8) CALL; } break;                  \
+    default: return PPES_UNSUPPORTED;                \
     }
 
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
-template <int P>
-static long long epb() { return Shape<P>::EPB; }
-
-static long long epb_of(int p)
-{
-    switch (p) {
-    case 1: return epb<1>();
-    case 2: return epb<2>();
-    case 3: return epb<3>();
-    default: return epb<4>();
-    }
-}
+// elements per workgroup of the stage kernels: BLOCK / NN
+static long long epb_of(int sid) { return BLOCK / ((sid_px(sid) + 1) * (sid_py(sid) + 1)); }
 
 // ------------------------------------------------------------------ timers
 
@@ -258,12 +258,14 @@ int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g)
 
 int Engine::add(const ppes_config &cfg, int *id)
 {
-    if (cfg.nx < 1 || cfg.ny < 1 || cfg.px != cfg.py || cfg.px < 1 || cfg.px > 4) return PPES_BAD_ARG;
+    if (cfg.nx < 1 || cfg.ny < 1 || cfg.px < 0 || cfg.py < 0) return PPES_BAD_ARG;
+    // degrees: px = py in 1..4, or py = 0 (the reference's 1D set-up) with px in 1..4
+    if (shape_id(cfg.px, cfg.py) < 0) return PPES_UNSUPPORTED;
     if (cfg.rk_stages != S) return PPES_BAD_ARG;
     if (!(cfg.right > cfg.left) || !(cfg.top > cfg.bottom) || !(cfg.gamma > 1.0)) return PPES_BAD_ARG;
     if (cfg.bc_kind < PPES_BC_PERIODIC || cfg.bc_kind > PPES_BC_JET) return PPES_BAD_ARG;
     // kernels index one variable plane with 32-bit offsets
-    if ((long long)(cfg.nx + 2) * (cfg.ny + 2) * (cfg.px + 1) * (cfg.px + 1) >= (1LL << 31)) return PPES_UNSUPPORTED;
+    if ((long long)(cfg.nx + 2) * (cfg.ny + 2) * (cfg.px + 1) * (cfg.py + 1) >= (1LL << 31)) return PPES_UNSUPPORTED;
     int slot = -1;
     for (size_t i = 0; i < hosts.size(); ++i)
         if (!hosts[i]) { slot = (int)i; break; }
@@ -277,55 +279,65 @@ int Engine::add(const ppes_config &cfg, int *id)
     if (rc) return rc;
     HostInst *h = new HostInst();
     h->cfg = cfg;
-    h->p = cfg.px;
-    h->N = h->p + 1;
-    h->NN = h->N * h->N;
+    h->px = cfg.px;
+    h->py = cfg.py;
+    h->sid = shape_id(cfg.px, cfg.py);
+    h->NX = h->px + 1;
+    h->NY = h->py + 1;
+    h->NN = h->NX * h->NY;
     h->S = S;
-    const int N = h->N, NN = h->NN, nx = cfg.nx, ny = cfg.ny;
-    h->qx.resize(N);
-    h->qw.resize(N);
-    h->D.resize(NN);
-    host_tables(h->p, h->qx.data(), h->qw.data(), h->D.data(), nullptr, nullptr);
+    const int NX = h->NX, NY = h->NY, NN = h->NN, nx = cfg.nx, ny = cfg.ny;
+    h->qx_x.resize(NX);
+    h->qw_x.resize(NX);
+    h->Dx.resize(NX * NX);
+    h->qx_y.resize(NY);
+    h->qw_y.resize(NY);
+    h->Dy.resize(NY * NY);
+    host_tables(h->px, h->qx_x.data(), h->qw_x.data(), h->Dx.data(), nullptr, nullptr);
+    host_tables(h->py, h->qx_y.data(), h->qw_y.data(), h->Dy.data(), nullptr, nullptr);
     // geometry (DG_solver.f90:93-106)
     h->hx = (cfg.right - cfg.left) / nx;
     h->hy = (cfg.top - cfg.bottom) / ny;
-    h->xx.resize((size_t)N * nx);
-    h->yy.resize((size_t)N * ny);
+    h->xx.resize((size_t)NX * nx);
+    h->yy.resize((size_t)NY * ny);
     for (int i = 1; i <= nx; ++i)
-        for (int k = 0; k < N; ++k) h->xx[k + (size_t)N * (i - 1)] = (0.5 * h->hx) * h->qx[k] + (cfg.left + (i - 0.5) * h->hx);
+        for (int k = 0; k < NX; ++k) h->xx[k + (size_t)NX * (i - 1)] = (0.5 * h->hx) * h->qx_x[k] + (cfg.left + (i - 0.5) * h->hx);
     for (int j = 1; j <= ny; ++j)
-        for (int k = 0; k < N; ++k) h->yy[k + (size_t)N * (j - 1)] = (0.5 * h->hy) * h->qx[k] + (cfg.bottom + (j - 0.5) * h->hy);
+        for (int k = 0; k < NY; ++k) h->yy[k + (size_t)NY * (j - 1)] = (0.5 * h->hy) * h->qx_y[k] + (cfg.bottom + (j - 0.5) * h->hy);
 
     Inst &r = h_recs[slot];
     memset(&r, 0, sizeof r);
     r.nx = nx;
     r.ny = ny;
-    r.p = h->p;
+    r.px = h->px;
+    r.py = h->py;
     r.S = S;
     r.vs = (long long)(nx + 2) * (ny + 2) * NN;
     r.vsi = (long long)nx * ny * NN;
     r.hx = h->hx;
     r.hy = h->hy;
     r.g = cfg.gamma;
-    for (int i = 0; i < N; ++i) r.qw[i] = h->qw[i];
-    for (int i = 0; i < NN; ++i) r.D[i] = h->D[i];
-    for (int l = 0; l < N; ++l)
-        for (int k = 0; k < N; ++k) {
-            r.wavg[k + N * l] = (0.25 * h->qw[k]) * h->qw[l];
-            r.went[k + N * l] = (((0.25 * h->hx) * h->hy) * h->qw[k]) * h->qw[l];
+    for (int i = 0; i < NX; ++i) r.qw_x[i] = h->qw_x[i];
+    for (int i = 0; i < NY; ++i) r.qw_y[i] = h->qw_y[i];
+    for (int i = 0; i < NX * NX; ++i) r.Dx[i] = h->Dx[i];
+    for (int i = 0; i < NY * NY; ++i) r.Dy[i] = h->Dy[i];
+    for (int l = 0; l < NY; ++l)
+        for (int k = 0; k < NX; ++k) {
+            r.wavg[k + NX * l] = (0.25 * h->qw_x[k]) * h->qw_y[l];
+            r.went[k + NX * l] = (((0.25 * h->hx) * h->hy) * h->qw_x[k]) * h->qw_y[l];
         }
-    {
-        // rows N-1 and 0 of the powers of D, in long double (OEDG face traces)
+    // rows N-1 and 0 of the powers of D, in long double (OEDG face traces)
+    auto drows = [](const std::vector<double> &D, int N, double (*out)[MAXN][MAXN]) {
         long double Dl[MAXN][MAXN], Pw[MAXN][MAXN], T[MAXN][MAXN];
         for (int a = 0; a < N; ++a)
             for (int b = 0; b < N; ++b) {
-                Dl[a][b] = h->D[a + N * b];
+                Dl[a][b] = D[a + N * b];
                 Pw[a][b] = (a == b) ? 1.0L : 0.0L;
             }
         for (int o = 0; o < N; ++o) {
             for (int a = 0; a < N; ++a) {
-                r.drow[0][o][a] = (double)Pw[N - 1][a];
-                r.drow[1][o][a] = (double)Pw[0][a];
+                out[0][o][a] = (double)Pw[N - 1][a];
+                out[1][o][a] = (double)Pw[0][a];
             }
             for (int a = 0; a < N; ++a)
                 for (int b = 0; b < N; ++b) {
@@ -336,12 +348,14 @@ int Engine::add(const ppes_config &cfg, int *id)
             for (int a = 0; a < N; ++a)
                 for (int b = 0; b < N; ++b) Pw[a][b] = T[a][b];
         }
-    }
+    };
+    drows(h->Dx, NX, r.drow_x);
+    drows(h->Dy, NY, r.drow_y);
     if (set_bc_rules(r, cfg.bc_kind, cfg.bc_state, nx, cfg.gamma) != PPES_OK) {
         delete h;
         return PPES_BAD_ARG;
     }
-    const long long E = epb_of(h->p);
+    const long long E = epb_of(h->sid);
     r.nblk_el = cdiv((long long)nx * ny, E);
     r.nblk_slot = cdiv((long long)(nx + 2) * (ny + 2), E);
     r.ctl.state = ST_IDLE;
@@ -372,7 +386,7 @@ int Engine::add(const ppes_config &cfg, int *id)
         return PPES_OOM;
     }
     std::vector<double> op((size_t)4 * NN * NN);
-    host_ldf_nodal(h->p, h->hx, h->hy, op.data());
+    host_ldf_nodal(h->px, h->py, h->hx, h->hy, op.data());
     ENG_TRY(hipMemcpyAsync(r.ldf, op.data(), op.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     for (int k = 0; k <= S; ++k) ENG_TRY(hipMemsetAsync(r.u[k], 0, nstate * sizeof(double), stream));
     ENG_TRY(hipMemsetAsync(r.uavg, 0, 8 * (size_t)(nx + 2) * (ny + 2) * sizeof(double), stream));
@@ -441,18 +455,18 @@ static void xcd_order(std::vector<Blk> &v)
     v.swap(out);
 }
 
-int Engine::build_group(Group &g, const std::vector<int> &members, int p)
+int Engine::build_group(Group &g, const std::vector<int> &members, int sid)
 {
-    g.p = p;
+    g.sid = sid;
     g.members = members;
     g.eo = false;
     for (int id : members) g.eo = g.eo || hosts[id]->ent_offset;
     std::vector<Blk> el, sl, fc, bc, r1, r2, rs, rsf[4];
     for (int id : members) {
         const Inst &r = h_recs[id];
-        const int NN = (p + 1) * (p + 1);
+        const int NN = (sid_px(sid) + 1) * (sid_py(sid) + 1);
         for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
-        const long long nres = cdiv((long long)r.nx * r.ny, resid_tb(p) / NN);
+        const long long nres = cdiv((long long)r.nx * r.ny, resid_tb(NN) / NN);
         for (long long b = 0; b < nres; ++b) rsf[hosts[id]->flux].push_back({id, (int)b});
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
         const long long nf = ((long long)(r.nx + 1) * r.ny + (long long)r.nx * (r.ny + 1)) * 8;
@@ -519,13 +533,13 @@ int Engine::build_plan()
     if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
     std::vector<int> all;
     for (int s = 0; s <= MAXS; ++s) any_oedg_stage[s] = false;
-    for (int p = 1; p <= 4; ++p) {
+    for (int sid = 1; sid <= 8; ++sid) {
         std::vector<int> mem;
         for (size_t i = 0; i < hosts.size(); ++i)
-            if (hosts[i] && hosts[i]->p == p && h_recs[i].ctl.state == ST_ACTIVE) mem.push_back((int)i);
+            if (hosts[i] && hosts[i]->sid == sid && h_recs[i].ctl.state == ST_ACTIVE) mem.push_back((int)i);
         if (mem.empty()) continue;
         Group g;
-        int rc = build_group(g, mem, p);
+        int rc = build_group(g, mem, sid);
         if (rc) return rc;
         groups.push_back(g);
         for (int id : mem) {
@@ -579,7 +593,7 @@ int Engine::launch_step(bool all_kernels)
         for (const auto &g : groups)
             for (int id : g.members) any_retry = any_retry || h_recs[id].ctl.retry_residual;
     if (any_retry) {
-        for (auto &g : groups) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 1));
+        for (auto &g : groups) ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 1));
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 1);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
@@ -590,9 +604,9 @@ int Engine::launch_step(bool all_kernels)
         for (auto &g : groups) {
             Timed tm(this, KT_STAGE);
             if (g.eo) {
-                ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, true>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+                ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, true>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
             } else {
-                ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+                ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
             }
         }
         if (any_oedg_stage[st]) {
@@ -604,14 +618,14 @@ int Engine::launch_step(bool all_kernels)
                 }
                 {
                     Timed tm(this, KT_OEDG);
-                    ENG_DISPATCH(g.p, LAUNCH(k_oedg_faces<P>, g.n_face, d_recs, (const Blk *)g.d_face, st, 0));
+                    ENG_DISPATCH(g.sid, LAUNCH((k_oedg_faces<PX, PY>), g.n_face, d_recs, (const Blk *)g.d_face, st, 0));
                 }
                 {
                     Timed tm(this, KT_LIMIT);
                     if (g.eo) {
-                        ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, true>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                        ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
                     } else {
-                        ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, false>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                        ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, false>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
                     }
                 }
             }
@@ -620,7 +634,7 @@ int Engine::launch_step(bool all_kernels)
             for (auto &g : groups) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, st, 0);
         for (auto &g : groups) {
             Timed tm(this, KT_BC);
-            ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, st));
+            ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, st));
         }
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 0);
         for (auto &g : groups) {
@@ -693,29 +707,29 @@ int Engine::run_steps(long long nsteps, bool use_graph, long long *done_steps)
 // BC, saved ghost layer, residual of u^n (drive.f90:66-73)
 int Engine::launch_startup(const std::vector<int> &ids)
 {
-    std::vector<int> by_p[5];
-    for (int id : ids) by_p[hosts[id]->p].push_back(id);
+    std::vector<int> by_sid[9];
+    for (int id : ids) by_sid[hosts[id]->sid].push_back(id);
     int *d_ids = nullptr;
     ENG_TRY(hipMalloc((void **)&d_ids, std::max<size_t>(ids.size(), 1) * sizeof(int)));
     ENG_TRY(hipMemcpy(d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
     const int n = (int)ids.size();
     int rc = PPES_OK;
     std::vector<Group> tmp;
-    for (int p = 1; p <= 4 && rc == PPES_OK; ++p) {
-        if (by_p[p].empty()) continue;
+    for (int sid = 1; sid <= 8 && rc == PPES_OK; ++sid) {
+        if (by_sid[sid].empty()) continue;
         Group g;
-        rc = build_group(g, by_p[p], p);
+        rc = build_group(g, by_sid[sid], sid);
         tmp.push_back(g);
     }
     if (rc == PPES_OK) {
         LAUNCH1(k_reset_flags, n, d_recs, (const int *)d_ids, n, 1 | 2 | 4);
-        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp)
-            ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, 0,
+            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, 0,
                                      (int)(F_GHOST_MEM | F_FORCE_NO_OEDG | F_ENT)));
         for (auto &g : tmp) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, 0, (int)(F_FORCE_NO_OEDG | F_ENT));
-        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
-        for (auto &g : tmp) ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
         for (auto &g : tmp) RESIDUAL_DISPATCH(g, 0, 1, 0);
     }
@@ -744,7 +758,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     if ((int)api_groups.size() < (int)hosts.size()) api_groups.resize(hosts.size());
     Group &g = api_groups[id];
     if (!g.d_el) {
-        int rc = build_group(g, std::vector<int>{id}, hosts[id]->p);
+        int rc = build_group(g, std::vector<int>{id}, hosts[id]->sid);
         if (rc) { free_group(g); return rc; }
     }
     int rc = PPES_OK;
@@ -754,13 +768,13 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         LAUNCH1(k_reset_flags, 1, d_recs, (const int *)d_ids, 1, flags);
         break;
     case API_BC:
-        ENG_DISPATCH(g.p, LAUNCH(k_bc<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, slot));
+        ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, slot));
         break;
     case API_GHOST_SAVE:
-        ENG_DISPATCH(g.p, LAUNCH(k_ghost_copy<P>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
+        ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         break;
     case API_COMBINE:
-        ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
+        ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
         break;
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
@@ -769,18 +783,18 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     case API_POST: {
         const bool eo = h_recs[id].ctl.ent_offset != 0;
         if (eo) {
-            ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, true>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
+            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, true>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
         } else {
-            ENG_DISPATCH(g.p, LAUNCH((k_stage_pp<P, false>), stage_blocks(P, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
+            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
         }
         if (flags & F_FORCE_OEDG) {
             LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, slot, flags, 0);
             LAUNCH(k_reduce_global, 1, d_recs, (const Blk *)g.d_red2, slot, flags, 1);
-            ENG_DISPATCH(g.p, LAUNCH(k_oedg_faces<P>, g.n_face, d_recs, (const Blk *)g.d_face, slot, flags));
+            ENG_DISPATCH(g.sid, LAUNCH((k_oedg_faces<PX, PY>), g.n_face, d_recs, (const Blk *)g.d_face, slot, flags));
             if (eo) {
-                ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, true>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
+                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
             } else {
-                ENG_DISPATCH(g.p, LAUNCH((k_oedg_limit<P, false>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
+                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, false>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
             }
         }
         if (flags & F_ENT) LAUNCH(k_reduce_ent, 1, d_recs, (const int *)d_ids, slot, flags);

@@ -18,9 +18,10 @@ namespace ppes {
 
 struct HostInst {
     ppes_config cfg;
-    int p = 0, N = 0, NN = 0, S = 0;
+    int px = 0, py = 0, sid = 0;     // degrees and shape id (sid_px / sid_py)
+    int NX = 0, NY = 0, NN = 0, S = 0;
     double hx = 0, hy = 0;
-    std::vector<double> qx, qw, D, xx, yy;
+    std::vector<double> qx_x, qw_x, Dx, qx_y, qw_y, Dy, xx, yy;
     std::vector<double *> allocs;
     size_t bytes = 0;
     int ent_offset = 0;  // the solve's ent_offset switch (drive.f90:32,280)
@@ -28,7 +29,7 @@ struct HostInst {
 };
 
 struct Group {
-    int p = 0;
+    int sid = 0;  // shape id of every member
     std::vector<int> members;
     Blk *d_el = nullptr, *d_slot = nullptr, *d_face = nullptr, *d_bc = nullptr, *d_red1 = nullptr, *d_red2 = nullptr;
     Blk *d_res = nullptr;  // element map of the residual kernel (its own workgroup size), by face flux kind
@@ -90,7 +91,7 @@ public:
 private:
     int ensure_capacity(int n);
     int build_plan();
-    int build_group(Group &g, const std::vector<int> &members, int p);
+    int build_group(Group &g, const std::vector<int> &members, int sid);
     int capture();
     void free_group(Group &g);
     int *d_all = nullptr;

@@ -101,11 +101,12 @@ constexpr int resid_waves(int p, bool fast) { return !fast ? 1 : (p == 4 ? PPES_
 #define PPES_FAST_ROUND 1
 #endif
 
-template <int P, bool FAST, int TB = resid_tb(P), int FLUX = FLUX_ESPP_HLL>
-__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_waves(P, FAST)))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
+template <int PX, int PY, bool FAST, int TB = resid_tb((PX + 1) * (PY + 1)), int FLUX = FLUX_ESPP_HLL>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_waves(PX == PY ? PX : 0, FAST)))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
-    using S = Shape<P>;
-    constexpr int N = S::N, NN = S::NN, EPB = TB / NN, REC = S::REC;
+    using S = Shape<PX, PY>;
+    constexpr int NX = S::NX, NY = S::NY, NN = S::NN, EPB = TB / NN, REC = S::REC;
+    constexpr int XP = S::XP, HX = S::HX, HY = S::HY, ROUNDS = S::ROUNDS;
     __shared__ Prim sprim[EPB][NN];
     __shared__ double sbuf[EPB][(FAST && PPES_FAST_ROUND) ? S::BUF_ROUND : S::BUF];
     __shared__ double sred[TB][2];
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     gdbl *__restrict__ ut = gout(I.ut[ut_slot]);
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
-    const int k = n % N, l = n / N;
+    const int k = n % NX, l = n / NX;
     // 32-bit element and node indices (one variable plane holds < 2^31
     // doubles, which the host checks); 64-bit arithmetic only in the uniform
     // plane base addresses
@@ -155,13 +156,15 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         for (int r = 0; r < FR; ++r) {
             const int f = n + r * NN;
             if (valid && f < S::FACE) {
-                const int side = f / N, a = f - side * N;
+                // faces: left, right (NY nodes each), bottom, top (NX each)
+                const int side = f < 2 * NY ? f / NY : 2 + (f - 2 * NY) / NX;
+                const int a = f < 2 * NY ? f - side * NY : f - 2 * NY - (side - 2) * NX;
                 unsigned b1, b2;
                 if (side < 2) {
-                    b1 = node(side == 0 ? i - 1 : i, j) + (N - 1) + N * a;
-                    b2 = node(side == 0 ? i : i + 1, j) + N * a;
+                    b1 = node(side == 0 ? i - 1 : i, j) + (NX - 1) + NX * a;
+                    b2 = node(side == 0 ? i : i + 1, j) + NX * a;
                 } else {
-                    b1 = node(i, side == 2 ? j - 1 : j) + a + N * (N - 1);
+                    b1 = node(i, side == 2 ? j - 1 : j) + a + NX * (NY - 1);
                     b2 = node(i, side == 2 ? j : j + 1) + a;
                 }
 #pragma unroll
@@ -195,8 +198,8 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     __syncthreads();
 
     // ---- volume terms (DG_solver.f90:813-851): divB, the diagonal terms, then
-    // the EC pair fluxes, each pair evaluated once (P rounds, all lanes
-    // busy); the component that is identically zero (4 in x, 5 in y) is not
+    // the EC pair fluxes, each pair evaluated once (ROUNDS rounds of NN
+    // lanes); the component that is identically zero (4 in x, 5 in y) is not
     // stored
     double acc[8];
     const double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
@@ -205,9 +208,9 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         {
             double a = 0.0, b = 0.0;
 #pragma unroll
-            for (int r = 0; r < N; ++r) a += I.D[k + N * r] * sprim[el][r + N * l].bx;
+            for (int r = 0; r < NX; ++r) a += I.Dx[k + NX * r] * sprim[el][r + NX * l].bx;
 #pragma unroll
-            for (int s = 0; s < N; ++s) b += sprim[el][k + N * s].by * I.D[l + N * s];
+            for (int s = 0; s < NY; ++s) b += sprim[el][k + NX * s].by * I.Dy[l + NY * s];
             divB = (2.0 / hx) * a + (2.0 / hy) * b;
         }
         const Prim &pr = sprim[el][n];
@@ -226,26 +229,27 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
                              ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
         const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
-        const double dxx = cx * I.D[k + N * k], dyy = cy * I.D[l + N * l];
+        const double dxx = cx * I.Dx[k + NX * k], dyy = cy * I.Dy[l + NY * l];
 #pragma unroll
         for (int v = 0; v < 8; ++v) acc[v] = 0.0 - dxx * F[v] - dyy * G[v] - divB * GP[v];
     }
 
-    // pair qp of the element: x-pairs (kk, rr), rr < kk, of row lr first, then
-    // y-pairs (ll, ss), ss < ll, of column kc
+    // pair qp of the element: x-pairs (kk, rr), rr < kk, of row lr first
+    // (qp < XP), then y-pairs (ll, ss), ss < ll, of column kc
     auto pair_flux = [&](int qp, double *o) {
         double ec[8];
         if constexpr (FAST) {
             // one evaluation for x- and y-pairs alike: a y-pair is the x-flux
             // of the x/y-swapped primitives, its output swapped back
-            const bool ydir = qp >= NN * P / 2;
-            const int qq = ydir ? qp - NN * P / 2 : qp;
-            const int line = qq / S::H, a = qq - line * S::H;
+            const bool ydir = qp >= XP;
+            const int qq = ydir ? qp - XP : qp;
+            const int H = ydir ? HY : HX;
+            const int line = qq / H, a = qq - line * H;
             int hi = 1;
             while ((hi + 1) * hi / 2 <= a) ++hi;
             const int lo = a - hi * (hi - 1) / 2;
-            const int nA = ydir ? line + N * hi : hi + N * line;
-            const int nB2 = ydir ? line + N * lo : lo + N * line;
+            const int nA = ydir ? line + NX * hi : hi + NX * line;
+            const int nB2 = ydir ? line + NX * lo : lo + NX * line;
             Prim A = sprim[el][nA], B = sprim[el][nB2];
             if (ydir) {
                 double x;
@@ -265,22 +269,20 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             o[6] = ec[7];
             return;
         }
-        if (qp < NN * P / 2) {
-            const int lr = qp / S::H, a = qp - lr * S::H;
+        if (qp < XP) {
+            const int lr = qp / HX, a = qp - lr * HX;
             int kk = 1;
             while ((kk + 1) * kk / 2 <= a) ++kk;
             const int rr = a - kk * (kk - 1) / 2;
-            if constexpr (FAST) ec_flux_fast<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], inv_gm1, ec);
-            else ec_flux<0>(sprim[el][kk + N * lr], sprim[el][rr + N * lr], g, ec);
+            ec_flux<0>(sprim[el][kk + NX * lr], sprim[el][rr + NX * lr], g, ec);
             o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[5]; o[5] = ec[6]; o[6] = ec[7];
         } else {
-            const int qq = qp - NN * P / 2;
-            const int kc = qq / S::H, a = qq - kc * S::H;
+            const int qq = qp - XP;
+            const int kc = qq / HY, a = qq - kc * HY;
             int ll = 1;
             while ((ll + 1) * ll / 2 <= a) ++ll;
             const int ss = a - ll * (ll - 1) / 2;
-            if constexpr (FAST) ec_flux_fast<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], inv_gm1, ec);
-            else ec_flux<1>(sprim[el][kc + N * ll], sprim[el][kc + N * ss], g, ec);
+            ec_flux<1>(sprim[el][kc + NX * ll], sprim[el][kc + NX * ss], g, ec);
             o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[4]; o[5] = ec[6]; o[6] = ec[7];
         }
     };
@@ -298,38 +300,39 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         // accumulation order (DG_solver.f90:828-851)
         if (valid) {
 #pragma unroll
-            for (int c = 0; c < P; ++c) pair_flux(n + c * NN, &sbuf[el][(n + c * NN) * REC]);
+            for (int c = 0; c < ROUNDS; ++c)
+                if (n + c * NN < S::PAIRS) pair_flux(n + c * NN, &sbuf[el][(n + c * NN) * REC]);
         }
         __syncthreads();
         if (valid) {
-            for (int r = 0; r < k; ++r) sub_x(cx * I.D[k + N * r], &sbuf[el][(l * S::H + k * (k - 1) / 2 + r) * REC]);
+            for (int r = 0; r < k; ++r) sub_x(cx * I.Dx[k + NX * r], &sbuf[el][(l * HX + k * (k - 1) / 2 + r) * REC]);
             for (int s = 0; s < l; ++s)
-                sub_y(cy * I.D[l + N * s], &sbuf[el][(NN * P / 2 + k * S::H + l * (l - 1) / 2 + s) * REC]);
-            for (int kk = k + 1; kk < N; ++kk)
-                sub_x(cx * I.D[k + N * kk], &sbuf[el][(l * S::H + kk * (kk - 1) / 2 + k) * REC]);
-            for (int ll = l + 1; ll < N; ++ll)
-                sub_y(cy * I.D[l + N * ll], &sbuf[el][(NN * P / 2 + k * S::H + ll * (ll - 1) / 2 + l) * REC]);
+                sub_y(cy * I.Dy[l + NY * s], &sbuf[el][(XP + k * HY + l * (l - 1) / 2 + s) * REC]);
+            for (int kk = k + 1; kk < NX; ++kk)
+                sub_x(cx * I.Dx[k + NX * kk], &sbuf[el][(l * HX + kk * (kk - 1) / 2 + k) * REC]);
+            for (int ll = l + 1; ll < NY; ++ll)
+                sub_y(cy * I.Dy[l + NY * ll], &sbuf[el][(XP + k * HY + ll * (ll - 1) / 2 + l) * REC]);
         }
     } else {
         // one round's records at a time (half the LDS of the ordered form):
         // after round c every node adds the pairs of that round it belongs to
 #pragma unroll
-        for (int c = 0; c < P; ++c) {
-            if (valid) pair_flux(n + c * NN, &sbuf[el][n * REC]);
+        for (int c = 0; c < ROUNDS; ++c) {
+            if (valid && n + c * NN < S::PAIRS) pair_flux(n + c * NN, &sbuf[el][n * REC]);
             __syncthreads();
             if (valid) {
-                for (int r = 0; r < N; ++r) {
+                for (int r = 0; r < NX; ++r) {
                     if (r == k) continue;
-                    const int qp = l * S::H + (r > k ? r * (r - 1) / 2 + k : k * (k - 1) / 2 + r);
-                    if (qp / NN == c) sub_x(cx * I.D[k + N * r], &sbuf[el][(qp - c * NN) * REC]);
+                    const int qp = l * HX + (r > k ? r * (r - 1) / 2 + k : k * (k - 1) / 2 + r);
+                    if (qp / NN == c) sub_x(cx * I.Dx[k + NX * r], &sbuf[el][(qp - c * NN) * REC]);
                 }
-                for (int s2 = 0; s2 < N; ++s2) {
+                for (int s2 = 0; s2 < NY; ++s2) {
                     if (s2 == l) continue;
-                    const int qp = NN * P / 2 + k * S::H + (s2 > l ? s2 * (s2 - 1) / 2 + l : l * (l - 1) / 2 + s2);
-                    if (qp / NN == c) sub_y(cy * I.D[l + N * s2], &sbuf[el][(qp - c * NN) * REC]);
+                    const int qp = XP + k * HY + (s2 > l ? s2 * (s2 - 1) / 2 + l : l * (l - 1) / 2 + s2);
+                    if (qp / NN == c) sub_y(cy * I.Dy[l + NY * s2], &sbuf[el][(qp - c * NN) * REC]);
                 }
             }
-            if (c + 1 < P) __syncthreads();
+            if (c + 1 < ROUNDS) __syncthreads();
         }
     }
     __syncthreads();  // pair records consumed; the buffer now holds face data
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             // y-faces go through the x-flux with the x/y components swapped
             // (swap_xy); the swap is a selection per component, so lanes of
             // x- and y-faces share one evaluation of the flux
-            const bool yf = f / N >= 2;
+            const bool yf = f >= 2 * NY;
             double wl[8], wr[8], tf[8], nB, sl, sr;
             swap_xy_if(yf, tl[r], wl);
             swap_xy_if(yf, tr[r], wr);
@@ -376,7 +379,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 #pragma unroll
         for (int v = 0; v < 8; ++v) acc[v] = sacc[t][v];
     }
-    if (valid && (k == 0 || k == N - 1 || l == 0 || l == N - 1)) {
+    if (valid && (k == 0 || k == NX - 1 || l == 0 || l == NY - 1)) {
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = up[v][base];
         const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
@@ -390,31 +393,39 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double p_tot = p_fld + p_mag;
         const double b_dot_u = bx * vx + by * vy + bz * vz;
         const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
-        if (k == 0 || k == N - 1) {
+        // left then right, bottom then top; with one node in y (PY = 0) the
+        // bottom and top terms both land on the node, in that order
+        if (k == 0 || k == NX - 1) {
             const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
                                  0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
-            const double *o = &sbuf[el][(((k == 0) ? 0 : 1) * N + l) * S::FREC];
-            const double c = (k == 0) ? (2.0 / hx) / I.qw[0] : (2.0 / hx) / I.qw[N - 1];
             if (k == 0) {
+                const double *o = &sbuf[el][l * S::FREC];
+                const double c = (2.0 / hx) / I.qw_x[0];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (F[v] - o[v] + GP[v] * (bx - o[8]));
                 ax = dmax(ax, o[10]);  // sr at the left face
-            } else {
+            }
+            if (k == NX - 1) {
+                const double *o = &sbuf[el][(NY + l) * S::FREC];
+                const double c = (2.0 / hx) / I.qw_x[NX - 1];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (F[v] - o[v] + GP[v] * (bx - o[8]));
                 ax = dmax(ax, -o[9]);  // -sl at the right face
             }
         }
-        if (l == 0 || l == N - 1) {
+        if (l == 0 || l == NY - 1) {
             const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
                                  ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
-            const double *o = &sbuf[el][(((l == 0) ? 2 : 3) * N + k) * S::FREC];
-            const double c = (l == 0) ? (2.0 / hy) / I.qw[0] : (2.0 / hy) / I.qw[N - 1];
             if (l == 0) {
+                const double *o = &sbuf[el][(2 * NY + k) * S::FREC];
+                const double c = (2.0 / hy) / I.qw_y[0];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (G[v] - o[v] + GP[v] * (by - o[8]));
                 ay = dmax(ay, o[10]);
-            } else {
+            }
+            if (l == NY - 1) {
+                const double *o = &sbuf[el][(2 * NY + NX + k) * S::FREC];
+                const double c = (2.0 / hy) / I.qw_y[NY - 1];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (G[v] - o[v] + GP[v] * (by - o[8]));
                 ay = dmax(ay, -o[9]);
@@ -434,21 +445,20 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     if (valid && n == 0) {
         double mx_ = 0.0, my_ = 0.0;
         for (int m = 0; m < NN; ++m) { mx_ = dmax(mx_, sred[el * NN + m][0]); my_ = dmax(my_, sred[el * NN + m][1]); }
-        const double dtl = 1.0 / ((2 * P + 1) * mx_ / hx + (2 * P + 1) * my_ / hy);
+        const double dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
         atomicMin(&I.ctl.dtlim_bits, (unsigned long long)__double_as_longlong(dtl));
     }
 }
 
 // --------------------------------------------------------------------------
-// Minimum over the N^2 lanes of each element, returned to every lane of the
-// element. When N^2 divides the wavefront (P1, P3) the element's lanes are one
+// Minimum over the NN lanes of each element, returned to every lane of the
+// element. When NN divides the wavefront (P1, P3) the element's lanes are one
 // aligned group of a wavefront and a butterfly of lane exchanges does it with
 // no barrier; otherwise through LDS (called by every thread of the workgroup).
 // min is exact, so both give the value of the reference's loop.
-template <int P>
+template <int NN>
 __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, int el, bool valid)
 {
-    constexpr int NN = (P + 1) * (P + 1);
     if constexpr (64 % NN == 0) {
 #pragma unroll
         for (int m = 1; m < NN; m <<= 1) x = dmin(x, __shfl_xor(x, m));
@@ -467,12 +477,12 @@ __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, i
 // Zhang-Shu positivity scaling of one element toward its mean
 // (DG_solver.f90:528-603). Called by every thread of the workgroup (it may
 // synchronise); lanes with valid = false pass neutral values.
-template <int P>
+template <int NN>
 __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8], bool valid, double g,
                                                  double (*sred)[2], int t, int el, bool &neg_node, bool &fail)
 {
     // density, once
-    const double den_min = elem_min<P>(valid ? q[0] : DBL_HUGE, sred, t, el, valid);
+    const double den_min = elem_min<NN>(valid ? q[0] : DBL_HUGE, sred, t, el, valid);
     neg_node = false;
     fail = false;
     if (valid && den_min < PP_EPS) {
@@ -490,7 +500,7 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
     for (int round = 0; round < 4; ++round) {
         if (round > 0 && !__syncthreads_or(active ? 1 : 0)) break;
         const double prs_min =
-            elem_min<P>(valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE, sred, t, el, valid);
+            elem_min<NN>(valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE, sred, t, el, valid);
         if (!valid) continue;
         if (round == 0) {
             if (prs_min < PP_EPS) {
@@ -519,11 +529,11 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
 // Cell entropy of one element summed in the reference's node order, then the
 // workgroup's sum over its counted elements in element order; thread 0
 // receives the workgroup partial (DG_solver.f90:635-649).
-template <int P>
+template <int NN>
 __device__ __forceinline__ double entropy_partial(const double q[8], bool valid, bool counted, const Inst &I,
                                                   double (*sred)[2], double *sent, int *scnt, int t, int el, int n)
 {
-    constexpr int NN = (P + 1) * (P + 1), EPB = BLOCK / NN;
+    constexpr int EPB = BLOCK / NN;
     const double g = I.g;
     sred[t][0] = valid ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
                        : 0.0;
@@ -547,11 +557,11 @@ __device__ __forceinline__ double entropy_partial(const double q[8], bool valid,
 // before LDF, summed in the reference's node order, NaN unless every node has
 // density and pressure >= PP_eps; with the entropy of the cell mean. Lane
 // n == 0 of each valid element writes them to sce[el]. All lanes call.
-template <int P>
+template <int NN>
 __device__ __forceinline__ void pre_ldf_entropy(const double q[8], const double avg[8], bool valid, const Inst &I,
                                                 double (*sred)[2], double (*sce)[2], int t, int el, int n)
 {
-    constexpr int NN = (P + 1) * (P + 1), EPB = BLOCK / NN;
+    constexpr int EPB = BLOCK / NN;
     const double g = I.g;
     double c = 0.0, bad = 0.0;
     if (valid) {
@@ -579,12 +589,12 @@ __device__ __forceinline__ void pre_ldf_entropy(const double q[8], const double 
 // entropy exceeds its value before LDF (ent_old; NaN: skip), scale the cell
 // toward its mean by theta = (ent_old - ent_avg) / (ent_new - ent_avg)
 // clamped to [0, 1]. All lanes call; sth is per-element LDS.
-template <int P>
+template <int NN>
 __device__ __forceinline__ void entropy_offset(double q[8], const double avg[8], bool valid, double ent_old,
                                                double ent_avg, const Inst &I, double (*sred)[2], double *sth, int t,
                                                int el, int n)
 {
-    constexpr int NN = (P + 1) * (P + 1), EPB = BLOCK / NN;
+    constexpr int EPB = BLOCK / NN;
     const double g = I.g;
     const bool on = valid && !__builtin_isnan(ent_old);
     sred[t][0] = on ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
@@ -619,7 +629,7 @@ struct GhostSrc {
     int mode;  // 0 the value, 1 the fixed state, 2 reflected in x (m_x, B_x negated), 3 reflected in y
 };
 
-template <int N>
+template <int NX, int NY>
 __device__ inline GhostSrc ghost_src(const Inst &I, int ex, int ey, int k, int l)
 {
     int side, along;
@@ -628,7 +638,7 @@ __device__ inline GhostSrc ghost_src(const Inst &I, int ex, int ey, int k, int l
     else if (ey == 0) { side = 2; along = ex; }
     else { side = 3; along = ex; }
     const int rule = I.side[side];
-    GhostSrc g{ex, ey, k + N * l, 0};
+    GhostSrc g{ex, ey, k + NX * l, 0};
     if (rule == SIDE_FIXED || along <= I.nfix[side]) {
         g.mode = 1;
         return g;
@@ -637,9 +647,9 @@ __device__ inline GhostSrc ghost_src(const Inst &I, int ex, int ey, int k, int l
     if (xs) g.sx = (rule == SIDE_PERIODIC) == lo ? I.nx : 1;
     else g.sy = (rule == SIDE_PERIODIC) == lo ? I.ny : 1;
     if (rule == SIDE_TRACE) {
-        g.sn = xs ? (lo ? 0 : N - 1) + N * l : k + N * (lo ? 0 : N - 1);
+        g.sn = xs ? (lo ? 0 : NX - 1) + NX * l : k + NX * (lo ? 0 : NY - 1);
     } else if (rule == SIDE_MIRROR) {
-        g.sn = xs ? (N - 1 - k) + N * l : k + N * (N - 1 - l);
+        g.sn = xs ? (NX - 1 - k) + NX * l : k + NX * (NY - 1 - l);
         g.mode = xs ? 2 : 3;
     }
     return g;
@@ -663,11 +673,11 @@ __device__ __forceinline__ double ghost_value(int mode, int v, double x)
 // exactly what BC_handle would copy (problems.f90:312-348,386-423,597-654),
 // or read from memory (F_GHOST_MEM).
 
-template <int P, bool EO>
+template <int PX, int PY, bool EO>
 __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
 {
-    using S = Shape<P>;
-    constexpr int N = S::N, NN = S::NN, EPB = S::EPB;
+    using S = Shape<PX, PY>;
+    constexpr int NX = S::NX, NY = S::NY, NN = S::NN, EPB = S::EPB;
     constexpr int M = 2 * NN;
     // scons (node values: averages, OEDG extrema), sdb (LDF input) and sred
     // (limiter, entropy) are never live at the same time: one region, which
@@ -694,7 +704,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     if (!inst_live(I)) return;
     const int t = threadIdx.x;
     const int el = t / NN, n = t - el * NN;
-    const int k = n % N, l = n / N;
+    const int k = n % NX, l = n / NX;
     const long long nslots = (long long)(I.nx + 2) * (I.ny + 2);
     const long long eg = (long long)bm.local * EPB + el;
     const int ex = (int)(eg % (I.nx + 2)), ey = (int)(eg / (I.nx + 2));
@@ -724,7 +734,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
 #pragma unroll
             for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(src, v * vs + base, v * I.vsi + gt);
         } else {
-            const GhostSrc gsrc = ghost_src<N>(I, ex, ey, k, l);
+            const GhostSrc gsrc = ghost_src<NX, NY>(I, ex, ey, k, l);
             if (gsrc.mode == 1) {
 #pragma unroll
                 for (int v = 0; v < 8; ++v) q[v] = I.bc_state[v];
@@ -772,7 +782,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     // ---- ent_offset: the cell entropy before LDF (workgroup-uniform switch)
     const bool eo = EO && I.ctl.ent_offset != 0;
     if constexpr (EO)
-        if (eo) pre_ldf_entropy<P>(q, avg, valid && ok, I, sred, sce, t, el, n);
+        if (eo) pre_ldf_entropy<NN>(q, avg, valid && ok, I, sred, sce, t, el, n);
 
     // ---- LDF projection of (Bx, By) as one dense nodal operator (LDS-staged)
     if (use_ldf) {
@@ -850,15 +860,15 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     // ---- PP path: positivity limiter of interior elements, entropy, final state
     bool neg_node = false, fail = false;
     const bool lim = interior && ok;
-    pp_limit_element<P>(q, avg, lim, I.g, sred, t, el, neg_node, fail);
+    pp_limit_element<NN>(q, avg, lim, I.g, sred, t, el, neg_node, fail);
     if (lim && n == 0) {
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
     }
     if constexpr (EO)
-        if (eo) entropy_offset<P>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
+        if (eo) entropy_offset<NN>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
     if (stage_ent(I, st, flags)) {
-        const double s = entropy_partial<P>(q, lim, lim, I, sred, sent, scnt, t, el, n);
+        const double s = entropy_partial<NN>(q, lim, lim, I, sred, sent, scnt, t, el, n);
         if (t == 0) gout(I.entpart)[bm.local] = s;
     }
     if (valid && ok)
@@ -927,15 +937,81 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk 
 // variable); x-faces i = 1..nx+1 between (i-1,j) and (i,j), then y-faces.
 // For the o-th normal derivative the reference differentiates both whole
 // elements o times and takes the face traces; here the traces come straight
-// from rows of D^o (precomputed per degree): trace_o(t) = (2/h)^o
-// sum_a (D^o)(N-1 or 0, a) u(a, t), which is N^2 work per order instead of
+// from rows of D^o (precomputed per direction): trace_o(t) = (2/h)^o
+// sum_a (D^o)(last or 0, a) u(a, t), which is N^2 work per order instead of
 // N^3. The transverse derivatives of the jump and the weighted sums follow
-// the reference's loops.
-template <int P>
+// the reference's loops. PN / PT are the normal and transverse degrees of
+// the face (PX, PY on x-faces; PY, PX on y-faces).
+template <int PN, int PT, int MXP, bool XF, int NX>
+__device__ __forceinline__ double oedg_face_sum(const double *A, const double *B, const double (*srn)[MAXN][MAXN],
+                                                const double *Dt, double h, double ht, const double *qwt)
+{
+    constexpr int NNO = PN + 1, NT = PT + 1;
+    constexpr int den0 = 2 * (2 * PN - 1 > 1 ? 2 * PN - 1 : 1);
+    // jmp[o][t]: jump of the o-th normal derivative at transverse node t
+    double jmp[NNO][NT];
+#pragma unroll
+    for (int o = 0; o <= PN; ++o)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) jmp[o][t] = 0.0;
+    // node (a, t) of the face-normal / transverse index pair: a + NX t on
+    // x-faces, t + NX a on y-faces
+#pragma unroll
+    for (int a = 0; a < NNO; ++a) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int m = XF ? a + NX * t : t + NX * a;
+#pragma unroll
+            for (int o = 0; o <= PN; ++o) jmp[o][t] += srn[1][o][a] * B[m] - srn[0][o][a] * A[m];
+        }
+    }
+    double D2t[NT * NT];
+#pragma unroll
+    for (int m = 0; m < NT * NT; ++m) D2t[m] = (2.0 / ht) * Dt[m];
+    double total = 0.0, scale = 1.0, h_pow_o = 1.0;
+    int fac_o = 1;
+#pragma unroll
+    for (int o = 0; o <= PN; ++o) {
+        double jump[NT], jt[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) jump[t] = scale * jmp[o][t];
+        int fac_q = 1;
+        double h_pow = h_pow_o;
+#pragma unroll
+        for (int q = 0; q <= MXP - o; ++q) {
+            const int mm = o + q;
+            double jl1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) jl1 += (0.5 * qwt[t]) * fabs(jump[t]);
+            const double coef = ((double)(2 * mm + 1) * h_pow) / (double)(den0 * fac_o * fac_q);
+            total = total + coef * jl1;
+            if (q < PT) {
+#pragma unroll
+                for (int a = 0; a < NT; ++a) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) acc += D2t[a + NT * b] * jump[b];
+                    jt[a] = acc;
+                }
+#pragma unroll
+                for (int a = 0; a < NT; ++a) jump[a] = jt[a];
+                fac_q = fac_q * (q + 1);
+                h_pow = h_pow * h;
+            }
+        }
+        if (o < PN) {
+            fac_o = fac_o * (o + 1);
+            h_pow_o = h_pow_o * h;
+            scale = scale * (2.0 / h);
+        }
+    }
+    return total;
+}
+
+template <int PX, int PY>
 __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
-    constexpr int N = P + 1, NN = N * N, mxp = P;
-    constexpr int den0 = 2 * (2 * P - 1 > 1 ? 2 * P - 1 : 1);
+    constexpr int NX = PX + 1, NY = PY + 1, NN = NX * NY, MXP = PX > PY ? PX : PY;
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
@@ -948,25 +1024,21 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
     const int v = (int)(tt % 8);
     const long long f = tt / 8;
     long long bA, bB;
-    double h, ht;  // normal and transverse spacing
     if (xface) {
         const int i = (int)(f % (I.nx + 1)) + 1, j = (int)(f / (I.nx + 1)) + 1;
         bA = gidx(I, i - 1, j) * NN;
         bB = gidx(I, i, j) * NN;
-        h = I.hx;
-        ht = I.hy;
     } else {
         const int i = (int)(f % I.nx) + 1, j = (int)(f / I.nx) + 1;
         bA = gidx(I, i, j - 1) * NN;
         bB = gidx(I, i, j) * NN;
-        h = I.hy;
-        ht = I.hx;
     }
-    // the rows of D^o, shared by the workgroup (one instance per workgroup)
-    __shared__ double srow[2][P + 1][N];
-    if (threadIdx.x < 2 * (P + 1) * N) {
-        const int x = threadIdx.x;
-        srow[x / ((P + 1) * N)][(x / N) % (P + 1)][x % N] = I.drow[x / ((P + 1) * N)][(x / N) % (P + 1)][x % N];
+    // the rows of Dx^o and Dy^o, shared by the workgroup (one instance per workgroup)
+    __shared__ double srx[2][MAXN][MAXN], sry[2][MAXN][MAXN];
+    if (threadIdx.x < 2 * MAXN * MAXN) {
+        const int x = threadIdx.x, e = x / (MAXN * MAXN), o = (x / MAXN) % MAXN, a = x % MAXN;
+        srx[e][o][a] = I.drow_x[e][o][a];
+        sry[e][o][a] = I.drow_y[e][o][a];
     }
     __syncthreads();
     if (!live) return;
@@ -975,77 +1047,19 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
     double A[NN], B[NN];
 #pragma unroll
     for (int m = 0; m < NN; ++m) { A[m] = uA[m]; B[m] = uB[m]; }
-    // jmp[o][t]: jump of the o-th normal derivative at transverse node t
-    double jmp[P + 1][N];
-#pragma unroll
-    for (int o = 0; o <= P; ++o)
-#pragma unroll
-        for (int t = 0; t < N; ++t) jmp[o][t] = 0.0;
-    // node (a, t) of the face-normal / transverse index pair: a + N t on
-    // x-faces, t + N a on y-faces (two code paths, no per-value selects)
-    auto traces = [&](auto XF) {
-#pragma unroll
-        for (int a = 0; a < N; ++a) {
-#pragma unroll
-            for (int t = 0; t < N; ++t) {
-                const int m = decltype(XF)::value ? a + N * t : t + N * a;
-#pragma unroll
-                for (int o = 0; o <= P; ++o) jmp[o][t] += srow[1][o][a] * B[m] - srow[0][o][a] * A[m];
-            }
-        }
-    };
-    if (xface) traces(std::true_type{});
-    else traces(std::false_type{});
-    double D2t[NN];
-#pragma unroll
-    for (int m = 0; m < NN; ++m) D2t[m] = (2.0 / ht) * I.D[m];
-    double total = 0.0, scale = 1.0, h_pow_o = 1.0;
-    int fac_o = 1;
-#pragma unroll
-    for (int o = 0; o <= P; ++o) {
-        double jump[N], jt[N];
-#pragma unroll
-        for (int t = 0; t < N; ++t) jump[t] = scale * jmp[o][t];
-        int fac_q = 1;
-        double h_pow = h_pow_o;
-#pragma unroll
-        for (int q = 0; q <= mxp - o; ++q) {
-            const int mm = o + q;
-            double jl1 = 0.0;
-#pragma unroll
-            for (int t = 0; t < N; ++t) jl1 += (0.5 * I.qw[t]) * fabs(jump[t]);
-            const double coef = ((double)(2 * mm + 1) * h_pow) / (double)(den0 * fac_o * fac_q);
-            total = total + coef * jl1;
-            if (q < P) {
-#pragma unroll
-                for (int a = 0; a < N; ++a) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int b = 0; b < N; ++b) acc += D2t[a + N * b] * jump[b];
-                    jt[a] = acc;
-                }
-#pragma unroll
-                for (int a = 0; a < N; ++a) jump[a] = jt[a];
-                fac_q = fac_q * (q + 1);
-                h_pow = h_pow * h;
-            }
-        }
-        if (o < P) {
-            fac_o = fac_o * (o + 1);
-            h_pow_o = h_pow_o * h;
-            scale = scale * (2.0 / h);
-        }
-    }
+    double total;
+    if (xface) total = oedg_face_sum<PX, PY, MXP, true, NX>(A, B, srx, I.Dy, I.hx, I.hy, I.qw_y);
+    else total = oedg_face_sum<PY, PX, MXP, false, NX>(A, B, sry, I.Dx, I.hy, I.hx, I.qw_x);
     if (xface) gout(I.sigx)[f * 8 + v] = total;
     else gout(I.sigy)[f * 8 + v] = total;
 }
 
 // OEDG path, limiter loop (DG_solver.f90:428-656): OEDG damping toward the
 // cell mean, the positivity limiter, the cell entropy, the final state.
-template <int P, bool EO>
+template <int PX, int PY, bool EO>
 __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *map, int st, int flags)
 {
-    using S = Shape<P>;
+    using S = Shape<PX, PY>;
     constexpr int NN = S::NN, EPB = S::EPB;
     __shared__ double sred[BLOCK][2];
     __shared__ double sent[EPB];
@@ -1104,7 +1118,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
         __syncthreads();
     }
     bool neg_node = false, fail = false;
-    pp_limit_element<P>(q, avg, valid, g, sred, t, el, neg_node, fail);
+    pp_limit_element<NN>(q, avg, valid, g, sred, t, el, neg_node, fail);
     if (valid && n == 0) {
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
@@ -1112,10 +1126,10 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     if constexpr (EO)
         if (I.ctl.ent_offset) {
             const gdbl *ce = gin(I.cent) + ge * 2;
-            entropy_offset<P>(q, avg, valid, valid ? ce[0] : 0.0, valid ? ce[1] : 0.0, I, sred, sth, t, el, n);
+            entropy_offset<NN>(q, avg, valid, valid ? ce[0] : 0.0, valid ? ce[1] : 0.0, I, sred, sth, t, el, n);
         }
     if (stage_ent(I, st, flags)) {
-        const double s = entropy_partial<P>(q, valid, valid, I, sred, sent, scnt, t, el, n);
+        const double s = entropy_partial<NN>(q, valid, valid, I, sred, sent, scnt, t, el, n);
         if (t == 0) gout(I.entpart)[bm.local] = s;
     }
     if (valid) {
@@ -1149,10 +1163,10 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_ent(Inst *insts, const int *li
 // Ghost strips of state slot `slot` by the instance's side rules (the
 // BC_handle fillers of problems.f90:312-654); one thread per (ghost element,
 // node)
-template <int P>
+template <int PX, int PY>
 __global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int slot)
 {
-    constexpr int N = P + 1, NN = N * N;
+    constexpr int NX = PX + 1, NY = PY + 1, NN = NX * NY;
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || I.bc_kind == BC_HOST) return;
@@ -1162,13 +1176,13 @@ __global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int s
     gdbl *__restrict__ u = gout(I.u[slot]);
     const int n = (int)(tid % NN);
     const long long s = tid / NN;
-    const int k = n % N, l = n / N;
+    const int k = n % NX, l = n / NX;
     int gx, gy;
     if (s < I.ny) { gx = 0; gy = (int)s + 1; }
     else if (s < 2LL * I.ny) { gx = I.nx + 1; gy = (int)(s - I.ny) + 1; }
     else if (s < 2LL * I.ny + I.nx) { gx = (int)(s - 2LL * I.ny) + 1; gy = 0; }
     else { gx = (int)(s - 2LL * I.ny - I.nx) + 1; gy = I.ny + 1; }
-    const GhostSrc g = ghost_src<N>(I, gx, gy, k, l);
+    const GhostSrc g = ghost_src<NX, NY>(I, gx, gy, k, l);
     const long long dst = gidx(I, gx, gy) * NN + n;
     if (g.mode == 1) {
 #pragma unroll
@@ -1185,10 +1199,9 @@ __global__ void __launch_bounds__(BLOCK) k_bc(Inst *insts, const Blk *map, int s
 // solution's ghost layer of the reference is never refreshed after the
 // start-up (drive.f90:304-311 copies the interior only), and a retried
 // residual (drive.f90:104) reads it.
-template <int P>
+template <int NN>
 __global__ void __launch_bounds__(BLOCK) k_ghost_copy(Inst *insts, const Blk *map, int dir)
 {
-    constexpr int NN = (P + 1) * (P + 1);
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I)) return;

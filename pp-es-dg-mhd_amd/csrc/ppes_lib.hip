@@ -26,13 +26,21 @@ using namespace ppes;
         }                                                                                                   \
     } while (0)
 
-#define DISPATCH_P(p, CALL)                       \
-    switch (p) {                                  \
-    case 1: { constexpr int P = 1; CALL; } break; \
-    case 2: { constexpr int P = 2; CALL; } break; \
-    case 3: { constexpr int P = 3; CALL; } break; \
-    case 4: { constexpr int P = 4; CALL; } break; \
-    default: return PPES_UNSUPPORTED;             \
+// node counts NXN, NYN of a shape id (see ppes_types.hpp)
+#define LIB_CASE_(SID) \
+    case SID: {        \
+        constexpr int NXN = sid_px(SID) + 1, NYN = sid_py(SID) + 1;
+#define DISPATCH_SHAPE(sid, CALL)  \
+    switch (sid) {                 \
+        LIB_CASE_(1) CALL; } break; \
+        LIB_CASE_(2) CALL; } break; \
+        LIB_CASE_(3) CALL; } break; \
+        LIB_CASE_(4) CALL; } break; \
+        LIB_CASE_(5) CALL; } break; \
+        LIB_CASE_(6) CALL; } break; \
+        LIB_CASE_(7) CALL; } break; \
+        LIB_CASE_(8) CALL; } break; \
+    default: return PPES_UNSUPPORTED; \
     }
 
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -53,22 +61,22 @@ namespace ppes {
 // only combines them with correctly rounded operations.
 // A non-separable function of x and y (the Alfven wave's phase) comes as a
 // per-node table `tn`.
-template <int P>
+template <int NXN, int NYN>
 __global__ void __launch_bounds__(BLOCK) k_init(Inst *insts, int id, int prob, const double *__restrict__ tx,
                                                 const double *__restrict__ ty, const double *__restrict__ tn,
                                                 const double *__restrict__ modes, int nk, double amp, double c0,
                                                 double c1, double c2)
 {
-    constexpr int N = P + 1, NN = N * N;
+    constexpr int NN = NXN * NYN;
     const Inst &I = insts[id];
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (long long)I.nx * I.ny * NN) return;
     const int n = (int)(tid % NN);
     const long long e = tid / NN;
     const int i = (int)(e % I.nx) + 1, j = (int)(e / I.nx) + 1;
-    const int k = n % N, l = n / N;
-    const long long NX = (long long)N * I.nx, NY = (long long)N * I.ny;
-    const long long xi = k + (long long)N * (i - 1), yi = l + (long long)N * (j - 1);
+    const int k = n % NXN, l = n / NXN;
+    const long long NX = (long long)NXN * I.nx, NY = (long long)NYN * I.ny;
+    const long long xi = k + (long long)NXN * (i - 1), yi = l + (long long)NYN * (j - 1);
     const double x = tx[xi], y = ty[yi], g = I.g;
     double up[8];
     switch (prob) {
@@ -280,12 +288,12 @@ static int inst_init_problem(Engine &e, int id, int prob, const double *pert, in
         const double alpha = pi / 6.0;
         c0 = sin(alpha);
         c1 = cos(alpha);
-        const int N = h.N, NN = h.NN, nx = h.cfg.nx, ny = h.cfg.ny;
+        const int NX = h.NX, NY = h.NY, NN = h.NN, nx = h.cfg.nx, ny = h.cfg.ny;
         tn.assign(2 * (size_t)nx * ny * NN, 0.0);
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i)
                 for (int n = 0; n < NN; ++n) {
-                    const double x = h.xx[n % N + (size_t)N * (i - 1)], y = h.yy[n / N + (size_t)N * (j - 1)];
+                    const double x = h.xx[n % NX + (size_t)NX * (i - 1)], y = h.yy[n / NX + (size_t)NY * (j - 1)];
                     const double x_para = x * cos(alpha) + y * sin(alpha);
                     const size_t t = ((size_t)(j - 1) * nx + (i - 1)) * NN + n;
                     // one sincos call, as the reference's compiled code makes for the
@@ -318,7 +326,7 @@ static int inst_init_problem(Engine &e, int id, int prob, const double *pert, in
     HIP_TRY(hipMemcpyAsync(d_tn, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     HIP_TRY(hipMemcpyAsync(d_modes, modes.data(), modes.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     const long long nthreads = (long long)h.cfg.nx * h.cfg.ny * h.NN;
-    DISPATCH_P(h.p, hipLaunchKernelGGL(k_init<P>, dim3((unsigned)cdiv(nthreads, BLOCK)), dim3(BLOCK), 0, e.stream,
+    DISPATCH_SHAPE(h.sid, hipLaunchKernelGGL((k_init<NXN, NYN>), dim3((unsigned)cdiv(nthreads, BLOCK)), dim3(BLOCK), 0, e.stream,
                                        e.d_recs, id, prob, (const double *)d_tx, (const double *)d_ty,
                                        (const double *)d_tn, (const double *)d_modes, nk, amp, c0, c1, c2));
     HIP_TRY(hipGetLastError());
@@ -446,7 +454,7 @@ extern "C" int ppes_tables(int p, double *x, double *w, double *D, double *Pmn, 
 extern "C" int ppes_ldf_nodal_operator(int p, double hx, double hy, double *op)
 {
     if (!op || !(hx > 0) || !(hy > 0)) return PPES_BAD_ARG;
-    return host_ldf_nodal(p, hx, hy, op) ? PPES_BAD_ARG : PPES_OK;
+    return host_ldf_nodal(p, p, hx, hy, op) ? PPES_BAD_ARG : PPES_OK;
 }
 
 extern "C" int ppes_ldf_modal_operator(int p, double *op)
@@ -472,8 +480,8 @@ extern "C" const char *ppes_status_string(int st)
 
 extern "C" long long ppes_instance_bytes(const ppes_config *cfg)
 {
-    if (!cfg || cfg->px < 1 || cfg->px > 4 || cfg->nx < 1 || cfg->ny < 1) return -1;
-    const long long N = cfg->px + 1, NN = N * N, nx = cfg->nx, ny = cfg->ny, S = cfg->rk_stages;
+    if (!cfg || shape_id(cfg->px, cfg->py) < 0 || cfg->nx < 1 || cfg->ny < 1) return -1;
+    const long long NN = (long long)(cfg->px + 1) * (cfg->py + 1), nx = cfg->nx, ny = cfg->ny, S = cfg->rk_stages;
     const long long E = BLOCK / NN;
     const long long vs = (nx + 2) * (ny + 2) * NN, vsi = nx * ny * NN;
     const long long nblk_el = cdiv(nx * ny, E), nblk_slot = cdiv((nx + 2) * (ny + 2), E);
@@ -603,7 +611,7 @@ static int error_functional(ppes_solver *s, int slot, int which, double T, doubl
 {
     if (!s || !err_l2 || !err_max || slot < 0 || slot > s->eng.S) return PPES_BAD_ARG;
     const HostInst &h = s->eng.host(s->id);
-    const int N = h.N, NN = h.NN, nx = h.cfg.nx, ny = h.cfg.ny;
+    const int NX = h.NX, NY = h.NY, NN = h.NN, nx = h.cfg.nx, ny = h.cfg.ny;
     std::vector<double> u((size_t)8 * NN * (nx + 2) * (ny + 2));
     int rc = inst_state_copy(s->eng, s->id, slot, u.data(), false, false);
     if (rc) return rc;
@@ -611,11 +619,11 @@ static int error_functional(ppes_solver *s, int slot, int which, double T, doubl
     double l2 = 0.0, mx = 0.0;
     for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
-                    const double *q = &u[8 * (k + (size_t)N * (l + (size_t)N * (i + (size_t)(nx + 2) * j)))];
-                    const double x = h.xx[k + (size_t)N * (i - 1)], y = h.yy[l + (size_t)N * (j - 1)];
-                    const double w = (((0.25 * h.hx) * h.hy) * h.qw[k]) * h.qw[l];
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
+                    const double *q = &u[8 * (k + (size_t)NX * (l + (size_t)NY * (i + (size_t)(nx + 2) * j)))];
+                    const double x = h.xx[k + (size_t)NX * (i - 1)], y = h.yy[l + (size_t)NY * (j - 1)];
+                    const double w = (((0.25 * h.hx) * h.hy) * h.qw_x[k]) * h.qw_y[l];
                     if (which == 0) {  // compute_error_smooth_alfven (problems.f90:659-697)
                         const double x_para = x * cos(alpha) + y * sin(alpha);
                         const double b_exact = 0.1 * sin(2 * pi * x_para);

@@ -88,12 +88,22 @@ static void tables_ld(int p, ld *x, ld *w, ld *D, ld *Pmn, ld *Pnm)
     }
 }
 
+// degree 0 (one node, gauss.f90:35-37): x = 0, w = 2, D = 0, P_0 = 1
+static void tables_any(int p, ld *x, ld *w, ld *D, ld *Pmn, ld *Pnm)
+{
+    if (p == 0) {
+        x[0] = 0.0L; w[0] = 2.0L; D[0] = 0.0L; Pmn[0] = 1.0L; Pnm[0] = 1.0L;
+        return;
+    }
+    tables_ld(p, x, w, D, Pmn, Pnm);
+}
+
 int host_tables(int p, double *x, double *w, double *D, double *Pmn, double *Pnm)
 {
-    if (p < 1 || p > 6) return 1;
+    if (p < 0 || p > 6) return 1;
     const int N = p + 1;
     ld lx[8], lw[8], lD[64], lPmn[64], lPnm[64];
-    tables_ld(p, lx, lw, lD, lPmn, lPnm);
+    tables_any(p, lx, lw, lD, lPmn, lPnm);
     for (int i = 0; i < N; ++i) {
         if (x) x[i] = (double)lx[i];
         if (w) w[i] = (double)lw[i];
@@ -139,28 +149,38 @@ static bool invert(int n, std::vector<ld> &A, std::vector<ld> &Ainv)
     return true;
 }
 
-// Modal LDF projector, 2N^2 x 2N^2 column-major in long double; modal index
-// a + N b for the coefficient of P_a(x) P_b(y), Bx block then By block.
-static bool ldf_modal_ld(int p, std::vector<ld> &Pm)
+// Modal LDF projector of degrees (px, py), 2NN x 2NN column-major in long
+// double (NN = NX NY); modal index a + NX b for the coefficient of
+// P_a(x) P_b(y), Bx block then By block.
+static void modal_dm(int p, std::vector<ld> &m, std::vector<ld> &Dm)
 {
-    const int N = p + 1, NN = N * N, M = 2 * NN;
-    std::vector<ld> m(N), Dm((size_t)N * N, 0.0L);
+    const int N = p + 1;
+    m.assign(N, 0.0L);
+    Dm.assign((size_t)N * N, 0.0L);
     for (int j = 0; j < N; ++j) {
         m[j] = 1.0L / (j + 0.5L);
         for (int i = 0; i < j; ++i) Dm[i + (size_t)N * j] = ((j - i) & 1) ? 2.0L * i + 1.0L : 0.0L;
     }
-    // divergence B (NN x M): div(a,b) = sum_c Dm(a,c) cx(c,b) + sum_d cy(a,d) Dm(b,d)
+}
+
+static bool ldf_modal_ld(int px, int py, std::vector<ld> &Pm)
+{
+    const int NX = px + 1, NY = py + 1, NN = NX * NY, M = 2 * NN;
+    std::vector<ld> mx, Dx, my, Dy;
+    modal_dm(px, mx, Dx);
+    modal_dm(py, my, Dy);
+    // divergence B (NN x M): div(a,b) = sum_c Dx(a,c) cx(c,b) + sum_d cy(a,d) Dy(b,d)
     std::vector<ld> B((size_t)NN * M, 0.0L), mm(M);
-    for (int b = 0; b < N; ++b)
-        for (int a = 0; a < N; ++a) {
-            const int r = a + N * b;
-            for (int c = 0; c < N; ++c) B[r + (size_t)NN * (c + N * b)] += Dm[a + (size_t)N * c];
-            for (int d = 0; d < N; ++d) B[r + (size_t)NN * (NN + a + N * d)] += Dm[b + (size_t)N * d];
-            mm[r] = m[a] * m[b];
-            mm[NN + r] = m[a] * m[b];
+    for (int b = 0; b < NY; ++b)
+        for (int a = 0; a < NX; ++a) {
+            const int r = a + NX * b;
+            for (int c = 0; c < NX; ++c) B[r + (size_t)NN * (c + NX * b)] += Dx[a + (size_t)NX * c];
+            for (int d = 0; d < NY; ++d) B[r + (size_t)NN * (NN + a + NX * d)] += Dy[b + (size_t)NY * d];
+            mm[r] = mx[a] * my[b];
+            mm[NN + r] = mx[a] * my[b];
         }
-    // C = B MM^-1 B^T on the range of B: the (p,p) divergence coefficient is
-    // identically zero (row NN-1 of B vanishes), so drop it
+    // C = B MM^-1 B^T on the range of B: the (px, py) divergence coefficient
+    // is identically zero (row NN-1 of B vanishes), so drop it
     const int R = NN - 1;
     std::vector<ld> C((size_t)R * R, 0.0L), Ci;
     for (int r1 = 0; r1 < R; ++r1)
@@ -192,27 +212,28 @@ int host_ldf_modal(int p, double *op)
 {
     if (p < 1 || p > 6) return 1;
     std::vector<ld> Pm;
-    if (!ldf_modal_ld(p, Pm)) return 1;
+    if (!ldf_modal_ld(p, p, Pm)) return 1;
     for (size_t i = 0; i < Pm.size(); ++i) op[i] = (double)Pm[i];
     return 0;
 }
 
-int host_ldf_nodal(int p, double hx, double hy, double *op)
+int host_ldf_nodal(int px, int py, double hx, double hy, double *op)
 {
-    if (p < 1 || p > 6) return 1;
-    const int N = p + 1, NN = N * N, M = 2 * NN;
+    if (px < 0 || px > 6 || py < 0 || py > 6 || px + py < 1) return 1;
+    const int NX = px + 1, NY = py + 1, NN = NX * NY, M = 2 * NN;
     std::vector<ld> Pm;
-    if (!ldf_modal_ld(p, Pm)) return 1;
-    ld x[8], w[8], D[64], Pmn[64], Pnm[64];
-    tables_ld(p, x, w, D, Pmn, Pnm);
+    if (!ldf_modal_ld(px, py, Pm)) return 1;
+    ld x[8], w[8], Dx[64], Pmnx[64], Pnmx[64], y[8], wy[8], Dy[64], Pmny[64], Pnmy[64];
+    tables_any(px, x, w, Dx, Pmnx, Pnmx);
+    tables_any(py, y, wy, Dy, Pmny, Pnmy);
     // W2: nodal -> modal, V2: modal -> nodal (tensor products)
     std::vector<ld> W2((size_t)NN * NN), V2((size_t)NN * NN);
-    for (int b = 0; b < N; ++b)
-        for (int a = 0; a < N; ++a)
-            for (int l = 0; l < N; ++l)
-                for (int k = 0; k < N; ++k) {
-                    W2[(a + N * b) + (size_t)NN * (k + N * l)] = Pnm[a + N * k] * Pnm[b + N * l];
-                    V2[(k + N * l) + (size_t)NN * (a + N * b)] = Pmn[k + N * a] * Pmn[l + N * b];
+    for (int b = 0; b < NY; ++b)
+        for (int a = 0; a < NX; ++a)
+            for (int l = 0; l < NY; ++l)
+                for (int k = 0; k < NX; ++k) {
+                    W2[(a + NX * b) + (size_t)NN * (k + NX * l)] = Pnmx[a + NX * k] * Pnmy[b + NY * l];
+                    V2[(k + NX * l) + (size_t)NN * (a + NX * b)] = Pmnx[k + NX * a] * Pmny[l + NY * b];
                 }
     // Piola scalings: modal x-part scaled by hy/2, y-part by hx/2 on the way
     // in, and by 2/hy, 2/hx on the way out (DG_solver.f90:241-242,261-262)

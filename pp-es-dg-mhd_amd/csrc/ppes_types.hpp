@@ -68,14 +68,16 @@ struct Ctl {
 };
 
 struct Inst {
-    int nx, ny, p, S;
+    int nx, ny, px, py, S, pad_;
     long long vs, vsi;               // variable strides of state / residual arrays
     double hx, hy, g;
-    double qw[MAXN];
-    double D[MAXN * MAXN];           // D[k + N r] = l_r'(x_k)
-    double wavg[MAXN * MAXN];        // (0.25 w_k) w_l
+    double qw_x[MAXN], qw_y[MAXN];   // GLL weights in x (NX) and y (NY)
+    double Dx[MAXN * MAXN];          // Dx[k + NX r] = l_r'(x_k)
+    double Dy[MAXN * MAXN];          // Dy[l + NY s]
+    double wavg[MAXN * MAXN];        // (0.25 w_k) w_l, node k + NX l
     double went[MAXN * MAXN];        // (((0.25 hx) hy) w_k) w_l
-    double drow[2][MAXN][MAXN];      // drow[0][o][a] = (D^o)(N-1, a), drow[1][o][a] = (D^o)(0, a)
+    double drow_x[2][MAXN][MAXN];    // drow_x[0][o][a] = (Dx^o)(NX-1, a), drow_x[1][o][a] = (Dx^o)(0, a)
+    double drow_y[2][MAXN][MAXN];    // the same for Dy
     int bc_kind, pad0;
     int side[4], nfix[4];            // SideRule per side; cells 1..nfix[side] take bc_state
     double bc_state[8];
@@ -102,28 +104,42 @@ struct Blk {
     int inst, local;
 };
 
-// Workgroup size of the residual kernel: 256 lanes by default; smaller
-// workgroups leave fewer waves waiting at each barrier (P4 keeps at least 128
-// so that five 25-lane elements fill it).
+// Workgroup size of the residual kernel by the element's node count nn:
+// smaller workgroups leave fewer waves waiting at each barrier (25-node P4
+// elements keep at least 128 lanes so that five of them fill it).
 #ifndef PPES_RESID_TB
 #define PPES_RESID_TB 64
 #endif
-constexpr int resid_tb(int p) { return (p == 4 && PPES_RESID_TB < 128) ? 128 : PPES_RESID_TB; }
+constexpr int resid_tb(int nn) { return (nn > 16 && PPES_RESID_TB < 128) ? 128 : PPES_RESID_TB; }
 
 
-template <int P>
+// Element shape of degrees PX in x and PY in y (PY = PX, or PY = 0: the
+// reference's 1D set-up with one node in y). Node n = k + NX l.
+template <int PX, int PY>
 struct Shape {
-    static constexpr int N = P + 1;
-    static constexpr int NN = N * N;
+    static constexpr int NX = PX + 1, NY = PY + 1;
+    static constexpr int NN = NX * NY;
     static constexpr int EPB = BLOCK / NN;                 // elements per workgroup
-    static constexpr int H = N * (N - 1) / 2;              // pairs per line
-    static constexpr int PAIRS = NN * P;                   // x- plus y-pairs per element
-    static constexpr int FACE = 4 * N;                     // face nodes per element
+    static constexpr int HX = NX * (NX - 1) / 2;           // pairs per x-line
+    static constexpr int HY = NY * (NY - 1) / 2;           // pairs per y-line
+    static constexpr int XP = NY * HX;                     // x-pairs per element
+    static constexpr int PAIRS = XP + NX * HY;             // x- plus y-pairs per element
+    static constexpr int ROUNDS = (PAIRS + NN - 1) / NN;   // pair rounds of NN lanes
+    static constexpr int FACE = 2 * NY + 2 * NX;           // face nodes per element: left, right, bottom, top
     static constexpr int REC = 7;                          // doubles per stored pair flux
     static constexpr int FREC = 11;                        // doubles per face node: flux, B_n*, sl, sr
     static constexpr int BUF = (PAIRS * REC > FACE * FREC) ? PAIRS * REC : FACE * FREC;
     static constexpr int BUF_ROUND = (NN * REC > FACE * FREC) ? NN * REC : FACE * FREC;  // one pair round
 };
+
+// Shape ids of the device code: 1..4 = (p, p); 5..8 = (p, 0).
+constexpr int sid_px(int sid) { return sid <= 4 ? sid : sid - 4; }
+constexpr int sid_py(int sid) { return sid <= 4 ? sid : 0; }
+inline int shape_id(int px, int py)
+{
+    if (px < 1 || px > 4) return -1;
+    return py == px ? px : (py == 0 ? 4 + px : -1);
+}
 
 __device__ __forceinline__ long long gidx(const Inst &I, int ex, int ey) { return (long long)ey * (I.nx + 2) + ex; }
 

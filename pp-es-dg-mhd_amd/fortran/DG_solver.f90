@@ -48,7 +48,7 @@ contains
 		real :: nan_
 		integer(c_int) :: rc
 		nan_ = ieee_value(1.0, ieee_signaling_nan)
-		if (px_in /= py_in) error stop 'init_DG_solver: px /= py is not supported on the device'
+		if (px_in /= py_in .and. py_in /= 0) error stop 'init_DG_solver: the device takes py = px or py = 0'
 		nx = nx_in; ny = ny_in; px = px_in; py = py_in; rknstg = rknstg_in
 		domain_left = domain_left_in; domain_right = domain_right_in
 		domain_bottom = domain_bottom_in; domain_top = domain_top_in

@@ -12,7 +12,7 @@ contains
 		real, intent(out) :: x(n), w(n)
 		real, intent(in), optional :: interval(2)
 		real :: D(n,n), Pmn(n,n), Pnm(n,n)
-		if (n < 2 .or. n > 7) error stop "gauss_lobatto: n must be in [2,7], otherwise not implemented."
+		if (n < 1 .or. n > 7) error stop "gauss_lobatto: n must be in [1,7], otherwise not implemented."
 		if (ppes_tables(n-1, x, w, D, Pmn, Pnm) /= PPES_OK) error stop 'gauss_lobatto: table failure'
 		if (present(interval)) then
 			x = (0.5*(interval(2)-interval(1))) * x + 0.5*(interval(1)+interval(2))
@@ -25,7 +25,7 @@ contains
 		real, dimension(p+1,p+1), intent(out) :: M, Minv, D, Pmn, Pnm
 		real :: x(p+1), w(p+1), gm(p+1)
 		integer :: j
-		if (p < 1 .or. p > 6) error stop "gauss_lobatto_prep_table: p must be in [1,6], otherwise not implemented."
+		if (p < 0 .or. p > 6) error stop "gauss_lobatto_prep_table: p must be in [0,6], otherwise not implemented."
 		if (ppes_tables(p, x, w, D, Pmn, Pnm) /= PPES_OK) error stop 'gauss_lobatto_matrix: table failure'
 		do j = 1, p+1
 			gm(j) = 2.0/(2*j-1)

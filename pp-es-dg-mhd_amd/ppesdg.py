@@ -3,8 +3,8 @@
 The class ``Solver`` mirrors the reference's module API (DG_solver.f90,
 drive.f90, problems.f90): ``post_process``, ``semidiscretize``,
 ``rk_combine``, ``apply_bc``, ``solve`` (= solve_execute). Host arrays use the
-reference's Fortran layouts: state (8, N, N, nx+2, ny+2), residual
-(8, N, N, nx, ny), averages (8, nx+2, ny+2). ``Batch`` runs many independent
+reference's Fortran layouts: state (8, NX, NY, nx+2, ny+2), residual
+(8, NX, NY, nx, ny), averages (8, nx+2, ny+2), NX = p + 1, NY = py + 1. ``Batch`` runs many independent
 instances (own mesh, degree, BC, dt and end time) through shared launches.
 
 There is no CPU fallback: if the HIP library is missing, loading raises.
@@ -147,9 +147,10 @@ def ldf_nodal_operator(p, hx, hy):
     return op
 
 
-def make_config(nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
+def make_config(nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3, py=None):
+    """Degree p in x and py in y (default p; py = 0: the reference's 1D set-up)."""
     cfg = Config()
-    cfg.nx, cfg.ny, cfg.px, cfg.py, cfg.rk_stages = nx, ny, p, p, rk
+    cfg.nx, cfg.ny, cfg.px, cfg.py, cfg.rk_stages = nx, ny, p, p if py is None else py, rk
     cfg.left, cfg.right, cfg.bottom, cfg.top = [float(v) for v in domain]
     cfg.gamma = float(gamma)
     cfg.bc_kind = BC[bc]
@@ -159,9 +160,9 @@ def make_config(nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3):
     return cfg
 
 
-def instance_bytes(nx, ny, p, rk=3):
+def instance_bytes(nx, ny, p, rk=3, py=None):
     """Device bytes one instance of this size holds (host-side formula)."""
-    cfg = make_config(nx, ny, p, (0.0, 1.0, 0.0, 1.0), 1.4, rk=rk)
+    cfg = make_config(nx, ny, p, (0.0, 1.0, 0.0, 1.0), 1.4, rk=rk, py=py)
     return lib().ppes_instance_bytes(ctypes.byref(cfg))
 
 
@@ -172,17 +173,18 @@ def _stats(st):
 class Solver:
     """One device-resident instance (init_DG_solver ... finalize_DG_solver)."""
 
-    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3, arith=None):
-        cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, rk)
+    def __init__(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3, arith=None, py=None):
+        cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, rk, py)
         h = ctypes.c_void_p()
         _check(lib().ppes_create(ctypes.byref(cfg), ctypes.byref(h)), "ppes_create")
         self.h = h
         if arith is not None:
             self.set_arith(arith)
-        self.nx, self.ny, self.p, self.N, self.rk = nx, ny, p, p + 1, rk
-        N = self.N
-        self.shape_u = (8, N, N, nx + 2, ny + 2)
-        self.shape_ut = (8, N, N, nx, ny)
+        self.nx, self.ny, self.p, self.py, self.rk = nx, ny, p, cfg.py, rk
+        self.N, self.NX, self.NY = p + 1, p + 1, cfg.py + 1
+        NX, NY = self.NX, self.NY
+        self.shape_u = (8, NX, NY, nx + 2, ny + 2)
+        self.shape_ut = (8, NX, NY, nx, ny)
         self.shape_avg = (8, nx + 2, ny + 2)
 
     def close(self):
@@ -196,7 +198,7 @@ class Solver:
     def geometry(self):
         hx, hy = ctypes.c_double(), ctypes.c_double()
         gx, gy = np.zeros(self.nx + 1), np.zeros(self.ny + 1)
-        xx, yy = np.zeros((self.N, self.nx), order="F"), np.zeros((self.N, self.ny), order="F")
+        xx, yy = np.zeros((self.NX, self.nx), order="F"), np.zeros((self.NY, self.ny), order="F")
         _check(lib().ppes_geometry(self.h, ctypes.byref(hx), ctypes.byref(hy), _dp(gx), _dp(gy), _dp(xx), _dp(yy)),
                "ppes_geometry")
         return dict(hx=hx.value, hy=hy.value, grid_x=gx, grid_y=gy, xx=xx, yy=yy)
@@ -323,12 +325,11 @@ class Batch:
     def __del__(self):
         self.close()
 
-    def add(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None):
-        cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, self.rk)
+    def add(self, nx, ny, p, domain, gamma, bc="periodic", bc_state=None, py=None):
+        cfg = make_config(nx, ny, p, domain, gamma, bc, bc_state, self.rk, py)
         i = ctypes.c_int()
         _check(lib().ppes_batch_add(self.h, ctypes.byref(cfg), ctypes.byref(i)), "ppes_batch_add")
-        N = p + 1
-        self.shapes[i.value] = (8, N, N, nx + 2, ny + 2)
+        self.shapes[i.value] = (8, p + 1, cfg.py + 1, nx + 2, ny + 2)
         return i.value
 
     def remove(self, i):

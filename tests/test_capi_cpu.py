@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 4, 5, 6])
 def test_library_tables_equal_oracle(p):
     a, b = ppesdg.tables(p), O.tables(p)
     for k in ("x", "w", "D", "Pmn", "Pnm"):
@@ -56,3 +56,16 @@ def test_ldf_nodal_operator_matches_modal_projector(p, hx, hy):
     assert np.max(np.abs(A - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
     # a constant field (in the nullspace of the divergence) is unchanged
     assert np.max(np.abs(A @ np.ones(2 * NN) - 1.0)) < 1e-13
+
+
+def test_element_shapes_accepted():
+    """The device takes py = px in 1..4 and the reference's 1D set-ups
+    py = 0 (px in 1..4); other pairs are rejected before any allocation."""
+    for p in (1, 2, 3, 4):
+        assert ppesdg.instance_bytes(64, 64, p) > 0
+        assert ppesdg.instance_bytes(480, 1, p, rk=5, py=0) > 0
+    for p, py in ((3, 1), (2, 3), (0, 0), (5, 5), (5, 0)):
+        assert ppesdg.instance_bytes(16, 16, p, py=py) == -1
+    # a P3 x P0 element holds a quarter of a P3 element's nodes
+    a, b = ppesdg.instance_bytes(480, 1, 3, rk=5, py=0), ppesdg.instance_bytes(480, 1, 3, rk=5)
+    assert 3.0 < b / a < 4.5

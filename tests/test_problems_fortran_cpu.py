@@ -23,14 +23,15 @@ PROGRAM = """
 program problems_check
 	use MHD_PDE_m, only: set_gamma
 	use problems
+	use gauss_m
 	implicit none
-	integer :: u, what, prob, nx, ny, px
-	real :: g, hx, hy, t, e(2)
+	integer :: u, what, prob, nx, ny, px, py
+	real :: g, hx, hy, t, e(2), x1(1), w1(1), m1(1,1,5)
 	real, allocatable :: xx(:,:), yy(:,:), uc(:,:,:,:,:)
 	open(newunit=u, file='in.bin', access='stream', form='unformatted', status='old')
-	read(u) what, prob, nx, ny, px
+	read(u) what, prob, nx, ny, px, py
 	read(u) g, hx, hy, t
-	allocate(xx(px+1, nx), yy(px+1, ny), uc(8, px+1, px+1, 0:nx+1, 0:ny+1))
+	allocate(xx(px+1, nx), yy(py+1, ny), uc(8, px+1, py+1, 0:nx+1, 0:ny+1))
 	read(u) xx, yy, uc
 	close(u)
 	call set_gamma(g)
@@ -46,20 +47,26 @@ program problems_check
 		end select
 		if (prob == 12) then
 			init_con_fun => YeeSjogreenInitCon
-			call init_by_fun_con(uc, xx, yy, nx, ny, px, px)
+			call init_by_fun_con(uc, xx, yy, nx, ny, px, py)
 		else
-			call init_by_fun_pri(uc, xx, yy, nx, ny, px, px)
+			call init_by_fun_pri(uc, xx, yy, nx, ny, px, py)
 		end if
 	case (1)  ! boundary filler
 		select case (prob)
-		case (4); call BC_BrioWu(uc, xx, yy, nx, ny, px, px)
-		case (5); call BC_cloud_shock(uc, xx, yy, nx, ny, px, px)
-		case (6); call BC_jet1_cell_ext(uc, xx, yy, nx, ny, px, px)
+		case (4); call BC_BrioWu(uc, xx, yy, nx, ny, px, py)
+		case (5); call BC_cloud_shock(uc, xx, yy, nx, ny, px, py)
+		case (6); call BC_jet1_cell_ext(uc, xx, yy, nx, ny, px, py)
 		end select
 	case (2)
-		call compute_error_smooth_alfven(uc, hx, hy, xx, yy, nx, ny, px, px, e(1), e(2))
+		call compute_error_smooth_alfven(uc, hx, hy, xx, yy, nx, ny, px, py, e(1), e(2))
 	case (3)
-		call compute_error_smooth1D(uc, hx, hy, xx, yy, nx, ny, px, px, t, e(1), e(2))
+		call compute_error_smooth1D(uc, hx, hy, xx, yy, nx, ny, px, py, t, e(1), e(2))
+	case (4)  ! the one-node tables of a degree-0 direction
+		call gauss_lobatto(1, x1, w1)
+		call gauss_lobatto_matrix(0, m1(:,:,1), m1(:,:,2), m1(:,:,3), m1(:,:,4), m1(:,:,5))
+		uc = 0.0
+		uc(1:5, 1, 1, 0, 0) = m1(1, 1, :)
+		e = [x1(1), w1(1)]
 	end select
 	open(newunit=u, file='out.bin', access='stream', form='unformatted', status='replace')
 	write(u) uc, e
@@ -87,9 +94,10 @@ def prog(tmp_path_factory):
 def call(d, what, prob, o, s, u, t=0.0):
     """Run the check program on oracle instance o (set-up s) and state u."""
     nx, ny, p = o.nx, o.ny, o.p
+    py = o.shape_u[2] - 1
     dom = s["domain"]
     with open(d / "in.bin", "wb") as f:
-        np.array([what, prob, nx, ny, p], dtype="<i4").tofile(f)
+        np.array([what, prob, nx, ny, p, py], dtype="<i4").tofile(f)
         np.array([s["gamma"], (dom[1] - dom[0]) / nx, (dom[3] - dom[2]) / ny, t], dtype="<f8").tofile(f)
         for a in (o.xx, o.yy, u):
             np.asfortranarray(a, dtype="<f8").ravel(order="F").tofile(f)
@@ -99,9 +107,9 @@ def call(d, what, prob, o, s, u, t=0.0):
     return out[:n].reshape(o.shape_u, order="F"), out[n:]
 
 
-def make(prob, nx, ny, p):
+def make(prob, nx, ny, p, py=None):
     s = setup(prob)
-    return O.Oracle(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"]), s
+    return O.Oracle(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"], py=py), s
 
 
 @pytest.mark.parametrize("p", [1, 3])
@@ -139,6 +147,35 @@ def test_error_functionals(prog):
     ref = o.error_smooth_alfven(u)
     assert abs(e[0] - ref[0]) <= 1e-14 * ref[0] and e[1] == ref[1]
     o, s = make("wave1d", 12, 2, 3)
+    o.init_problem("wave1d")
+    u = np.array(o.u, copy=True)
+    _, e = call(prog, 3, 0, o, s, u, t=0.05)
+    ref = o.error_smooth1d(0.05, u)
+    assert abs(e[0] - ref[0]) <= 1e-14 * ref[0] and e[1] == ref[1]
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_degree_zero_direction(prog, p):
+    """The reference's 1D set-ups (py = 0, main.f90 cases 0 and 1): the
+    one-node Gauss-Lobatto rule and tables (x = 0, w = 2, M = 2, M^-1 = 1/2,
+    D = 0, Pmn = Pnm = 1, gauss.f90:35-37,165-170), and the Brio-Wu data and
+    filler and the 1D error functional on a P x P0 strip."""
+    o, s = make("briowu", 16, 1, p, py=0)
+    got, e = call(prog, 4, 0, o, s, np.zeros(o.shape_u))
+    assert list(e) == [0.0, 2.0]
+    assert list(got[:5, 0, 0, 0, 0]) == [2.0, 0.5, 0.0, 1.0, 1.0]
+    o.init_problem("briowu")
+    got, _ = call(prog, 0, O.PROB["briowu"], o, s, np.zeros(o.shape_u))
+    assert np.array_equal(got[:, :, :, 1:-1, 1:-1], np.array(o.u)[:, :, :, 1:-1, 1:-1])
+    u = np.array(o.u, copy=True)
+    u[:, :, :, 1:-1, 1:-1] *= 1.0 + 0.01 * np.random.default_rng(5).standard_normal(u[:, :, :, 1:-1, 1:-1].shape)
+    got, _ = call(prog, 1, O.BC["briowu"], o, s, u)
+    ref = np.array(u, copy=True)
+    o.apply_bc(ref)
+    for sl in ((slice(None),) * 3 + (0, slice(1, -1)), (slice(None),) * 3 + (-1, slice(1, -1)),
+               (slice(None),) * 3 + (slice(1, -1), 0), (slice(None),) * 3 + (slice(1, -1), -1)):
+        assert np.array_equal(got[sl], ref[sl])
+    o, s = make("wave1d", 24, 1, p, py=0)
     o.init_problem("wave1d")
     u = np.array(o.u, copy=True)
     _, e = call(prog, 3, 0, o, s, u, t=0.05)
