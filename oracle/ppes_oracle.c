@@ -25,7 +25,11 @@
 #define OEDG_EPS 1.0e-8    /* DG_solver.f90:148 */
 #define MAXN 7
 
+/* extended precision of the table builders; the operation-counting build
+ * (flopcount.cpp), which renames double, declares it first */
+#ifndef ORC_HAVE_LDBL
 typedef long double ldbl;
+#endif
 
 struct orc_solver {
     int nx, ny, px, py, NX, NY, rk; /* degrees and node counts in x and y (py may be 0: N_y = 1) */

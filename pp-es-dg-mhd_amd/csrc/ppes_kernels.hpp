@@ -40,7 +40,7 @@ __device__ __forceinline__ StageSrc stage_src(const Inst &I, int st)
 // RK stage combine of one value from u^n, the residual and the earlier stage
 // the method uses (u1 in RK2 / RK3 stage 2, u2 in RK3 stage 3, u(st-1) in RK5
 // stages 2-4), in the reference's written form for each method and stage
-// (drive.f90:130-272). RK5 stage 5 has five sources: rk_combine_value.
+// (drive.f90:130-272). RK5 stage 5 has five sources: rk_combine_node.
 __device__ __forceinline__ double combine3(int S, int st, double dt, double un, double t, double up)
 {
     switch (S) {
@@ -63,19 +63,46 @@ __device__ __forceinline__ double combine3(int S, int st, double dt, double un, 
     }
 }
 
-// the earlier stage combine3 reads (nullptr when it reads none): u1 in RK2 /
-// RK3 stage 2, u2 in RK3 stage 3, u(st-1) in RK5 stages 2-4
-__device__ __forceinline__ const gdbl *combine_prev(const StageSrc &c) { return c.prev; }
-
-// RK stage combine of one value read straight from memory. os / ot index the
-// state / residual arrays.
-__device__ __forceinline__ double rk_combine_value(const StageSrc &c, long long os, long long ot)
+// RK stage combine of the eight values of one node read straight from
+// memory (os / ot index variable 0 of the state / residual arrays, vs / vsi
+// their plane sizes). Every load is issued before the first use, so that a
+// wave has all of them in flight at once. RK5 stage 5 has five sources;
+// every other stage reads u^n, the residual and at most one earlier stage
+// (u1 in RK2 / RK3 stage 2, u2 in RK3 stage 3, u(st-1) in RK5 stages 2-4).
+__device__ __forceinline__ void rk_combine_node(const StageSrc &c, long long os, long long ot, long long vs,
+                                                long long vsi, double q[8])
 {
-    if (c.S == 5 && c.st == 5)
-        return 0.517231671970585 * c.u2[os] + 0.096059710526147 * c.u3[os] + (0.063692468666290 * c.dt) * c.t4[ot] +
-               0.386708617503269 * c.u4[os] + (0.226007483236906 * c.dt) * c.t[ot];
-    const gdbl *pv = combine_prev(c);
-    return combine3(c.S, c.st, c.dt, c.un[os], c.t[ot], pv ? pv[os] : 0.0);
+    if (c.S == 5 && c.st == 5) {
+        double a[8], b[8], d[8], e[8], f[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            a[v] = c.u2[os + v * vs];
+            b[v] = c.u3[os + v * vs];
+            d[v] = c.t4[ot + v * vsi];
+            e[v] = c.u4[os + v * vs];
+            f[v] = c.t[ot + v * vsi];
+        }
+#pragma unroll
+        for (int v = 0; v < 8; ++v)
+            q[v] = 0.517231671970585 * a[v] + 0.096059710526147 * b[v] + (0.063692468666290 * c.dt) * d[v] +
+                   0.386708617503269 * e[v] + (0.226007483236906 * c.dt) * f[v];
+        return;
+    }
+    double un[8], tt[8], up[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        un[v] = c.un[os + v * vs];
+        tt[v] = c.t[ot + v * vsi];
+    }
+    if (c.prev) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) up[v] = c.prev[os + v * vs];
+    } else {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) up[v] = 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) q[v] = combine3(c.S, c.st, c.dt, un[v], tt[v], up[v]);
 }
 
 // --------------------------------------------------------------------------
@@ -151,6 +178,30 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = up[v][base];
     }
+    // row k of Dx and row l of Dy: the only differentiation entries this
+    // node's volume terms use, loaded once beside the state (the record lives
+    // in global memory, so an indexed read inside the pair loops would wait
+    // on memory at every use). They stay in registers, or, for elements of
+    // more than 16 nodes, whose registers are all taken, in LDS.
+    constexpr bool DREG = NN <= 16;
+    __shared__ double sdrow[DREG ? 1 : NX * NX + NY * NY];
+    double dxr[DREG ? NX : 1], dyr[DREG ? NY : 1];
+    if constexpr (DREG) {
+#pragma unroll
+        for (int r = 0; r < NX; ++r) dxr[r] = I.Dx[k + NX * r];
+#pragma unroll
+        for (int s = 0; s < NY; ++s) dyr[s] = I.Dy[l + NY * s];
+    } else {
+        for (int a = t; a < NX * NX + NY * NY; a += TB) sdrow[a] = a < NX * NX ? I.Dx[a] : I.Dy[a - NX * NX];
+    }
+    auto DX = [&](int r) -> double {
+        if constexpr (DREG) return dxr[r];
+        else return sdrow[k + NX * r];
+    };
+    auto DY = [&](int s) -> double {
+        if constexpr (DREG) return dyr[s];
+        else return sdrow[NX * NX + l + NY * s];
+    };
     auto load_traces = [&]() {
 #pragma unroll
         for (int r = 0; r < FR; ++r) {
@@ -208,9 +259,9 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         {
             double a = 0.0, b = 0.0;
 #pragma unroll
-            for (int r = 0; r < NX; ++r) a += I.Dx[k + NX * r] * sprim[el][r + NX * l].bx;
+            for (int r = 0; r < NX; ++r) a += DX(r) * sprim[el][r + NX * l].bx;
 #pragma unroll
-            for (int s = 0; s < NY; ++s) b += sprim[el][k + NX * s].by * I.Dy[l + NY * s];
+            for (int s = 0; s < NY; ++s) b += sprim[el][k + NX * s].by * DY(s);
             divB = (2.0 / hx) * a + (2.0 / hy) * b;
         }
         const Prim &pr = sprim[el][n];
@@ -229,7 +280,12 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
                              ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
         const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
-        const double dxx = cx * I.Dx[k + NX * k], dyy = cy * I.Dy[l + NY * l];
+        double dkk = 0.0, dll = 0.0;
+#pragma unroll
+        for (int r = 0; r < NX; ++r) dkk = r == k ? DX(r) : dkk;
+#pragma unroll
+        for (int s = 0; s < NY; ++s) dll = s == l ? DY(s) : dll;
+        const double dxx = cx * dkk, dyy = cy * dll;
 #pragma unroll
         for (int v = 0; v < 8; ++v) acc[v] = 0.0 - dxx * F[v] - dyy * G[v] - divB * GP[v];
     }
@@ -305,13 +361,20 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         }
         __syncthreads();
         if (valid) {
-            for (int r = 0; r < k; ++r) sub_x(cx * I.Dx[k + NX * r], &sbuf[el][(l * HX + k * (k - 1) / 2 + r) * REC]);
-            for (int s = 0; s < l; ++s)
-                sub_y(cy * I.Dy[l + NY * s], &sbuf[el][(XP + k * HY + l * (l - 1) / 2 + s) * REC]);
-            for (int kk = k + 1; kk < NX; ++kk)
-                sub_x(cx * I.Dx[k + NX * kk], &sbuf[el][(l * HX + kk * (kk - 1) / 2 + k) * REC]);
-            for (int ll = l + 1; ll < NY; ++ll)
-                sub_y(cy * I.Dy[l + NY * ll], &sbuf[el][(XP + k * HY + ll * (ll - 1) / 2 + l) * REC]);
+            // partners below the node in x, below in y, above in x, above in
+            // y: the unrolled loops keep the row entries in registers
+#pragma unroll
+            for (int r = 0; r < NX; ++r)
+                if (r < k) sub_x(cx * DX(r), &sbuf[el][(l * HX + k * (k - 1) / 2 + r) * REC]);
+#pragma unroll
+            for (int s = 0; s < NY; ++s)
+                if (s < l) sub_y(cy * DY(s), &sbuf[el][(XP + k * HY + l * (l - 1) / 2 + s) * REC]);
+#pragma unroll
+            for (int kk = 0; kk < NX; ++kk)
+                if (kk > k) sub_x(cx * DX(kk), &sbuf[el][(l * HX + kk * (kk - 1) / 2 + k) * REC]);
+#pragma unroll
+            for (int ll = 0; ll < NY; ++ll)
+                if (ll > l) sub_y(cy * DY(ll), &sbuf[el][(XP + k * HY + ll * (ll - 1) / 2 + l) * REC]);
         }
     } else {
         // one round's records at a time (half the LDS of the ordered form):
@@ -321,15 +384,17 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             if (valid && n + c * NN < S::PAIRS) pair_flux(n + c * NN, &sbuf[el][n * REC]);
             __syncthreads();
             if (valid) {
+#pragma unroll
                 for (int r = 0; r < NX; ++r) {
                     if (r == k) continue;
                     const int qp = l * HX + (r > k ? r * (r - 1) / 2 + k : k * (k - 1) / 2 + r);
-                    if (qp / NN == c) sub_x(cx * I.Dx[k + NX * r], &sbuf[el][(qp - c * NN) * REC]);
+                    if (qp / NN == c) sub_x(cx * DX(r), &sbuf[el][(qp - c * NN) * REC]);
                 }
+#pragma unroll
                 for (int s2 = 0; s2 < NY; ++s2) {
                     if (s2 == l) continue;
                     const int qp = XP + k * HY + (s2 > l ? s2 * (s2 - 1) / 2 + l : l * (l - 1) / 2 + s2);
-                    if (qp / NN == c) sub_y(cy * I.Dy[l + NY * s2], &sbuf[el][(qp - c * NN) * REC]);
+                    if (qp / NN == c) sub_y(cy * DY(s2), &sbuf[el][(qp - c * NN) * REC]);
                 }
             }
             if (c + 1 < ROUNDS) __syncthreads();
@@ -731,8 +796,7 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
             for (int v = 0; v < 8; ++v) q[v] = out[v * vs + base];
         } else if (interior) {
             const long long gt = ((long long)(ey - 1) * I.nx + (ex - 1)) * NN + n;
-#pragma unroll
-            for (int v = 0; v < 8; ++v) q[v] = rk_combine_value(src, v * vs + base, v * I.vsi + gt);
+            rk_combine_node(src, base, gt, vs, I.vsi, q);
         } else {
             const GhostSrc gsrc = ghost_src<NX, NY>(I, ex, ey, k, l);
             if (gsrc.mode == 1) {
@@ -741,9 +805,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
             } else {
                 const long long gs = gidx(I, gsrc.sx, gsrc.sy) * NN + gsrc.sn;
                 const long long gt = ((long long)(gsrc.sy - 1) * I.nx + (gsrc.sx - 1)) * NN + gsrc.sn;
+                rk_combine_node(src, gs, gt, vs, I.vsi, q);
 #pragma unroll
-                for (int v = 0; v < 8; ++v)
-                    q[v] = ghost_value(gsrc.mode, v, rk_combine_value(src, v * vs + gs, v * I.vsi + gt));
+                for (int v = 0; v < 8; ++v) q[v] = ghost_value(gsrc.mode, v, q[v]);
             }
         }
         if (flags & F_COMBINE_ONLY) {
@@ -1065,6 +1129,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     __shared__ double sent[EPB];
     __shared__ int scnt[EPB];
     __shared__ double sth[EO ? EPB : 1];
+    __shared__ double ssig[EPB][16];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -1086,6 +1151,13 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
         const gdbl *ua = gin(I.uavg) + ge * 8;
 #pragma unroll
         for (int v = 0; v < 8; ++v) { q[v] = u[v * vs + base]; avg[v] = ua[v]; }
+        // the element's face indicator sums, x-faces (left + right) then
+        // y-faces (bottom + top) per variable, loaded with the state by the
+        // element's lanes into LDS
+        const gdbl *sxL = gin(I.sigx) + 8 * ((long long)(i - 1) + (long long)(I.nx + 1) * (j - 1));
+        const gdbl *syB = gin(I.sigy) + 8 * ((long long)(i - 1) + (long long)I.nx * (j - 1));
+        for (int a = n; a < 16; a += NN)
+            ssig[el][a] = a < 8 ? sxL[a] + sxL[8 + a] : syB[a - 8] + syB[8 * (long long)I.nx + a - 8];
     }
     {
         double sx = 0.0, sy = 0.0;
@@ -1096,17 +1168,13 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
         if (valid) {
             double mex = 0.0, mey = 0.0;
             for (int m = 0; m < NN; ++m) { mex = dmax(mex, sred[el * NN + m][0]); mey = dmax(mey, sred[el * NN + m][1]); }
-            const gdbl *sxL = gin(I.sigx) + 8 * ((long long)(i - 1) + (long long)(I.nx + 1) * (j - 1));
-            const gdbl *sxR = gin(I.sigx) + 8 * ((long long)i + (long long)(I.nx + 1) * (j - 1));
-            const gdbl *syB = gin(I.sigy) + 8 * ((long long)(i - 1) + (long long)I.nx * (j - 1));
-            const gdbl *syT = gin(I.sigy) + 8 * ((long long)(i - 1) + (long long)I.nx * j);
             double dm = 0.0;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const double var = I.ctl.glob[8 + v];
                 double delta;
                 if (var > OEDG_EPS)
-                    delta = ((mex / I.hx) * (sxL[v] + sxR[v]) + (mey / I.hy) * (syB[v] + syT[v])) / var;
+                    delta = ((mex / I.hx) * ssig[el][v] + (mey / I.hy) * ssig[el][8 + v]) / var;
                 else
                     delta = 0.0;
                 dm = (v == 0) ? delta : dmax(dm, delta);
