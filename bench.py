@@ -15,8 +15,9 @@ and the final states of both paths are compared (matched L1 error).
 
 Also reported: the single-instance workload (Orszag-Tang 2pi + seeded
 perturbation, P3, 1024 x 1024, 200 RK steps), per-kernel device times (HIP
-events on the engine's stream), the roofline of the dominant kernel and a
-CPU baseline (the CPU oracle on a bounded sample of the mixed workload).
+events on the engine's stream), the HBM roofline of the dominant kernel and
+the residual's algorithmic FP64 rate against the FP64 peak, and a CPU
+baseline (the CPU oracle on a bounded sample of the mixed workload).
 
 value = DOF-stage updates per second: a DOF is one GLL node of one element
 (8 FP64 variables); a DOF-stage update is one node advanced through one RK
@@ -52,6 +53,19 @@ ALG_BYTES_PER_NODE = {"residual": 128.0, "stage": (192.0 + 256.0 + 256.0) / 3.0,
                       "limit": 128.0, "reduce": 0.0, "bc": 0.0}
 # whole path, SSP-RK3 with OEDG on the final stage: 640 B per node-step
 STEP_BYTES_PER_NODE_STAGE = 640.0 / 3.0
+
+# Algorithmic FP64 operations per GLL node (add, sub, mul, div, sqrt, log,
+# exp each one; the reference's algorithm: each EC pair flux and each face
+# flux evaluated once), counted by oracle/flopcount.cpp on an Orszag-Tang
+# state: one residual, and one node-stage of SSP-RK3 with OEDG on the final
+# stage (residual + combine + post-process).
+ALG_FLOPS_PER_NODE = {
+    1: {"residual": 788.19, "node_stage": 1177.85},
+    2: {"residual": 796.79, "node_stage": 1347.11},
+    3: {"residual": 884.47, "node_stage": 1659.74},
+    4: {"residual": 1003.60, "node_stage": 2062.88},
+}
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix alike)
 
 
 def add_instance(b, inst, max_steps):
@@ -147,6 +161,8 @@ def run_single(ppesdg, steps, warmup):
            "residual_avg_launch_ms": r[0] / max(r[1], 1),
            "residual_hbm_frac": ALG_BYTES_PER_NODE["residual"] * nodes / (r[0] / max(r[1], 1) / 1e3) / 1e9
            / HBM_PEAK_GBS,
+           "residual_fp64_frac": ALG_FLOPS_PER_NODE[p]["residual"] * nodes / (r[0] / max(r[1], 1) / 1e3) / 1e12
+           / FP64_PEAK_TFLOPS,
            "min_rho": float(u[0].min()), "min_p": float(prs.min()), "retries": after["retries"]}
     s.close()
     return res
@@ -268,6 +284,10 @@ def main():
     bytes_per_launch = alg_bytes_step / per_step_launches
     avg_s = tot_ms / launches / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
+    # the same kernel's algorithmic FP64 rate (residual: per-degree counts)
+    resid_flops_step = 3 * sum(W.nodes(m) * ALG_FLOPS_PER_NODE[m["p"]]["residual"] for m in mix)
+    resid_s_step = kt["residual"][0] / a.timing_steps / 1e3
+    stage_flops = sum(W.nodes(m) * ALG_FLOPS_PER_NODE[m["p"]]["node_stage"] for m in mix) / nodes_total
 
     out = {
         "metric": METRIC,
@@ -291,9 +311,15 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(dominant),
                      "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": bytes_per_launch,
                      "timing_steps": a.timing_steps},
+        "roofline_fp64": {"kernel": "residual", "alg_flops_per_step": resid_flops_step,
+                          "achieved": resid_flops_step / resid_s_step / 1e12, "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": resid_flops_step / resid_s_step / 1e12 / FP64_PEAK_TFLOPS},
         "step_roofline": {"alg_bytes_per_node_stage": STEP_BYTES_PER_NODE_STAGE,
                           "achieved_GBs": STEP_BYTES_PER_NODE_STAGE * value / 1e9,
-                          "frac_hbm": STEP_BYTES_PER_NODE_STAGE * value / 1e9 / HBM_PEAK_GBS},
+                          "frac_hbm": STEP_BYTES_PER_NODE_STAGE * value / 1e9 / HBM_PEAK_GBS,
+                          "alg_flops_per_node_stage": stage_flops,
+                          "achieved_TFLOPs": stage_flops * value / 1e12,
+                          "frac_fp64": stage_flops * value / 1e12 / FP64_PEAK_TFLOPS},
         "kernel_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()},
     }
     print(f"# batched {value:.4e} DOF-stage/s", file=sys.stderr, flush=True)

@@ -202,28 +202,28 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         if constexpr (DREG) return dyr[s];
         else return sdrow[NX * NX + l + NY * s];
     };
-    auto load_traces = [&]() {
-#pragma unroll
-        for (int r = 0; r < FR; ++r) {
-            const int f = n + r * NN;
-            if (valid && f < S::FACE) {
-                // faces: left, right (NY nodes each), bottom, top (NX each)
-                const int side = f < 2 * NY ? f / NY : 2 + (f - 2 * NY) / NX;
-                const int a = f < 2 * NY ? f - side * NY : f - 2 * NY - (side - 2) * NX;
-                unsigned b1, b2;
-                if (side < 2) {
-                    b1 = node(side == 0 ? i - 1 : i, j) + (NX - 1) + NX * a;
-                    b2 = node(side == 0 ? i : i + 1, j) + NX * a;
-                } else {
-                    b1 = node(i, side == 2 ? j - 1 : j) + a + NX * (NY - 1);
-                    b2 = node(i, side == 2 ? j : j + 1) + a;
-                }
-#pragma unroll
-                for (int v = 0; v < 8; ++v) { tl[r][v] = up[v][b1]; tr[r][v] = up[v][b2]; }
+    auto load_round = [&](int r) {
+        const int f = n + r * NN;
+        if (valid && f < S::FACE) {
+            // faces: left, right (NY nodes each), bottom, top (NX each)
+            const int side = f < 2 * NY ? f / NY : 2 + (f - 2 * NY) / NX;
+            const int a = f < 2 * NY ? f - side * NY : f - 2 * NY - (side - 2) * NX;
+            unsigned b1, b2;
+            if (side < 2) {
+                b1 = node(side == 0 ? i - 1 : i, j) + (NX - 1) + NX * a;
+                b2 = node(side == 0 ? i : i + 1, j) + NX * a;
+            } else {
+                b1 = node(i, side == 2 ? j - 1 : j) + a + NX * (NY - 1);
+                b2 = node(i, side == 2 ? j : j + 1) + a;
             }
+#pragma unroll
+            for (int v = 0; v < 8; ++v) { tl[r][v] = up[v][b1]; tr[r][v] = up[v][b2]; }
         }
     };
-    if constexpr (PREFETCH) load_traces();
+    if constexpr (PREFETCH) {
+#pragma unroll
+        for (int r = 0; r < FR; ++r) load_round(r);
+    }
 
     // ---- node primitives (DG_solver.f90:734-801)
     double msx = 0.0, msy = 0.0;
@@ -411,9 +411,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 
     // ---- face fluxes of the element's four faces; both neighbours compute a
     // shared face from identical inputs, so they agree bit for bit
-    if constexpr (!PREFETCH) load_traces();
 #pragma unroll
     for (int r = 0; r < FR; ++r) {
+        // the fast form loads each round's traces just before its face
+        // flux: one round's traces in registers at a time
+        if constexpr (!PREFETCH) load_round(r);
         const int f = n + r * NN;
         if (valid && f < S::FACE) {
             // y-faces go through the x-flux with the x/y components swapped

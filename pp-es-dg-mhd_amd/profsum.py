@@ -8,6 +8,12 @@
         workload, from FETCH_SIZE / WRITE_SIZE (--pmc, KiB per dispatch), each
         scaled by what the same counter reads for a plain 1 GiB device copy
         (bench.py --calibrate-copy); prints a JSON record and a Markdown table
+    python pp-es-dg-mhd_amd/profsum.py fp64 --pmc D1 --trace D2 --kernel residual [--command C]
+        FP64 operations per launch of one kernel class in the mixed workload:
+        executed, from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of a
+        --pmc run (wave instructions x 64 lanes, an FMA two operations) and
+        from SQ_INSTS_VALU_FLOPS_FP64, and algorithmic (bench.py's counts per
+        node), with the rates over the launch durations of a --kernel-trace run
 
 Launches in which every workgroup exits at once (the retry residual of a step
 with no retry) are left out: their counters are under 2% of the largest
@@ -141,6 +147,83 @@ def traffic(a):
     return rec
 
 
+def full_durations(d):
+    """average duration (ns) of the full launches of each kernel in a
+    --kernel-trace run (largest grid, over 5% of the longest)"""
+    tr = list(csv.DictReader(open(one_csv(d, "*kernel_trace.csv"))))
+    per = collections.defaultdict(list)
+    for r in tr:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        per[name].append((int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]),
+                          int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+    out = {}
+    for name, v in per.items():
+        g = max(x[0] for x in v)
+        dmax = max(x[1] for x in v if x[0] == g)
+        full = [x[1] for x in v if x[0] == g and x[1] > 0.05 * dmax]
+        out[name] = sum(full) / len(full)
+    return out
+
+
+def fp64(a):
+    import workloads as W
+    sys.path.insert(0, os.path.dirname(HERE))
+    from bench import ALG_FLOPS_PER_NODE, FP64_PEAK_TFLOPS
+    mix = W.mixed_batch(64, seed=1)
+    nodes_p = collections.Counter()
+    for m in mix:
+        nodes_p[m["p"]] += W.nodes(m)
+    rows = list(csv.DictReader(open(one_csv(a.pmc, "*counter_collection.csv"))))
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    grid = {}
+    for r in rows:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        key = (name, int(r["Grid_Size"]), r["Dispatch_Id"])
+        per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+        grid[name] = max(grid.get(name, 0), int(r["Grid_Size"]))
+    dur = full_durations(a.trace)
+    kname = {v: k for k, v in CLASS_OF.items()}[a.kernel]
+    res = {}
+    for p in (1, 2, 3, 4):
+        names = [n for n in grid if n.startswith(f"ppes::{kname}<{p}, {p},")]
+        if not names:
+            continue
+        name = names[0]
+        disp = [c for (n, g, _), c in per.items() if n == name and g == grid[name]]
+        vmax = max(c["SQ_INSTS_VALU"] for c in disp)
+        disp = [c for c in disp if c["SQ_INSTS_VALU"] > 0.02 * vmax]
+        avg = lambda key: sum(c[key] for c in disp) / len(disp)
+        executed = 64.0 * (avg("SQ_INSTS_VALU_ADD_F64") + avg("SQ_INSTS_VALU_MUL_F64") + avg("SQ_INSTS_VALU_TRANS_F64")
+                           + 2.0 * avg("SQ_INSTS_VALU_FMA_F64"))
+        alg = ALG_FLOPS_PER_NODE[p][a.kernel] * nodes_p[p]
+        t = dur[name] * 1e-9
+        res[p] = {"kernel": name, "launches_counted": len(disp), "avg_launch_us": t * 1e6, "nodes": nodes_p[p],
+                  "alg_flops": alg, "executed_flops": executed, "flops_fp64_counter": avg("SQ_INSTS_VALU_FLOPS_FP64"),
+                  "valu_insts": avg("SQ_INSTS_VALU"),
+                  "alg_TFLOPs": alg / t / 1e12, "executed_TFLOPs": executed / t / 1e12,
+                  "alg_frac": alg / t / 1e12 / FP64_PEAK_TFLOPS, "executed_frac": executed / t / 1e12 / FP64_PEAK_TFLOPS}
+    tot_alg = sum(v["alg_flops"] for v in res.values())
+    tot_exe = sum(v["executed_flops"] for v in res.values())
+    tot_t = sum(v["avg_launch_us"] for v in res.values()) * 1e-6
+    out = [f"# FP64 operations of the {a.kernel} kernel: mixed 64-instance workload", "",
+           f"Command: `{a.command}`", "",
+           "Executed operations = 64 x (ADD + MUL + TRANS + 2 FMA) FP64 wave instructions per launch",
+           "(SQ_INSTS_VALU_*_F64); algorithmic = the reference algorithm's operation count per node",
+           "(oracle/flopcount.cpp) x nodes; rates over the average duration of the same launches in a",
+           f"--kernel-trace run; peak {FP64_PEAK_TFLOPS} TFLOP/s.", "",
+           "| degree | avg launch (us) | alg. GFLOP | executed GFLOP | alg. TFLOP/s | executed TFLOP/s | "
+           "alg. frac | executed frac |", "|---|---|---|---|---|---|---|---|"]
+    for p, v in res.items():
+        out.append(f"| P{p} | {v['avg_launch_us']:.1f} | {v['alg_flops'] / 1e9:.2f} | {v['executed_flops'] / 1e9:.2f} | "
+                   f"{v['alg_TFLOPs']:.2f} | {v['executed_TFLOPs']:.2f} | {v['alg_frac']:.3f} | "
+                   f"{v['executed_frac']:.3f} |")
+    out.append(f"| all degrees | {tot_t * 1e6:.1f} | {tot_alg / 1e9:.2f} | {tot_exe / 1e9:.2f} | "
+               f"{tot_alg / tot_t / 1e12:.2f} | {tot_exe / tot_t / 1e12:.2f} | "
+               f"{tot_alg / tot_t / 1e12 / FP64_PEAK_TFLOPS:.3f} | {tot_exe / tot_t / 1e12 / FP64_PEAK_TFLOPS:.3f} |")
+    out += ["", "```json", json.dumps({"per_degree": res}, indent=1), "```"]
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -153,9 +236,16 @@ def main():
         t.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
     t.add_argument("--kernel", default="residual")
     t.add_argument("--command", default="")
+    fl = sub.add_parser("fp64")
+    fl.add_argument("--pmc", required=True)
+    fl.add_argument("--trace", required=True)
+    fl.add_argument("--kernel", default="residual")
+    fl.add_argument("--command", default="")
     a = ap.parse_args()
     if a.cmd == "stats":
         sys.stdout.write(stats(a.dir, a.title, a.command))
+    elif a.cmd == "fp64":
+        sys.stdout.write(fp64(a))
     else:
         print(json.dumps(traffic(a), indent=1))
 
