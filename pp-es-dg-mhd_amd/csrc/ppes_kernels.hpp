@@ -756,7 +756,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     double(*sdb)[M] = reinterpret_cast<double(*)[M]>(sunion);
     double(*sred)[2] = reinterpret_cast<double(*)[2]>(sunion);
     __shared__ double savg[EPB][8];
-    __shared__ double sldf[M * M];
+    // the LDF operator as sldf[c][n] = (A[c][n], A[c][NN + n]): the two
+    // entries a node's (Bx, By) outputs take from input c, read together
+    __shared__ double sldf[M][NN][2];
     __shared__ int sok[EPB];
     __shared__ int scnt[EPB];
     __shared__ double sent[EPB];
@@ -785,9 +787,23 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     const bool use_ldf = I.ctl.use_ldf != 0;
     const StageSrc src = stage_src(I, st >= 1 ? st : 1);
 
-    if (use_ldf && !(flags & F_COMBINE_ONLY)) {
+    // the operator's loads are issued here, before the state's, and stored to
+    // LDS after them, so that both sets are in flight together (a loop that
+    // stored each entry before loading the next would wait on memory
+    // M^2 / BLOCK times in a row)
+    const bool stage_ldf = use_ldf && !(flags & F_COMBINE_ONLY);
+    constexpr int R = (M * NN + BLOCK - 1) / BLOCK;
+    double ax[R], ay[R];
+    if (stage_ldf) {
         const gdbl *ldf = gin(I.ldf);
-        for (int c = t; c < M * M; c += BLOCK) sldf[c] = ldf[c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int a = t + r * BLOCK, c = a / NN, m = a - c * NN;
+            if (a < M * NN) {
+                ax[r] = ldf[c * M + m];
+                ay[r] = ldf[c * M + NN + m];
+            }
+        }
     }
 
     // ---- the stage state at this node
@@ -822,6 +838,16 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
         }
     }
     if (flags & F_COMBINE_ONLY) return;
+    if (stage_ldf) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int a = t + r * BLOCK, c = a / NN, m = a - c * NN;
+            if (a < M * NN) {
+                sldf[c][m][0] = ax[r];
+                sldf[c][m][1] = ay[r];
+            }
+        }
+    }
     __syncthreads();
 
     // ---- cell average in the reference's order (DG_solver.f90:177-184), one
@@ -862,8 +888,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
 #pragma unroll 4
             for (int c = 0; c < M; ++c) {
                 const double d = sdb[el][c];
-                bxs = bxs + sldf[c * M + n] * d;
-                bys = bys + sldf[c * M + NN + n] * d;
+                bxs = bxs + sldf[c][n][0] * d;
+                bys = bys + sldf[c][n][1] * d;
             }
             q[4] = bxs + avg[4];
             q[5] = bys + avg[5];
