@@ -133,6 +133,10 @@ Engine::~Engine()
     }
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (auto &gs : gstreams) (void)hipStreamSynchronize(gs);
+    for (auto &gs : gstreams) (void)hipStreamDestroy(gs);
+    for (auto &ge : gevents) (void)hipEventDestroy(ge);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
     for (auto &g : groups) free_group(g);
     for (auto &g : api_groups) if (g.d_el) free_group(g);
     if (d_all) (void)hipFree(d_all);
@@ -586,12 +590,25 @@ int Engine::launch_step(bool all_kernels)
     int rc = build_plan();
     if (rc) return rc;
     if (n_all == 0) return PPES_OK;
-    LAUNCH1(k_step_begin, n_all, d_recs, (const int *)d_all, n_all);
     // a retried step recomputes L(u^n) from the saved ghost layer (drive.f90:104)
     bool any_retry = all_kernels;
     if (!all_kernels)
         for (const auto &g : groups)
             for (int id : g.members) any_retry = any_retry || h_recs[id].ctl.retry_residual;
+    if (concurrent && !timing && groups.size() > 1) {
+        // fork: every group's step on its own stream after the work queued so
+        // far; join: the engine's stream waits for all of them
+        if ((rc = ensure_group_streams(groups.size()))) return rc;
+        ENG_TRY(hipEventRecord(fork_ev, stream));
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            ENG_TRY(hipStreamWaitEvent(gstreams[gi], fork_ev, 0));
+            if ((rc = launch_group_step(groups[gi], any_retry, gstreams[gi]))) return rc;
+            ENG_TRY(hipEventRecord(gevents[gi], gstreams[gi]));
+        }
+        for (size_t gi = 0; gi < groups.size(); ++gi) ENG_TRY(hipStreamWaitEvent(stream, gevents[gi], 0));
+        return PPES_OK;
+    }
+    LAUNCH1(k_step_begin, n_all, d_recs, (const int *)d_all, n_all);
     if (any_retry) {
         for (auto &g : groups) ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 1));
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 1);
@@ -646,6 +663,58 @@ int Engine::launch_step(bool all_kernels)
     return PPES_OK;
 }
 
+int Engine::ensure_group_streams(size_t n)
+{
+    if (!fork_ev) ENG_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    while (gstreams.size() < n) {
+        hipStream_t s2;
+        hipEvent_t e2;
+        ENG_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        gstreams.push_back(s2);
+        ENG_TRY(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+        gevents.push_back(e2);
+    }
+    return PPES_OK;
+}
+
+// One RK step of the instances of one degree group on `stream`: the same
+// launches as launch_step's, over the group's member list. Groups hold
+// disjoint instances, so their steps are independent of one another.
+int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
+{
+    const int nm = (int)g.members.size();
+    const int *dl = g.d_list;
+    LAUNCH1(k_step_begin, nm, d_recs, dl, nm);
+    if (any_retry) {
+        ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 1));
+        LAUNCH1(k_pre_residual, nm, d_recs, dl, nm, 1);
+        RESIDUAL_DISPATCH(g, 0, 1, 1);
+    }
+    for (int st = 1; st <= S; ++st) {
+        if (g.eo) {
+            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, true>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+        } else {
+            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
+        }
+        if (any_oedg_stage[st]) {
+            LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, st, 0, 0);
+            LAUNCH(k_reduce_global, (long long)nm, d_recs, (const Blk *)g.d_red2, st, 0, 1);
+            ENG_DISPATCH(g.sid, LAUNCH((k_oedg_faces<PX, PY>), g.n_face, d_recs, (const Blk *)g.d_face, st, 0));
+            if (g.eo) {
+                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+            } else {
+                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, false>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+            }
+        }
+        if (st == S) LAUNCH(k_reduce_ent, (long long)nm, d_recs, dl, st, 0);
+        ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, st));
+        LAUNCH1(k_pre_residual, nm, d_recs, dl, nm, 0);
+        RESIDUAL_DISPATCH(g, st, (st % S) + 1, 0);
+    }
+    LAUNCH1(k_step_end, nm, d_recs, dl, nm);
+    return PPES_OK;
+}
+
 // capture one step of the current plan as a hipGraph
 int Engine::capture()
 {
@@ -653,6 +722,11 @@ int Engine::capture()
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     const bool was_timing = timing;
     timing = false;  // no events inside a captured graph
+    // the group streams exist before the capture that forks onto them
+    if (concurrent && groups.size() > 1) {
+        int rc0 = ensure_group_streams(groups.size());
+        if (rc0) { timing = was_timing; return rc0; }
+    }
     ENG_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int rc = launch_step(true);
     hipError_t ec = hipStreamEndCapture(stream, &graph);

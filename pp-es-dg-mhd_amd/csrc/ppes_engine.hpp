@@ -12,6 +12,10 @@
 #include <vector>
 
 #include "../../include/ppesdg.h"
+
+#ifndef PPES_CONCURRENT
+#define PPES_CONCURRENT 1
+#endif
 #include "ppes_types.hpp"
 
 namespace ppes {
@@ -65,6 +69,10 @@ public:
     // launch helpers over an explicit instance list (plans are built on demand)
     int launch_startup(const std::vector<int> &ids);             // drive.f90:66-73
     int launch_step(bool all_kernels);                           // one RK step of every active instance
+    // the degree groups' steps on streams of their own, forked from and
+    // joined back to the engine's stream (PPES_CONCURRENT, default on; off
+    // while kernels are timed)
+    bool concurrent = PPES_CONCURRENT != 0;
     int run_steps(long long nsteps, bool use_graph, long long *done_steps);
     int sync_every = 1;                                          // steps queued per host synchronisation
     int arith = PPES_ARITH_FAST;                                 // residual arithmetic (PPES_ARITH_*)
@@ -94,6 +102,11 @@ private:
     int build_group(Group &g, const std::vector<int> &members, int sid);
     int capture();
     void free_group(Group &g);
+    int launch_group_step(Group &g, bool any_retry, hipStream_t stream);  // one step of one group
+    int ensure_group_streams(size_t n);
+    std::vector<hipStream_t> gstreams;  // default priority, non-blocking
+    std::vector<hipEvent_t> gevents;
+    hipEvent_t fork_ev = nullptr;
     int *d_all = nullptr;
     int n_all = 0;
     std::vector<Group> groups;
