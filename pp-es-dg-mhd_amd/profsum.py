@@ -36,6 +36,22 @@ ALG_BYTES_PER_NODE = {"residual": 128.0, "stage": (192.0 + 256.0 + 256.0) / 3.0,
 LAUNCHES_PER_STEP = {"residual": 3, "stage": 3}
 
 
+def concurrent_launches(tr):
+    """indices of the launches of a kernel trace that ran beside another
+    kernel (overlap over 10% of the shorter of the two; back-to-back launches
+    of one stream touch at their ends)"""
+    spans = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), i) for i, r in enumerate(tr))
+    out = set()
+    reach, reach_0, reach_i = -1, -1, -1
+    for a0, a1, i in spans:
+        if a0 < reach and min(a1, reach) - a0 > 0.1 * min(a1 - a0, reach - reach_0):
+            out.add(i)
+            out.add(reach_i)
+        if a1 > reach:
+            reach, reach_0, reach_i = a1, a0, i
+    return out
+
+
 def one_csv(d, pattern):
     files = glob.glob(os.path.join(d, "**", pattern), recursive=True)
     if not files:
@@ -65,16 +81,23 @@ def stats(d, title, command):
     for k, (t, n) in cls.items():
         out.append(f"| {k} | {n} | {t / 1e6:.2f} | {t / n / 1e3:.1f} |")
     # full launches only: the largest grid of each kernel (one launch covers a
-    # whole degree group) and not an immediate exit (over 5% of the longest)
+    # whole degree group) and not an immediate exit (over 5% of the longest),
+    # and none that ran beside another kernel (the batched graph runs the
+    # degree groups on parallel streams; bench.py times its kernels in the
+    # sequential form)
     tr = list(csv.DictReader(open(one_csv(d, "*kernel_trace.csv"))))
+    overlapped = concurrent_launches(tr)
     per = collections.defaultdict(list)
-    for r in tr:
+    for i, r in enumerate(tr):
+        if i in overlapped:
+            continue
         name = r["Kernel_Name"].split("(")[0].replace("void ", "")
         dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
         per[name].append((int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]), dur))
-    out += ["", "Full launches (largest grid of each kernel, duration over 5% of the longest): the launches",
-            "bench.py times; the class average is the mean over the degree groups, as each group has",
-            "the same number of launches per step.", "",
+    out += ["", f"Full launches (largest grid of each kernel, duration over 5% of the longest, not overlapping",
+            f"another kernel: {len(overlapped)} of {len(tr)} launches ran concurrently and are left out): the",
+            "launches bench.py times; the class average is the mean over the degree groups, as each group",
+            "has the same number of launches per step.", "",
             "| kernel | full launches | avg (us) |", "|---|---|---|"]
     cls_full = collections.defaultdict(list)
     for name, v in sorted(per.items()):
@@ -149,10 +172,14 @@ def traffic(a):
 
 def full_durations(d):
     """average duration (ns) of the full launches of each kernel in a
-    --kernel-trace run (largest grid, over 5% of the longest)"""
+    --kernel-trace run (largest grid, over 5% of the longest, not
+    overlapping another kernel)"""
     tr = list(csv.DictReader(open(one_csv(d, "*kernel_trace.csv"))))
+    overlapped = concurrent_launches(tr)
     per = collections.defaultdict(list)
-    for r in tr:
+    for i, r in enumerate(tr):
+        if i in overlapped:
+            continue
         name = r["Kernel_Name"].split("(")[0].replace("void ", "")
         per[name].append((int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]),
                           int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
