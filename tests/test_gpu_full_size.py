@@ -1,0 +1,101 @@
+"""The benchmark configurations at their full sizes, where the oracle cannot
+follow, through the properties the method guarantees whatever the size:
+exact positivity of density and pressure at every GLL node, mass
+conservation to round-off on a periodic domain, and an instance giving the
+same bits alone and inside the 64-instance batch.
+
+(Point symmetry of the blast wave is not one of them: the reference
+algorithm itself breaks it, by 3.5e-4 rel. L1 in density after 100 steps on
+a 40 x 40 P2 grid of the CPU oracle.)
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ppesdg
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PP_EPS = 1e-11
+
+
+def mass(u, p, hx, hy):
+    t = ppesdg.tables(p)
+    w = np.outer(t["w"], t["w"])
+    return float(np.einsum("kl,klij->", w, u[0][:, :, 1:-1, 1:-1])) * hx * hy / 4.0
+
+
+def rho_prs_min(u, g):
+    q = u[:, :, :, 1:-1, 1:-1]
+    prs = (g - 1.0) * (q[7] - 0.5 * ((q[1] ** 2 + q[2] ** 2 + q[3] ** 2) / q[0] + (q[4] ** 2 + q[5] ** 2 + q[6] ** 2)))
+    return float(q[0].min()), float(prs.min())
+
+
+def test_ot_perturbed_p3_1024_mass_and_positivity():
+    """Orszag-Tang 2pi + seeded perturbation, P3, 1024 x 1024, 200 SSP-RK3
+    steps at CFL 0.2 (the single-instance benchmark): periodic, so the
+    quadrature mass is conserved to round-off."""
+    g, nx, p = 5.0 / 3.0, 1024, 3
+    h = 2 * math.pi / nx
+    s = ppesdg.Solver(nx, nx, p, (0.0, 2 * math.pi, 0.0, 2 * math.pi), g, "periodic")
+    s.init_problem("otpert", W.perturbation_table(0), 1e-3)
+    m0 = mass(s.download_state(0), p, h, h)
+    st = s.solve(-1.0, 200, 0.2, 1, True)
+    u = s.download_state(0)
+    s.close()
+    m1 = mass(u, p, h, h)
+    rmin, pmin = rho_prs_min(u, g)
+    print(f"OT 1024^2 P3: mass {m0!r} -> {m1!r} (rel {(m1 - m0) / m0:.2e}), min rho {rmin}, min p {pmin}")
+    assert st["status"] == 0 and st["steps"] == 200
+    assert abs(m1 - m0) <= 1e-13 * abs(m0)
+    assert rmin > 0.0 and pmin > 0.0
+
+
+def test_blast_p2_400_positivity_to_final_time():
+    """The positivity stress case: MHD blast wave (plasma beta ~2.5e-4), P2,
+    400 x 400, CFL 0.15, OEDG on every stage, to t = 0.01 (about 17 000
+    steps): no failure, and density and pressure positive at every node."""
+    g = 1.4
+    s = ppesdg.Solver(400, 400, 2, (-0.5, 0.5, -0.5, 0.5), g, "extrap")
+    s.init_problem("blast")
+    st = s.solve(0.01, 0, 0.15, 2, True)
+    u = s.download_state(0)
+    s.close()
+    rmin, pmin = rho_prs_min(u, g)
+    print(f"blast 400^2 P2: {st['steps']} steps, {st['retries']} retries, {st['negcells']} limited cells, "
+          f"min rho {rmin}, min p {pmin}")
+    assert st["status"] == 0 and abs(st["t"] - 0.01) <= 1e-15
+    assert rmin >= PP_EPS and pmin > 0.0
+
+
+def test_mixed_batch_full_size_alone_equals_in_batch():
+    """The 64 uneven instances of the batched benchmark, 20 steps together:
+    every node of every instance positive; the largest P4 instance and the
+    smallest instance give the same bits alone."""
+    mix = W.mixed_batch(64, seed=1)
+    b = ppesdg.Batch(3)
+    ids = []
+    for m in mix:
+        dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[m["prob"]]
+        i = b.add(m["nx"], m["ny"], m["p"], dom, g, bc, bcs)
+        b.init_problem(i, m["prob"])
+        b.start(i, -1.0, 20, cfl, oedg, True)
+        ids.append(i)
+    b.step(20, True)
+    for m, i in zip(mix, ids):
+        rmin, pmin = rho_prs_min(b.download_state(i, 0), W.PROBLEMS[m["prob"]][1])
+        assert rmin > 0.0 and pmin > 0.0, (m, rmin, pmin)
+    big = max(range(len(mix)), key=lambda q: (mix[q]["p"], W.nodes(mix[q])))
+    small = min(range(len(mix)), key=lambda q: W.nodes(mix[q]))
+    for q in (big, small):
+        m = mix[q]
+        dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[m["prob"]]
+        s = ppesdg.Solver(m["nx"], m["ny"], m["p"], dom, g, bc, bcs)
+        s.init_problem(m["prob"])
+        s.solve(-1.0, 20, cfl, oedg, True)
+        alone = s.download_state(0)[:, :, :, 1:-1, 1:-1]
+        s.close()
+        assert np.array_equal(alone, b.download_state(ids[q], 0)[:, :, :, 1:-1, 1:-1]), m
+    b.close()
