@@ -137,7 +137,11 @@ Engine::~Engine()
     for (auto &gs : gstreams) (void)hipStreamDestroy(gs);
     for (auto &ge : gevents) (void)hipEventDestroy(ge);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
-    for (auto &g : groups) free_group(g);
+    for (MapPool *ps : {plan_pools, start_pools})
+        for (int sid = 0; sid < 9; ++sid) {
+            for (Blk *p : ps[sid].d) if (p) (void)hipFree(p);
+            if (ps[sid].dl) (void)hipFree(ps[sid].dl);
+        }
     for (auto &g : api_groups) if (g.d_el) free_group(g);
     if (d_all) (void)hipFree(d_all);
     for (size_t i = 0; i < hosts.size(); ++i)
@@ -459,7 +463,7 @@ static void xcd_order(std::vector<Blk> &v)
     v.swap(out);
 }
 
-int Engine::build_group(Group &g, const std::vector<int> &members, int sid)
+int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool)
 {
     g.sid = sid;
     g.members = members;
@@ -487,23 +491,65 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid)
         rs.insert(rs.end(), rsf[f].begin(), rsf[f].end());
         g.res_off[f + 1] = (long long)rs.size();
     }
-    auto up = [&](Blk **d, const std::vector<Blk> &h) -> int {
-        ENG_TRY(hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(Blk)));
-        if (!h.empty()) ENG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(Blk), hipMemcpyHostToDevice));
-        return PPES_OK;
-    };
-    int rc;
-    if ((rc = up(&g.d_el, el)) || (rc = up(&g.d_slot, sl)) || (rc = up(&g.d_face, fc)) || (rc = up(&g.d_bc, bc)) ||
-        (rc = up(&g.d_red1, r1)) || (rc = up(&g.d_red2, r2)) || (rc = up(&g.d_res, rs)))
-        return rc;
-    ENG_TRY(hipMalloc((void **)&g.d_list, std::max<size_t>(members.size(), 1) * sizeof(int)));
-    ENG_TRY(hipMemcpy(g.d_list, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice));
     g.n_el = (long long)el.size();
     g.n_slot = (long long)sl.size();
     g.n_face = (long long)fc.size();
     g.n_bc = (long long)bc.size();
     g.n_red1 = (long long)r1.size();
     g.n_res = (long long)rs.size();
+    int rc;
+    if (pool) {
+        // pooled: grow a buffer only when it is too small (after the work
+        // that reads it is done), then upload on the engine's stream, ahead
+        // of the launches that will read it
+        auto put = [&](int kind, Blk **d, std::vector<Blk> &h) -> int {
+            pool->h[kind].swap(h);
+            const size_t need = std::max<size_t>(pool->h[kind].size(), 1);
+            if (need > pool->cap[kind]) {
+                if (pool->d[kind]) {
+                    ENG_TRY(hipStreamSynchronize(stream));
+                    ENG_TRY(hipFree(pool->d[kind]));
+                    pool->d[kind] = nullptr;
+                    pool->cap[kind] = 0;
+                }
+                const size_t c = std::max(need, 2 * pool->cap[kind]);
+                ENG_TRY(hipMalloc((void **)&pool->d[kind], c * sizeof(Blk)));
+                pool->cap[kind] = c;
+            }
+            if (!pool->h[kind].empty())
+                ENG_TRY(hipMemcpyAsync(pool->d[kind], pool->h[kind].data(), pool->h[kind].size() * sizeof(Blk),
+                                       hipMemcpyHostToDevice, stream));
+            *d = pool->d[kind];
+            return PPES_OK;
+        };
+        if ((rc = put(0, &g.d_el, el)) || (rc = put(1, &g.d_slot, sl)) || (rc = put(2, &g.d_face, fc)) ||
+            (rc = put(3, &g.d_bc, bc)) || (rc = put(4, &g.d_red1, r1)) || (rc = put(5, &g.d_red2, r2)) ||
+            (rc = put(6, &g.d_res, rs)))
+            return rc;
+        pool->hl = members;
+        if (pool->hl.size() > pool->lcap) {
+            if (pool->dl) {
+                ENG_TRY(hipStreamSynchronize(stream));
+                ENG_TRY(hipFree(pool->dl));
+                pool->dl = nullptr;
+            }
+            pool->lcap = std::max(pool->hl.size(), 2 * pool->lcap);
+            ENG_TRY(hipMalloc((void **)&pool->dl, pool->lcap * sizeof(int)));
+        }
+        ENG_TRY(hipMemcpyAsync(pool->dl, pool->hl.data(), pool->hl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        g.d_list = pool->dl;
+        return PPES_OK;
+    }
+    auto up = [&](Blk **d, const std::vector<Blk> &h) -> int {
+        ENG_TRY(hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(Blk)));
+        if (!h.empty()) ENG_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(Blk), hipMemcpyHostToDevice));
+        return PPES_OK;
+    };
+    if ((rc = up(&g.d_el, el)) || (rc = up(&g.d_slot, sl)) || (rc = up(&g.d_face, fc)) || (rc = up(&g.d_bc, bc)) ||
+        (rc = up(&g.d_red1, r1)) || (rc = up(&g.d_red2, r2)) || (rc = up(&g.d_res, rs)))
+        return rc;
+    ENG_TRY(hipMalloc((void **)&g.d_list, std::max<size_t>(members.size(), 1) * sizeof(int)));
+    ENG_TRY(hipMemcpy(g.d_list, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice));
     return PPES_OK;
 }
 
@@ -532,9 +578,8 @@ int Engine::build_plan()
         return PPES_OK;
     }
     graph_ok = false;
-    for (auto &g : groups) free_group(g);
-    groups.clear();
-    if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
+    plan_steps = 0;
+    groups.clear();  // their maps live in plan_pools
     std::vector<int> all;
     for (int s = 0; s <= MAXS; ++s) any_oedg_stage[s] = false;
     for (int sid = 1; sid <= 8; ++sid) {
@@ -543,7 +588,7 @@ int Engine::build_plan()
             if (hosts[i] && hosts[i]->sid == sid && h_recs[i].ctl.state == ST_ACTIVE) mem.push_back((int)i);
         if (mem.empty()) continue;
         Group g;
-        int rc = build_group(g, mem, sid);
+        int rc = build_group(g, mem, sid, &plan_pools[sid]);
         if (rc) return rc;
         groups.push_back(g);
         for (int id : mem) {
@@ -554,8 +599,17 @@ int Engine::build_plan()
         }
     }
     n_all = (int)all.size();
-    ENG_TRY(hipMalloc((void **)&d_all, std::max<size_t>(all.size(), 1) * sizeof(int)));
-    if (!all.empty()) ENG_TRY(hipMemcpy(d_all, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+    h_all.swap(all);
+    if (h_all.size() > all_cap) {
+        if (d_all) {
+            ENG_TRY(hipStreamSynchronize(stream));
+            ENG_TRY(hipFree(d_all));
+            d_all = nullptr;
+        }
+        all_cap = std::max(h_all.size(), 2 * all_cap);
+        ENG_TRY(hipMalloc((void **)&d_all, all_cap * sizeof(int)));
+    }
+    ENG_TRY(hipMemcpyAsync(d_all, h_all.data(), h_all.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     built_key = key;
     built_n_all = n_all;
     plan_ok = true;
@@ -750,15 +804,20 @@ int Engine::run_steps(long long nsteps, bool use_graph, long long *done_steps)
     long long k = 0;
     while (nsteps <= 0 || k < nsteps) {
         if (active_count() == 0) break;
-        if (use_graph && !graph_ok && (rc = capture())) return rc;
+        // the graph is captured once a plan has run a step eagerly: a plan
+        // that lasts one step (a membership that changes every step) is
+        // never captured
+        const bool graph_now = use_graph && (graph_ok || plan_steps >= 1);
+        if (graph_now && !graph_ok && (rc = capture())) return rc;
         long long chunk = std::max(1, sync_every);
         if (nsteps > 0) chunk = std::min(chunk, nsteps - k);
         for (long long c = 0; c < chunk; ++c) {
-            if (use_graph) {
+            if (graph_now) {
                 ENG_TRY(hipGraphLaunch(graph_exec, stream));
             } else if ((rc = launch_step(chunk > 1))) {
                 return rc;
             }
+            ++plan_steps;
         }
         if ((rc = pull())) return rc;
         k += chunk;
@@ -792,7 +851,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
     for (int sid = 1; sid <= 8 && rc == PPES_OK; ++sid) {
         if (by_sid[sid].empty()) continue;
         Group g;
-        rc = build_group(g, by_sid[sid], sid);
+        rc = build_group(g, by_sid[sid], sid, &start_pools[sid]);
         tmp.push_back(g);
     }
     if (rc == PPES_OK) {
@@ -807,8 +866,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
         LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
         for (auto &g : tmp) RESIDUAL_DISPATCH(g, 0, 1, 0);
     }
-    (void)hipStreamSynchronize(stream);
-    for (auto &g : tmp) free_group(g);
+    (void)hipStreamSynchronize(stream);  // (tmp's maps live in start_pools)
     (void)hipFree(d_ids);
     return rc;
 }

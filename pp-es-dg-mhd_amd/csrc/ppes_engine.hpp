@@ -43,6 +43,19 @@ struct Group {
     bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
 };
 
+// Device buffers of one degree's block maps, kept across plan rebuilds and
+// grown when needed (a batch whose membership changes every step then
+// allocates nothing); the host copies stay alive while an asynchronous upload
+// may read them.
+struct MapPool {
+    Blk *d[7] = {};
+    size_t cap[7] = {};
+    std::vector<Blk> h[7];
+    int *dl = nullptr;
+    size_t lcap = 0;
+    std::vector<int> hl;
+};
+
 enum KTimerId { KT_RESIDUAL = 0, KT_STAGE = 1, KT_OEDG = 2, KT_LIMIT = 3, KT_REDUCE = 4, KT_BC = 5, KT_COUNT = 6 };
 
 struct KTimer {
@@ -99,7 +112,11 @@ public:
 private:
     int ensure_capacity(int n);
     int build_plan();
-    int build_group(Group &g, const std::vector<int> &members, int sid);
+    int build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool = nullptr);
+    MapPool plan_pools[9], start_pools[9];  // by shape id: the step plan's maps, the start-up's
+    size_t all_cap = 0;
+    std::vector<int> h_all;
+    long long plan_steps = 0;  // steps launched with the current plan (the graph is captured from the second)
     int capture();
     void free_group(Group &g);
     int launch_group_step(Group &g, bool any_retry, hipStream_t stream);  // one step of one group
