@@ -194,16 +194,18 @@ def cpu_baseline(mix, budget_s=15.0):
                       f"start-up + 6 RK3 steps each, {spent:.1f} s"}
 
 
-def run_serving(duration, rate):
+def run_serving(duration, rate, sides):
     """Scaled serving stream: Poisson arrivals (seed 2) of requests from the
-    mixed workload's problems with sides {32, 64, 128}, P1-P4 and t_final a
+    mixed workload's problems with the given grid sides, P1-P4 and t_final a
     uniform fraction in [0.1, 1] of a per-problem horizon (OT 0.05, rotor
     0.01, blast 0.001); continuous batching under a 200 GB budget against
     the plain path serving the same stream one request at a time (first come
-    first served). Throughput is work over the makespan (first arrival to
-    last result) for both."""
+    first served). The default stream (sides 32-256) brings more work than
+    the batched server clears in its arrival window, so both run saturated.
+    Throughput is work over the makespan (first arrival to last result) for
+    both."""
     import serve
-    reqs = W.serving_stream(rate=rate, duration=duration, seed=2, sides=(32, 64, 128),
+    reqs = W.serving_stream(rate=rate, duration=duration, seed=2, sides=sides,
                             horizon={"ot2pi": 0.05, "rotor": 0.01, "blast": 0.001})
     t_first = reqs[0]["arrival"]
     done, wall, rejected = serve.run_stream(reqs, 200e9, use_graph=True)
@@ -215,8 +217,8 @@ def run_serving(duration, rate):
     span_seq = max(r["t_done"] for r in seq) - t_first
     work_seq = sum(serve.work(r) for r in seq)
     out = {"requests": len(reqs), "finished": len(done), "rejected": len(rejected),
-           "stream": f"Poisson {rate}/s for {duration} s (seed 2), sides 32-128, P1-P4, t_final in [0.1, 1] x "
-                     f"(OT 0.05, rotor 0.01, blast 0.001)",
+           "stream": f"Poisson {rate}/s for {duration} s (seed 2), sides {'/'.join(map(str, sides))}, P1-P4, "
+                     f"t_final in [0.1, 1] x (OT 0.05, rotor 0.01, blast 0.001)",
            "batched": {"value": total_work / span, "makespan_s": span, **serve.percentiles(lat)},
            "plain_sequential": {"value": work_seq / span_seq, "makespan_s": span_seq, "busy_s": busy,
                                 **serve.percentiles(lat_seq)},
@@ -248,6 +250,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-duration", type=float, default=5.0)
     ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-sides", default="32,64,128,256")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -335,7 +338,7 @@ def main():
         out["single_instance"] = run_single(ppesdg, a.single_steps, 2)
         print("# single instance done", file=sys.stderr, flush=True)
     if not a.no_serve:
-        out["serving"] = run_serving(a.serve_duration, a.serve_rate)
+        out["serving"] = run_serving(a.serve_duration, a.serve_rate, tuple(int(x) for x in a.serve_sides.split(",")))
         print("# serving done", file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mix)
