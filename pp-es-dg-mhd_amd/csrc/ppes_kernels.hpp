@@ -127,6 +127,11 @@ constexpr int resid_waves(int p, bool fast) { return !fast ? 1 : (p == 4 ? PPES_
 #ifndef PPES_FAST_ROUND
 #define PPES_FAST_ROUND 1
 #endif
+// the stage kernel's LDF product on FP64 MFMA (1) or the VALU (0)
+#ifndef PPES_LDF_MFMA
+#define PPES_LDF_MFMA 1
+#endif
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <int PX, int PY, bool FAST, int TB = resid_tb((PX + 1) * (PY + 1)), int FLUX = FLUX_ESPP_HLL>
 __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_waves(PX == PY ? PX : 0, FAST)))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
@@ -741,7 +746,7 @@ __device__ __forceinline__ double ghost_value(int mode, int v, double x)
 // or read from memory (F_GHOST_MEM).
 
 template <int PX, int PY, bool EO>
-__global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
 {
     using S = Shape<PX, PY>;
     constexpr int NX = S::NX, NY = S::NY, NN = S::NN, EPB = S::EPB;
@@ -756,9 +761,9 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     double(*sdb)[M] = reinterpret_cast<double(*)[M]>(sunion);
     double(*sred)[2] = reinterpret_cast<double(*)[2]>(sunion);
     __shared__ double savg[EPB][8];
-    // the LDF operator as sldf[c][n] = (A[c][n], A[c][NN + n]): the two
-    // entries a node's (Bx, By) outputs take from input c, read together
-    __shared__ double sldf[M][NN][2];
+    // the LDF operator as stored: sldf[c * M + o] takes input c to output o
+    // (o < NN: Bx of node o; otherwise By of node o - NN)
+    __shared__ double sldf[M * M];
     __shared__ int sok[EPB];
     __shared__ int scnt[EPB];
     __shared__ double sent[EPB];
@@ -792,17 +797,14 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     // stored each entry before loading the next would wait on memory
     // M^2 / BLOCK times in a row)
     const bool stage_ldf = use_ldf && !(flags & F_COMBINE_ONLY);
-    constexpr int R = (M * NN + BLOCK - 1) / BLOCK;
-    double ax[R], ay[R];
+    constexpr int R = (M * M + BLOCK - 1) / BLOCK;
+    double ax[R];
     if (stage_ldf) {
         const gdbl *ldf = gin(I.ldf);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int a = t + r * BLOCK, c = a / NN, m = a - c * NN;
-            if (a < M * NN) {
-                ax[r] = ldf[c * M + m];
-                ay[r] = ldf[c * M + NN + m];
-            }
+            const int a = t + r * BLOCK;
+            if (a < M * M) ax[r] = ldf[a];
         }
     }
 
@@ -841,11 +843,8 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
     if (stage_ldf) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int a = t + r * BLOCK, c = a / NN, m = a - c * NN;
-            if (a < M * NN) {
-                sldf[c][m][0] = ax[r];
-                sldf[c][m][1] = ay[r];
-            }
+            const int a = t + r * BLOCK;
+            if (a < M * M) sldf[a] = ax[r];
         }
     }
     __syncthreads();
@@ -883,18 +882,69 @@ __global__ void __launch_bounds__(BLOCK) k_stage_pp(Inst *insts, const Blk *map,
             sdb[el][NN + n] = q[5] - avg[5];
         }
         __syncthreads();
+#if PPES_LDF_MFMA
+        // Y (M x EPB) = Op (M x M) X (M x EPB) on FP64 MFMA 16x16x4 tiles,
+        // with X[c][e] = sdb[e][c] and Op[o][c] = sldf[c * M + o]. Lane l of a
+        // wave feeds Op[16 mt + l % 16][4 ks + l / 16] and
+        // X[4 ks + l / 16][16 et + l % 16]; register r of its result holds
+        // Y[16 mt + 4 r + l / 16][16 et + l % 16]. The (mt, et) tiles go round
+        // the waves; rows, columns and inputs past M and EPB are zero.
+        {
+            constexpr int MT = (M + 15) / 16, ET = (EPB + 15) / 16, KT = (M + 3) / 4;
+            constexpr int JOBS = MT * ET, WAVES = BLOCK / 64, JPW = (JOBS + WAVES - 1) / WAVES;
+            const int lane = t & 63, wv = t >> 6, lo = lane & 15, kq = lane >> 4;
+            f64x4 yacc[JPW];
+#pragma unroll
+            for (int jj = 0; jj < JPW; ++jj) {
+                yacc[jj] = f64x4{0.0, 0.0, 0.0, 0.0};
+                const int job = wv + jj * WAVES;  // uniform over the wave
+                if (job < JOBS) {
+                    const int mt = job / ET, et = job - mt * ET;
+                    const int o = 16 * mt + lo, e = 16 * et + lo;
+#pragma unroll
+                    for (int ks = 0; ks < KT; ++ks) {
+                        const int c = 4 * ks + kq;
+                        const double a = (o < M && c < M) ? sldf[c * M + o] : 0.0;
+                        const double b = (e < EPB && c < M) ? sdb[e < EPB ? e : 0][c < M ? c : 0] : 0.0;
+                        yacc[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, yacc[jj], 0, 0, 0);
+                    }
+                }
+            }
+            __syncthreads();  // every wave has read sdb: its region takes the products
+            double(*sy)[EPB] = reinterpret_cast<double(*)[EPB]>(sunion);
+#pragma unroll
+            for (int jj = 0; jj < JPW; ++jj) {
+                const int job = wv + jj * WAVES;
+                if (job < JOBS) {
+                    const int mt = job / ET, et = job - mt * ET;
+                    const int e = 16 * et + lo;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int o = 16 * mt + 4 * r + kq;
+                        if (o < M && e < EPB) sy[o][e] = yacc[jj][r];
+                    }
+                }
+            }
+            __syncthreads();
+            if (valid && ok) {
+                q[4] = sy[n][el] + avg[4];
+                q[5] = sy[NN + n][el] + avg[5];
+            }
+        }
+#else
         if (valid && ok) {
             double bxs = 0.0, bys = 0.0;
 #pragma unroll 4
             for (int c = 0; c < M; ++c) {
                 const double d = sdb[el][c];
-                bxs = bxs + sldf[c][n][0] * d;
-                bys = bys + sldf[c][n][1] * d;
+                bxs = bxs + sldf[c * M + n] * d;
+                bys = bys + sldf[c * M + NN + n] * d;
             }
             q[4] = bxs + avg[4];
             q[5] = bys + avg[5];
         }
-        __syncthreads();  // sdb is read; the region is reused for sred / scons
+#endif
+        __syncthreads();  // sdb / the products are read; the region is reused for sred / scons
     }
     if (valid && ok)
         for (int v = n; v < 8; v += NN) gout(I.uavg)[eg * 8 + v] = savg[el][v];
