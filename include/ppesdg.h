@@ -230,6 +230,19 @@ int ppes_batch_set_timing(ppes_batch *b, int on);
 int ppes_batch_set_arith(ppes_batch *b, int mode);
 int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long long *count);
 
+/* Residual autotuner (no reference counterpart). For each degree among the
+ * active instances whose members all use the fast ES-HLL residual (the
+ * default), times DG_semidiscretize of the current states with workgroups of
+ * 64, 128 and 256 lanes (64 / N^2 to 256 / N^2 elements each, N^2 nodes per
+ * element) over `reps` launches and keeps the fastest size for that degree
+ * in this batch / solver; tb_out[sid - 1] receives it per shape id (1-4:
+ * P1-P4 x same degree in y, 5-8: P1-P4 x P0), 0 where nothing was tuned.
+ * Recomputing L(u^n) into its own slot leaves every state as it was, and
+ * results do not depend on the size. Call between steps (for a solver: after
+ * ppes_solve_begin). */
+int ppes_batch_autotune(ppes_batch *b, int reps, int *tb_out);
+int ppes_autotune(ppes_solver *s, int reps, int *tb_out);
+
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
  * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major),
  * N = p + 1, p in 0..6 (p = 0: the one-node rule x = 0, w = 2). */

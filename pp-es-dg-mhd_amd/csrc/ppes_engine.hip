@@ -29,10 +29,22 @@ namespace ppes {
             ENG_TRY(hipGetLastError());                                                        \
         }                                                                                      \
     } while (0)
-#define RESIDUAL_DISPATCH_F(sid, FK, grid, ...)                                                                  \
+// tb: the workgroup size the group's residual map was built for; the fast
+// ES-HLL residual exists for every autotuner candidate, every other variant
+// for the default size only (build_group keeps to that)
+#define RESIDUAL_DISPATCH_F(sid, FK, tb, grid, ...)                                                              \
     do {                                                                                                         \
-        if (arith == PPES_ARITH_FAST) { ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); } \
-        else { ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, false, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); } \
+        if (arith == PPES_ARITH_FAST && (FK) == FLUX_ESPP_HLL && (tb) == 64) {                                   \
+            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, 64, FLUX_ESPP_HLL>), grid, 64, __VA_ARGS__));   \
+        } else if (arith == PPES_ARITH_FAST && (FK) == FLUX_ESPP_HLL && (tb) == 128) {                           \
+            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, 128, FLUX_ESPP_HLL>), grid, 128, __VA_ARGS__)); \
+        } else if (arith == PPES_ARITH_FAST && (FK) == FLUX_ESPP_HLL && (tb) == 256) {                           \
+            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, 256, FLUX_ESPP_HLL>), grid, 256, __VA_ARGS__)); \
+        } else if (arith == PPES_ARITH_FAST) {                                                                   \
+            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); \
+        } else {                                                                                                 \
+            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, false, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); \
+        }                                                                                                        \
     } while (0)
 // the residual over a group: one launch per face flux kind present
 #define RESIDUAL_DISPATCH(g, ...)                                                                                \
@@ -42,10 +54,10 @@ namespace ppes {
             if (n_ <= 0) continue;                                                                               \
             const Blk *m_ = (g).d_res + (g).res_off[fk_];                                                        \
             switch (fk_) {                                                                                       \
-            case FLUX_PP_HLL: RESIDUAL_DISPATCH_F((g).sid, FLUX_PP_HLL, n_, d_recs, m_, __VA_ARGS__); break;      \
-            case FLUX_ESPP_LF: RESIDUAL_DISPATCH_F((g).sid, FLUX_ESPP_LF, n_, d_recs, m_, __VA_ARGS__); break;    \
-            case FLUX_EC_LF: RESIDUAL_DISPATCH_F((g).sid, FLUX_EC_LF, n_, d_recs, m_, __VA_ARGS__); break;        \
-            default: RESIDUAL_DISPATCH_F((g).sid, FLUX_ESPP_HLL, n_, d_recs, m_, __VA_ARGS__); break;             \
+            case FLUX_PP_HLL: RESIDUAL_DISPATCH_F((g).sid, FLUX_PP_HLL, (g).res_tb, n_, d_recs, m_, __VA_ARGS__); break;      \
+            case FLUX_ESPP_LF: RESIDUAL_DISPATCH_F((g).sid, FLUX_ESPP_LF, (g).res_tb, n_, d_recs, m_, __VA_ARGS__); break;    \
+            case FLUX_EC_LF: RESIDUAL_DISPATCH_F((g).sid, FLUX_EC_LF, (g).res_tb, n_, d_recs, m_, __VA_ARGS__); break;        \
+            default: RESIDUAL_DISPATCH_F((g).sid, FLUX_ESPP_HLL, (g).res_tb, n_, d_recs, m_, __VA_ARGS__); break;             \
             }                                                                                                    \
         }                                                                                                        \
     } while (0)
@@ -469,12 +481,13 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapP
     g.members = members;
     g.eo = false;
     for (int id : members) g.eo = g.eo || hosts[id]->ent_offset;
+    g.res_tb = residual_tb(sid, members);
     std::vector<Blk> el, sl, fc, bc, r1, r2, rs, rsf[4];
     for (int id : members) {
         const Inst &r = h_recs[id];
         const int NN = (sid_px(sid) + 1) * (sid_py(sid) + 1);
         for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
-        const long long nres = cdiv((long long)r.nx * r.ny, resid_tb(NN) / NN);
+        const long long nres = cdiv((long long)r.nx * r.ny, (long long)(g.res_tb / NN));
         for (long long b = 0; b < nres; ++b) rsf[hosts[id]->flux].push_back({id, (int)b});
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
         const long long nf = ((long long)(r.nx + 1) * r.ny + (long long)r.nx * (r.ny + 1)) * 8;
@@ -769,6 +782,80 @@ int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
     return PPES_OK;
 }
 
+int Engine::residual_tb(int sid, const std::vector<int> &members) const
+{
+    const int def = resid_tb((sid_px(sid) + 1) * (sid_py(sid) + 1));
+    if (arith != PPES_ARITH_FAST || tuned_tb[sid] == 0) return def;
+    for (int id : members)
+        if (hosts[id]->flux != FLUX_ESPP_HLL) return def;
+    return tuned_tb[sid];
+}
+
+int Engine::autotune(int reps, int *tb_out)
+{
+    if (tb_out)
+        for (int s2 = 0; s2 < 8; ++s2) tb_out[s2] = 0;
+    int rc = build_plan();
+    if (rc) return rc;
+    if (arith != PPES_ARITH_FAST || n_all == 0) return PPES_OK;
+    reps = std::max(1, reps);
+    ENG_TRY(hipStreamSynchronize(stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ENG_TRY(hipEventCreate(&e0));
+    ENG_TRY(hipEventCreate(&e1));
+    int err = PPES_OK;
+    for (const Group &g : groups) {
+        bool fast_hll = true;
+        for (int id : g.members) fast_hll = fast_hll && hosts[id]->flux == FLUX_ESPP_HLL;
+        if (!fast_hll) continue;
+        const int NN = (sid_px(g.sid) + 1) * (sid_py(g.sid) + 1);
+        int best = 0;
+        float best_ms = 0.0f;
+        for (int tb : {64, 128, 256}) {
+            if (tb < NN) continue;
+            std::vector<Blk> rs;
+            for (int id : g.members) {
+                const Inst &r = h_recs[id];
+                const long long nres = cdiv((long long)r.nx * r.ny, (long long)(tb / NN));
+                for (long long b2 = 0; b2 < nres; ++b2) rs.push_back({id, (int)b2});
+            }
+            xcd_order(rs);
+            Blk *d = nullptr;
+            if (hipMalloc((void **)&d, rs.size() * sizeof(Blk)) != hipSuccess) { err = PPES_OOM; break; }
+            Group t = g;
+            t.res_tb = tb;
+            t.d_res = d;
+            t.res_off[0] = 0;
+            for (int f = 1; f <= 4; ++f) t.res_off[f] = (long long)rs.size();
+            float ms = 0.0f;
+            auto run = [&]() -> int {
+                ENG_TRY(hipMemcpy(d, rs.data(), rs.size() * sizeof(Blk), hipMemcpyHostToDevice));
+                RESIDUAL_DISPATCH(t, 0, 1, 0);  // warm-up
+                ENG_TRY(hipEventRecord(e0, stream));
+                for (int r = 0; r < reps; ++r) RESIDUAL_DISPATCH(t, 0, 1, 0);
+                ENG_TRY(hipEventRecord(e1, stream));
+                ENG_TRY(hipEventSynchronize(e1));
+                ENG_TRY(hipEventElapsedTime(&ms, e0, e1));
+                return PPES_OK;
+            };
+            err = run();
+            (void)hipFree(d);
+            if (err) break;
+            if (best == 0 || ms < best_ms) { best = tb; best_ms = ms; }
+        }
+        if (err) break;
+        tuned_tb[g.sid] = best;
+        if (tb_out) tb_out[g.sid - 1] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the next launch rebuilds the plan with the chosen sizes
+    built_key.clear();
+    plan_ok = false;
+    graph_ok = false;
+    return err;
+}
+
 // capture one step of the current plan as a hipGraph
 int Engine::capture()
 {
@@ -889,6 +976,9 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
 {
     if ((int)api_groups.size() < (int)hosts.size()) api_groups.resize(hosts.size());
     Group &g = api_groups[id];
+    // (rebuilt when the residual size its map was made for is no longer the
+    // one the instance's mode and flux select)
+    if (g.d_el && g.res_tb != residual_tb(hosts[id]->sid, std::vector<int>{id})) free_group(g);
     if (!g.d_el) {
         int rc = build_group(g, std::vector<int>{id}, hosts[id]->sid);
         if (rc) { free_group(g); return rc; }

@@ -39,6 +39,7 @@ struct Group {
     Blk *d_res = nullptr;  // element map of the residual kernel (its own workgroup size), by face flux kind
     long long res_off[5] = {0, 0, 0, 0, 0};  // blocks of flux kind f: [res_off[f], res_off[f + 1])
     int *d_list = nullptr;
+    int res_tb = 0;  // workgroup size of the residual launches (the residual map's elements per block follow)
     long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
     bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
 };
@@ -90,7 +91,22 @@ public:
     int sync_every = 1;                                          // steps queued per host synchronisation
     int arith = PPES_ARITH_FAST;                                 // residual arithmetic (PPES_ARITH_*)
     long long stage_blocks(int, long long nmap) const { return nmap; }  // one element group per workgroup
-    void set_arith(int a) { arith = a; graph_ok = false; }
+    void set_arith(int a)
+    {
+        arith = a;
+        graph_ok = false;
+        built_key.clear();  // the residual maps depend on the mode (tuned sizes are fast-mode only)
+        plan_ok = false;
+    }
+    // Residual autotuner: for each degree group of the current plan whose
+    // members all use the fast ES-HLL residual, time the residual of the
+    // current states at every candidate workgroup size (64, 128, 256 lanes,
+    // so 64/N^2 to 256/N^2 elements per workgroup) and keep the fastest for
+    // that shape; tb_out[sid - 1] receives it (0: not tuned). Recomputing
+    // L(u^n) into its own slot leaves the states and dt_lim as they were.
+    int autotune(int reps, int *tb_out);
+    int tuned_tb[9] = {};  // by shape id; 0: the default size
+    int residual_tb(int sid, const std::vector<int> &members) const;
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
     int set_flux(int id, int kind);                                      // face flux of one instance (FluxKind)
 

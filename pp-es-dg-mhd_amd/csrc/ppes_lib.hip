@@ -909,3 +909,31 @@ extern "C" int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_m
 {
     return b ? engine_kernel_time(b->eng, kernel, total_ms, count) : PPES_BAD_ARG;
 }
+
+extern "C" int ppes_batch_autotune(ppes_batch *b, int reps, int *tb_out)
+{
+    if (!b) return PPES_BAD_ARG;
+    return b->eng.autotune(reps, tb_out);
+}
+
+extern "C" int ppes_autotune(ppes_solver *s, int reps, int *tb_out)
+{
+    if (!s || !s->begun) return PPES_BAD_ARG;
+    // the instance is active only inside ppes_advance: activate it for the
+    // timing, then put its state back
+    Engine &e = s->eng;
+    int rc = e.pull();
+    if (rc) return rc;
+    Ctl &c = e.rec(s->id).ctl;
+    if (c.state == ST_FAILED) return PPES_BAD_ARG;
+    const int saved = c.state;
+    c.state = ST_ACTIVE;
+    if ((rc = e.push())) return rc;
+    e.invalidate_plan();
+    rc = e.autotune(reps, tb_out);
+    int rc2 = e.pull();
+    e.rec(s->id).ctl.state = saved;
+    if (!rc2) rc2 = e.push();
+    e.invalidate_plan();
+    return rc ? rc : rc2;
+}

@@ -92,6 +92,8 @@ def lib():
             "ppes_set_timing": [vp, ctypes.c_int],
             "ppes_set_arith": [vp, ctypes.c_int],
             "ppes_batch_set_arith": [vp, ctypes.c_int],
+            "ppes_autotune": [vp, ctypes.c_int, ip],
+            "ppes_batch_autotune": [vp, ctypes.c_int, ip],
             "ppes_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
             "ppes_batch_create": [ctypes.c_int, ctypes.POINTER(vp)],
             "ppes_batch_destroy": [vp],
@@ -168,6 +170,18 @@ def instance_bytes(nx, ny, p, rk=3, py=None):
 
 def _stats(st):
     return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
+
+
+def _tuned(call, name):
+    out = (ctypes.c_int * 8)()
+    _check(call(out), name)
+    res = {}
+    for sid in range(1, 9):
+        if out[sid - 1]:
+            px = (sid - 1) % 4 + 1
+            py = px if sid <= 4 else 0
+            res[f"P{px}x{py}"] = int(out[sid - 1])
+    return res
 
 
 class Solver:
@@ -291,6 +305,11 @@ class Solver:
         """"exact" (the reference's rounding order) or "fast" (default)"""
         _check(lib().ppes_set_arith(self.h, ARITH[mode]), "ppes_set_arith")
 
+    def autotune(self, reps=5):
+        """Residual workgroup size by timing (after solve_begin); returns
+        {"P<px>x<py>": lanes} for the tuned shape."""
+        return _tuned(lambda out: lib().ppes_autotune(self.h, int(reps), out), "ppes_autotune")
+
     def set_flux(self, kind):
         """face flux: espp_hll (default), pp_hll, espp_lf or ec_lf (ESPP_HLL_flux_x,
         PP_HLL_flux_x, ESPP_LF_flux_x, EC_LF_flux_x of DG_solver.f90:894-1497)"""
@@ -384,6 +403,11 @@ class Batch:
 
     def set_arith(self, mode):
         _check(lib().ppes_batch_set_arith(self.h, ARITH[mode]), "ppes_batch_set_arith")
+
+    def autotune(self, reps=5):
+        """Residual workgroup size per degree among the active instances, by
+        timing; returns {"P<px>x<py>": lanes} for the tuned shapes."""
+        return _tuned(lambda out: lib().ppes_batch_autotune(self.h, int(reps), out), "ppes_batch_autotune")
 
     def set_flux(self, i, kind):
         """face flux of instance i (see Solver.set_flux)"""
