@@ -745,6 +745,8 @@ __device__ __forceinline__ double ghost_value(int mode, int v, double x)
 // exactly what BC_handle would copy (problems.f90:312-348,386-423,597-654),
 // or read from memory (F_GHOST_MEM).
 
+// four waves per SIMD asked of the register allocator: the P4 variant's MFMA
+// accumulators then stay in VGPRs (130 registers with AGPRs would allow three)
 template <int PX, int PY, bool EO>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
 {
