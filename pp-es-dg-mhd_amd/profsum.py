@@ -14,6 +14,8 @@
         --pmc run (wave instructions x 64 lanes, an FMA two operations) and
         from SQ_INSTS_VALU_FLOPS_FP64, and algorithmic (bench.py's counts per
         node), with the rates over the launch durations of a --kernel-trace run
+    python pp-es-dg-mhd_amd/profsum.py counters --pmc D [--title T] [--command C]
+        per-launch averages of every counter of a --pmc run, per kernel
 
 Launches in which every workgroup exits at once (the retry residual of a step
 with no retry) are left out: their counters are under 2% of the largest
@@ -99,7 +101,9 @@ def stats(d, title, command):
             "launches bench.py times; the class average is the mean over the degree groups, as each group",
             "has the same number of launches per step.", "",
             "| kernel | full launches | avg (us) |", "|---|---|---|"]
-    cls_full = collections.defaultdict(list)
+    # per (class, degree): the variant with the most full launches is the one
+    # the steps run (the others are the autotuner's trial sizes)
+    best = {}
     for name, v in sorted(per.items()):
         base = name.split("<")[0].replace("ppes::", "")
         if base not in CLASS_OF:
@@ -108,8 +112,13 @@ def stats(d, title, command):
         dmax = max(x[1] for x in v if x[0] == g)
         full = [x[1] for x in v if x[0] == g and x[1] > 0.05 * dmax]
         avg = sum(full) / len(full)
-        cls_full[CLASS_OF[base]].append(avg)
         out.append(f"| {name} | {len(full)} | {avg / 1e3:.1f} |")
+        key = (CLASS_OF[base], name.split("<")[1].split(",")[0].rstrip(">") if "<" in name else "")
+        if key not in best or len(full) > best[key][0]:
+            best[key] = (len(full), avg)
+    cls_full = collections.defaultdict(list)
+    for (cls, _), (_, avg) in sorted(best.items()):
+        cls_full[cls].append(avg)
     out += ["", "| class | avg per full launch over degree groups (us) |", "|---|---|"]
     for k, v in cls_full.items():
         out.append(f"| {k} | {sum(v) / len(v) / 1e3:.1f} |")
@@ -145,7 +154,8 @@ def traffic(a):
         keys = [k for k in F if k[0] == f"ppes::{kname}<{p}>" or k[0].startswith(f"ppes::{kname}<{p},")]
         if not keys:
             continue
-        name = keys[0][0]
+        # the variant the steps run: the most dispatches (the others are autotuner trials)
+        name = max({k[0] for k in keys}, key=lambda n: sum(len(F[k]) for k in keys if k[0] == n))
         g = max(k[1] for k in keys if k[0] == name)
         f, w = F[(name, g)], Wr[(name, g)]
         fmax, wmax = max(f), max(w)
@@ -215,7 +225,8 @@ def fp64(a):
         names = [n for n in grid if n.startswith(f"ppes::{kname}<{p}, {p},")]
         if not names:
             continue
-        name = names[0]
+        # the variant the steps run: the most dispatches (the others are autotuner trials)
+        name = max(names, key=lambda n: sum(1 for (m, _, _) in per if m == n))
         disp = [c for (n, g, _), c in per.items() if n == name and g == grid[name]]
         vmax = max(c["SQ_INSTS_VALU"] for c in disp)
         disp = [c for c in disp if c["SQ_INSTS_VALU"] > 0.02 * vmax]
@@ -251,6 +262,36 @@ def fp64(a):
     return "\n".join(out) + "\n"
 
 
+def counters(a):
+    """average of every counter over the full launches of each kernel class
+    and degree in a --pmc run (largest grid, over 2% of the largest
+    SQ_INSTS_VALU or first counter)"""
+    rows = list(csv.DictReader(open(one_csv(a.pmc, "*counter_collection.csv"))))
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    grid = {}
+    for r in rows:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        base = name.split("<")[0].replace("ppes::", "")
+        if base not in CLASS_OF:
+            continue
+        per[(name, int(r["Grid_Size"]), r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
+        grid[name] = max(grid.get(name, 0), int(r["Grid_Size"]))
+    names = sorted(grid)
+    cols = sorted({c for v in per.values() for c in v})
+    key = "SQ_INSTS_VALU" if "SQ_INSTS_VALU" in cols else cols[0]
+    out = [f"# {a.title}", "", f"Command: `{a.command}`", "",
+           "Per-launch averages over the full launches (largest grid of each kernel, counter over 2% of its",
+           "largest) of a --pmc run.", "", "| kernel | launches | " + " | ".join(cols) + " |",
+           "|---|---|" + "---|" * len(cols)]
+    for name in names:
+        disp = [c for (n, g, _), c in per.items() if n == name and g == grid[name]]
+        vmax = max(c.get(key, 0.0) for c in disp)
+        disp = [c for c in disp if c.get(key, 0.0) > 0.02 * vmax] or disp
+        out.append(f"| {name} | {len(disp)} | " +
+                   " | ".join(f"{sum(c.get(k, 0.0) for c in disp) / len(disp):.4g}" for k in cols) + " |")
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -263,6 +304,10 @@ def main():
         t.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
     t.add_argument("--kernel", default="residual")
     t.add_argument("--command", default="")
+    cn = sub.add_parser("counters")
+    cn.add_argument("--pmc", required=True)
+    cn.add_argument("--title", default="PMC counters per launch")
+    cn.add_argument("--command", default="")
     fl = sub.add_parser("fp64")
     fl.add_argument("--pmc", required=True)
     fl.add_argument("--trace", required=True)
@@ -273,6 +318,8 @@ def main():
         sys.stdout.write(stats(a.dir, a.title, a.command))
     elif a.cmd == "fp64":
         sys.stdout.write(fp64(a))
+    elif a.cmd == "counters":
+        sys.stdout.write(counters(a))
     else:
         print(json.dumps(traffic(a), indent=1))
 
