@@ -371,6 +371,34 @@ int Engine::add(const ppes_config &cfg, int *id)
     };
     drows(h->Dx, NX, r.drow_x);
     drows(h->Dy, NY, r.drow_y);
+    // OEDG face-sum tables of each face direction: the scaled transverse D,
+    // and the weights in the recurrence of DG_solver.f90:317-342 (h^m and
+    // the factorials advance only while the derivative order does)
+    auto oedg_tabs = [&](int d, int pn, int pt, double hn, double ht, const std::vector<double> &Dt) {
+        const int NT = pt + 1;
+        for (int m = 0; m < NT * NT; ++m) r.oedg_d2[d][m] = (2.0 / ht) * Dt[m];
+        const int den0 = 2 * (2 * pn - 1 > 1 ? 2 * pn - 1 : 1);
+        const int MXP = std::max(h->px, h->py);
+        double h_pow_o = 1.0;
+        int fac_o = 1;
+        for (int o = 0; o <= pn; ++o) {
+            int fac_q = 1;
+            double h_pow = h_pow_o;
+            for (int q = 0; q <= MXP - o; ++q) {
+                r.oedg_coef[d][o][q] = ((double)(2 * (o + q) + 1) * h_pow) / (double)(den0 * fac_o * fac_q);
+                if (q < pt) {
+                    fac_q = fac_q * (q + 1);
+                    h_pow = h_pow * hn;
+                }
+            }
+            if (o < pn) {
+                fac_o = fac_o * (o + 1);
+                h_pow_o = h_pow_o * hn;
+            }
+        }
+    };
+    oedg_tabs(0, h->px, h->py, h->hx, h->hy, h->Dy);
+    oedg_tabs(1, h->py, h->px, h->hy, h->hx, h->Dx);
     if (set_bc_rules(r, cfg.bc_kind, cfg.bc_state, nx, cfg.gamma) != PPES_OK) {
         delete h;
         return PPES_BAD_ARG;
@@ -490,8 +518,9 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapP
         const long long nres = cdiv((long long)r.nx * r.ny, (long long)(g.res_tb / NN));
         for (long long b = 0; b < nres; ++b) rsf[hosts[id]->flux].push_back({id, (int)b});
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
-        const long long nf = ((long long)(r.nx + 1) * r.ny + (long long)r.nx * (r.ny + 1)) * 8;
-        for (long long b = 0; b < cdiv(nf, BLOCK); ++b) fc.push_back({id, (int)b});
+        // x-face workgroups, then y-face ones (the kernel's split)
+        const long long nf = cdiv((long long)(r.nx + 1) * r.ny * 8, BLOCK) + cdiv((long long)r.nx * (r.ny + 1) * 8, BLOCK);
+        for (long long b = 0; b < nf; ++b) fc.push_back({id, (int)b});
         const long long nb = (2LL * r.nx + 2LL * r.ny) * NN;
         for (long long b = 0; b < cdiv(nb, BLOCK); ++b) bc.push_back({id, (int)b});
         for (long long b = 0; b < cdiv(r.nblk_slot, BLOCK); ++b) r1.push_back({id, (int)b});

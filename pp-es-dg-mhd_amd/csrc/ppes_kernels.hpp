@@ -1084,74 +1084,63 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk 
 // from rows of D^o (precomputed per direction): trace_o(t) = (2/h)^o
 // sum_a (D^o)(last or 0, a) u(a, t), which is N^2 work per order instead of
 // N^3. The transverse derivatives of the jump and the weighted sums follow
-// the reference's loops. PN / PT are the normal and transverse degrees of
-// the face (PX, PY on x-faces; PY, PX on y-faces).
+// the reference's loops, one normal order at a time (each order's jump is
+// finished before the next is formed, so only one order's jump is held);
+// the transverse matrix (2/h_t) D_t and the weights come precomputed from
+// the instance record. PN / PT are the normal and transverse degrees of the
+// face (PX, PY on x-faces; PY, PX on y-faces).
 template <int PN, int PT, int MXP, bool XF, int NX>
 __device__ __forceinline__ double oedg_face_sum(const double *A, const double *B, const double (*srn)[MAXN][MAXN],
-                                                const double *Dt, double h, double ht, const double *qwt)
+                                                const double *d2t, const double (*coef)[MAXN], const double *qwt,
+                                                double h)
 {
     constexpr int NNO = PN + 1, NT = PT + 1;
-    constexpr int den0 = 2 * (2 * PN - 1 > 1 ? 2 * PN - 1 : 1);
-    // jmp[o][t]: jump of the o-th normal derivative at transverse node t
-    double jmp[NNO][NT];
-#pragma unroll
-    for (int o = 0; o <= PN; ++o)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) jmp[o][t] = 0.0;
-    // node (a, t) of the face-normal / transverse index pair: a + NX t on
-    // x-faces, t + NX a on y-faces
-#pragma unroll
-    for (int a = 0; a < NNO; ++a) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int m = XF ? a + NX * t : t + NX * a;
-#pragma unroll
-            for (int o = 0; o <= PN; ++o) jmp[o][t] += srn[1][o][a] * B[m] - srn[0][o][a] * A[m];
-        }
-    }
-    double D2t[NT * NT];
-#pragma unroll
-    for (int m = 0; m < NT * NT; ++m) D2t[m] = (2.0 / ht) * Dt[m];
-    double total = 0.0, scale = 1.0, h_pow_o = 1.0;
-    int fac_o = 1;
+    double total = 0.0, scale = 1.0;
 #pragma unroll
     for (int o = 0; o <= PN; ++o) {
+        // jump of the o-th normal derivative at transverse node t; node
+        // (a, t) of the face-normal / transverse index pair is a + NX t on
+        // x-faces, t + NX a on y-faces
         double jump[NT], jt[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) jump[t] = scale * jmp[o][t];
-        int fac_q = 1;
-        double h_pow = h_pow_o;
+        for (int t = 0; t < NT; ++t) jump[t] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NNO; ++a) {
+            const double r1 = srn[1][o][a], r0 = srn[0][o][a];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int m = XF ? a + NX * t : t + NX * a;
+                jump[t] += r1 * B[m] - r0 * A[m];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) jump[t] = scale * jump[t];
 #pragma unroll
         for (int q = 0; q <= MXP - o; ++q) {
-            const int mm = o + q;
             double jl1 = 0.0;
 #pragma unroll
             for (int t = 0; t < NT; ++t) jl1 += (0.5 * qwt[t]) * fabs(jump[t]);
-            const double coef = ((double)(2 * mm + 1) * h_pow) / (double)(den0 * fac_o * fac_q);
-            total = total + coef * jl1;
+            total = total + coef[o][q] * jl1;
             if (q < PT) {
 #pragma unroll
                 for (int a = 0; a < NT; ++a) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int b = 0; b < NT; ++b) acc += D2t[a + NT * b] * jump[b];
+                    for (int b = 0; b < NT; ++b) acc += d2t[a + NT * b] * jump[b];
                     jt[a] = acc;
                 }
 #pragma unroll
                 for (int a = 0; a < NT; ++a) jump[a] = jt[a];
-                fac_q = fac_q * (q + 1);
-                h_pow = h_pow * h;
             }
         }
-        if (o < PN) {
-            fac_o = fac_o * (o + 1);
-            h_pow_o = h_pow_o * h;
-            scale = scale * (2.0 / h);
-        }
+        if (o < PN) scale = scale * (2.0 / h);
     }
     return total;
 }
 
+// The face direction is uniform over a workgroup: an instance's x-face
+// workgroups (cdiv(8 (nx+1) ny, BLOCK) of them) come first, then its y-face
+// ones, so every table the sums read is one uniform address.
 template <int PX, int PY>
 __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
@@ -1159,12 +1148,12 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
-    const long long tid = (long long)bm.local * BLOCK + threadIdx.x;
     const long long nxf = (long long)(I.nx + 1) * I.ny * 8, nyf = (long long)I.nx * (I.ny + 1) * 8;
-    const bool live = tid < nxf + nyf;
+    const long long nbx = (nxf + BLOCK - 1) / BLOCK;
+    const bool xface = bm.local < nbx;
+    const long long tt = (xface ? (long long)bm.local : bm.local - nbx) * BLOCK + threadIdx.x;
+    if (tt >= (xface ? nxf : nyf)) return;
     const gdbl *__restrict__ u = gin(I.u[st]);
-    const bool xface = tid < nxf;
-    const long long tt = live ? (xface ? tid : tid - nxf) : 0;
     const int v = (int)(tt % 8);
     const long long f = tt / 8;
     long long bA, bB;
@@ -1177,26 +1166,20 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *ma
         bA = gidx(I, i, j - 1) * NN;
         bB = gidx(I, i, j) * NN;
     }
-    // the rows of Dx^o and Dy^o, shared by the workgroup (one instance per workgroup)
-    __shared__ double srx[2][MAXN][MAXN], sry[2][MAXN][MAXN];
-    if (threadIdx.x < 2 * MAXN * MAXN) {
-        const int x = threadIdx.x, e = x / (MAXN * MAXN), o = (x / MAXN) % MAXN, a = x % MAXN;
-        srx[e][o][a] = I.drow_x[e][o][a];
-        sry[e][o][a] = I.drow_y[e][o][a];
-    }
-    __syncthreads();
-    if (!live) return;
     const gdbl *uA = u + v * I.vs + bA;
     const gdbl *uB = u + v * I.vs + bB;
     double A[NN], B[NN];
 #pragma unroll
     for (int m = 0; m < NN; ++m) { A[m] = uA[m]; B[m] = uB[m]; }
-    double total;
-    if (xface) total = oedg_face_sum<PX, PY, MXP, true, NX>(A, B, srx, I.Dy, I.hx, I.hy, I.qw_y);
-    else total = oedg_face_sum<PY, PX, MXP, false, NX>(A, B, sry, I.Dx, I.hy, I.hx, I.qw_x);
-    if (xface) gout(I.sigx)[f * 8 + v] = total;
-    else gout(I.sigy)[f * 8 + v] = total;
+    if (xface) {
+        const double total = oedg_face_sum<PX, PY, MXP, true, NX>(A, B, I.drow_x, I.oedg_d2[0], I.oedg_coef[0], I.qw_y, I.hx);
+        gout(I.sigx)[f * 8 + v] = total;
+    } else {
+        const double total = oedg_face_sum<PY, PX, MXP, false, NX>(A, B, I.drow_y, I.oedg_d2[1], I.oedg_coef[1], I.qw_x, I.hy);
+        gout(I.sigy)[f * 8 + v] = total;
+    }
 }
+
 
 // OEDG path, limiter loop (DG_solver.f90:428-656): OEDG damping toward the
 // cell mean, the positivity limiter, the cell entropy, the final state.

@@ -78,6 +78,13 @@ struct Inst {
     double went[MAXN * MAXN];        // (((0.25 hx) hy) w_k) w_l
     double drow_x[2][MAXN][MAXN];    // drow_x[0][o][a] = (Dx^o)(NX-1, a), drow_x[1][o][a] = (Dx^o)(0, a)
     double drow_y[2][MAXN][MAXN];    // the same for Dy
+    // OEDG face sums, by face direction d (0: x-faces, 1: y-faces):
+    // oedg_d2[d] = (2 / h_t) D_t, the transverse differentiation matrix
+    // (Dy on x-faces, Dx on y-faces) scaled to the physical element;
+    // oedg_coef[d][o][q] = (2(o + q) + 1) h^(o+q) / (den0 o! q!), the weight of
+    // the jump of the o-th normal and q-th transverse derivative
+    double oedg_d2[2][MAXN * MAXN];
+    double oedg_coef[2][MAXN][MAXN];
     int bc_kind, pad0;
     int side[4], nfix[4];            // SideRule per side; cells 1..nfix[side] take bc_state
     double bc_state[8];
