@@ -302,12 +302,16 @@ __device__ __forceinline__ void ec_lf_flux_x(const double wl[8], const double wr
 // Shared-reciprocal forms of the two flux functions (the library's fast
 // arithmetic mode): the same mathematics with one division per distinct
 // denominator -- (zeta - 1) / (zeta + 1) as (L - R) / (L + R), x / p as
-// x * (1 / p), / (g - 1) as * (1 / (g - 1)) -- and no fused multiply-adds,
-// so that both elements sharing a face still compute bit-identical face
-// fluxes. Results differ from the reference's rounding by a few ulp.
+// x * (1 / p), / (g - 1) as * (1 / (g - 1)) -- and a * b + c contracted to a
+// fused multiply-add within each statement (`fp contract(on)`). Contraction
+// is fixed per source expression, not left to instruction scheduling, so
+// every inlined copy rounds alike and both elements sharing a face still
+// compute bit-identical face fluxes. Results differ from the reference's
+// rounding by a few ulp.
 
 __device__ __forceinline__ double log_avg_divisor_fast(double LL, double RR)
 {
+#pragma clang fp contract(on)
     const double f = (LL - RR) / (LL + RR);
     const double u = f * f;
     if (u < 1.4e-4) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
@@ -317,6 +321,7 @@ __device__ __forceinline__ double log_avg_divisor_fast(double LL, double RR)
 template <int DIR>
 __device__ __forceinline__ void ec_flux_fast(const Prim &L, const Prim &R, double inv_gm1, double f[8])
 {
+#pragma clang fp contract(on)
     const double rho_bar = 0.5 * (L.rho + R.rho);
     const double vx_bar = 0.5 * (L.vx + R.vx);
     const double vy_bar = 0.5 * (L.vy + R.vy);
@@ -362,6 +367,7 @@ __device__ __forceinline__ void ec_flux_fast(const Prim &L, const Prim &R, doubl
 __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const double wr[8], double g, double inv_gm1,
                                                      double nflux[8], double &nBn, double &sl_out, double &sr_out)
 {
+#pragma clang fp contract(on)
     const double rho_l = wl[0], mx_l = wl[1], my_l = wl[2], mz_l = wl[3], bx_l = wl[4], by_l = wl[5], bz_l = wl[6], e_l = wl[7];
     const double rho_r = wr[0], mx_r = wr[1], my_r = wr[2], mz_r = wr[3], bx_r = wr[4], by_r = wr[5], bz_r = wr[6], e_r = wr[7];
     const double inv_rho_l = 1.0 / rho_l, inv_rho_r = 1.0 / rho_r;
