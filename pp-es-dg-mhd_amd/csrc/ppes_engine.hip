@@ -149,7 +149,7 @@ Engine::~Engine()
     for (auto &gs : gstreams) (void)hipStreamDestroy(gs);
     for (auto &ge : gevents) (void)hipEventDestroy(ge);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
-    for (MapPool *ps : {plan_pools, start_pools})
+    for (MapPool *ps : {plan_pools, start_pools, early_pools})
         for (int sid = 0; sid < 9; ++sid) {
             for (Blk *p : ps[sid].d) if (p) (void)hipFree(p);
             if (ps[sid].dl) (void)hipFree(ps[sid].dl);
@@ -621,7 +621,8 @@ int Engine::build_plan()
     }
     graph_ok = false;
     plan_steps = 0;
-    groups.clear();  // their maps live in plan_pools
+    groups.clear();  // their maps live in plan_pools (and early_pools)
+    early_groups.clear();
     std::vector<int> all;
     for (int s = 0; s <= MAXS; ++s) any_oedg_stage[s] = false;
     for (int sid = 1; sid <= 8; ++sid) {
@@ -632,13 +633,25 @@ int Engine::build_plan()
         Group g;
         int rc = build_group(g, mem, sid, &plan_pools[sid]);
         if (rc) return rc;
-        groups.push_back(g);
+        std::vector<int> every;
         for (int id : mem) {
             all.push_back(id);
             const int mode = h_recs[id].ctl.oedg_mode;
             for (int st = 1; st <= S; ++st)
                 if (mode == 2 || (mode == 1 && st == S)) any_oedg_stage[st] = true;
+            g.oedg_final = g.oedg_final || mode != 0;
+            if (mode == 2) every.push_back(id);
         }
+        // stages before the last launch the OEDG kernels over the every-stage
+        // members only (the others' workgroups would exit at once)
+        g.n_every = (int)every.size();
+        if (!every.empty() && every.size() < mem.size()) {
+            Group e;
+            if ((rc = build_group(e, every, sid, &early_pools[sid]))) return rc;
+            g.early = (int)early_groups.size();
+            early_groups.push_back(e);
+        }
+        groups.push_back(g);
     }
     n_all = (int)all.size();
     h_all.swap(all);
@@ -723,7 +736,10 @@ int Engine::launch_step(bool all_kernels)
             }
         }
         if (any_oedg_stage[st]) {
-            for (auto &g : groups) {
+            for (auto &g0 : groups) {
+                const Group *og = oedg_group(g0, st);
+                if (!og) continue;
+                const Group &g = *og;
                 {
                     Timed tm(this, KT_REDUCE);
                     LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, st, 0, 0);
@@ -792,14 +808,15 @@ int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
         } else {
             ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
         }
-        if (any_oedg_stage[st]) {
-            LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, st, 0, 0);
-            LAUNCH(k_reduce_global, (long long)nm, d_recs, (const Blk *)g.d_red2, st, 0, 1);
-            ENG_DISPATCH(g.sid, LAUNCH((k_oedg_faces<PX, PY>), g.n_face, d_recs, (const Blk *)g.d_face, st, 0));
-            if (g.eo) {
-                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+        if (const Group *og = oedg_group(g, st)) {
+            const Group &o = *og;
+            LAUNCH(k_reduce_global, o.n_red1, d_recs, (const Blk *)o.d_red1, st, 0, 0);
+            LAUNCH(k_reduce_global, (long long)o.members.size(), d_recs, (const Blk *)o.d_red2, st, 0, 1);
+            ENG_DISPATCH(o.sid, LAUNCH((k_oedg_faces<PX, PY>), o.n_face, d_recs, (const Blk *)o.d_face, st, 0));
+            if (o.eo) {
+                ENG_DISPATCH(o.sid, LAUNCH((k_oedg_limit<PX, PY, true>), o.n_el, d_recs, (const Blk *)o.d_el, st, 0));
             } else {
-                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, false>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
+                ENG_DISPATCH(o.sid, LAUNCH((k_oedg_limit<PX, PY, false>), o.n_el, d_recs, (const Blk *)o.d_el, st, 0));
             }
         }
         if (st == S) LAUNCH(k_reduce_ent, (long long)nm, d_recs, dl, st, 0);

@@ -42,6 +42,11 @@ struct Group {
     int res_tb = 0;  // workgroup size of the residual launches (the residual map's elements per block follow)
     long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
     bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
+    // OEDG participation: some member applies it on the final stage; the
+    // members that apply it on every stage (all, none, or the sub-plan
+    // early_groups[early], whose maps cover only them)
+    bool oedg_final = false;
+    int n_every = 0, early = -1;
 };
 
 // Device buffers of one degree's block maps, kept across plan rebuilds and
@@ -130,6 +135,16 @@ private:
     int build_plan();
     int build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool = nullptr);
     MapPool plan_pools[9], start_pools[9];  // by shape id: the step plan's maps, the start-up's
+    MapPool early_pools[9];                 // by shape id: the every-stage OEDG sub-plans' maps
+    std::vector<Group> early_groups;        // every-stage OEDG members of a group, when not all of them
+    // the maps that cover exactly the instances of group g that may apply
+    // OEDG on stage st (the group itself or its sub-plan), or none
+    const Group *oedg_group(const Group &g, int st) const
+    {
+        if (st == S) return g.oedg_final ? &g : nullptr;
+        if (g.n_every == 0) return nullptr;
+        return g.early < 0 ? &g : &early_groups[g.early];
+    }
     size_t all_cap = 0;
     std::vector<int> h_all;
     long long plan_steps = 0;  // steps launched with the current plan (the graph is captured from the second)
