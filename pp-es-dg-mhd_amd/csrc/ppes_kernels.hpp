@@ -115,7 +115,7 @@ __device__ __forceinline__ void rk_combine_node(const StageSrc &c, long long os,
 // Build-time shape of the fast residual: waves per SIMD asked of the register
 // allocator, early trace loads, one-round pair buffer.
 #ifndef PPES_FAST_WAVES
-#define PPES_FAST_WAVES 3  // P4
+#define PPES_FAST_WAVES 4  // P4 (4 waves with a short spill beat 3 without one)
 #endif
 #ifndef PPES_FAST_WAVES_LOW
 #define PPES_FAST_WAVES_LOW 1  // P1-P3
