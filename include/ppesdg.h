@@ -255,6 +255,26 @@ int ppes_ldf_nodal_operator(int p, double hx, double hy, double *op);
  * Legendre coefficients (Bx then By, x index fastest), 2N^2 x 2N^2 (host only) */
 int ppes_ldf_modal_operator(int p, double *op);
 
+/* DG_semidiscretize without allocating (no counterpart in the reference: for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same residual
+ * kernels as ppes_semidiscretize, on arrays the caller owns, in the library's
+ * device layout (structure of arrays, N^2 = (px+1)(py+1) nodes per element):
+ *   u   state, device memory, 8 (nx+2)(ny+2) N^2 doubles (ghost strips filled):
+ *       u[((v (ny+2) + ey) (nx+2) + ex) N^2 + k + (px+1) l]
+ *   ut  residual, device memory, 8 nx ny N^2 doubles:
+ *       ut[((v ny + (ey-1)) nx + (ex-1)) N^2 + k + (px+1) l]
+ * flux: PPES_FLUX_*; arith: PPES_ARITH_*. *dt_lim (host) receives dt_lim.
+ * Runs on `stream` (a hipStream_t; NULL: the null stream) and waits for it.
+ * Its only device memory beyond u and ut is a fixed workspace per GPU (one
+ * instance record and the residual block map, PPES_NOALLOC_MAX_BLOCKS
+ * workgroups at most: grids that need more return PPES_BAD_ARG), held until
+ * the process ends. It calls no hipMalloc or hipFree. Calls on one GPU share
+ * that workspace, so calls that overlap on one GPU race. */
+#define PPES_NOALLOC_MAX_BLOCKS 262144
+int ppes_semidiscretize_noalloc(const ppes_config *cfg, int flux, int arith, const double *u, double *ut,
+                                double *dt_lim, void *stream);
+
 /* wait for the solver's stream */
 int ppes_sync(ppes_solver *s);
 const char *ppes_status_string(int status);

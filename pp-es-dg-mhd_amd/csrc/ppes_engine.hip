@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "ppes_engine.hpp"
 #include "ppes_kernels.hpp"
@@ -276,6 +277,37 @@ int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g)
     return PPES_OK;
 }
 
+// Sizes, geometry, gamma and the 1D tables of an instance record: every
+// field the residual reads apart from the buffer pointers and the drive-loop
+// state (DG_solver.f90:47-106). Stack arrays only.
+static void record_base(Inst &r, const ppes_config &cfg, int S)
+{
+    const int NX = cfg.px + 1, NY = cfg.py + 1, NN = NX * NY, nx = cfg.nx, ny = cfg.ny;
+    double qx[MAXN], qw_x[MAXN], qw_y[MAXN], Dx[MAXN * MAXN], Dy[MAXN * MAXN];
+    host_tables(cfg.px, qx, qw_x, Dx, nullptr, nullptr);
+    host_tables(cfg.py, qx, qw_y, Dy, nullptr, nullptr);
+    const double hx = (cfg.right - cfg.left) / nx, hy = (cfg.top - cfg.bottom) / ny;
+    r.nx = nx;
+    r.ny = ny;
+    r.px = cfg.px;
+    r.py = cfg.py;
+    r.S = S;
+    r.vs = (long long)(nx + 2) * (ny + 2) * NN;
+    r.vsi = (long long)nx * ny * NN;
+    r.hx = hx;
+    r.hy = hy;
+    r.g = cfg.gamma;
+    for (int i = 0; i < NX; ++i) r.qw_x[i] = qw_x[i];
+    for (int i = 0; i < NY; ++i) r.qw_y[i] = qw_y[i];
+    for (int i = 0; i < NX * NX; ++i) r.Dx[i] = Dx[i];
+    for (int i = 0; i < NY * NY; ++i) r.Dy[i] = Dy[i];
+    for (int l = 0; l < NY; ++l)
+        for (int k = 0; k < NX; ++k) {
+            r.wavg[k + NX * l] = (0.25 * qw_x[k]) * qw_y[l];
+            r.went[k + NX * l] = (((0.25 * hx) * hy) * qw_x[k]) * qw_y[l];
+        }
+}
+
 int Engine::add(const ppes_config &cfg, int *id)
 {
     if (cfg.nx < 1 || cfg.ny < 1 || cfg.px < 0 || cfg.py < 0) return PPES_BAD_ARG;
@@ -327,25 +359,7 @@ int Engine::add(const ppes_config &cfg, int *id)
 
     Inst &r = h_recs[slot];
     memset(&r, 0, sizeof r);
-    r.nx = nx;
-    r.ny = ny;
-    r.px = h->px;
-    r.py = h->py;
-    r.S = S;
-    r.vs = (long long)(nx + 2) * (ny + 2) * NN;
-    r.vsi = (long long)nx * ny * NN;
-    r.hx = h->hx;
-    r.hy = h->hy;
-    r.g = cfg.gamma;
-    for (int i = 0; i < NX; ++i) r.qw_x[i] = h->qw_x[i];
-    for (int i = 0; i < NY; ++i) r.qw_y[i] = h->qw_y[i];
-    for (int i = 0; i < NX * NX; ++i) r.Dx[i] = h->Dx[i];
-    for (int i = 0; i < NY * NY; ++i) r.Dy[i] = h->Dy[i];
-    for (int l = 0; l < NY; ++l)
-        for (int k = 0; k < NX; ++k) {
-            r.wavg[k + NX * l] = (0.25 * h->qw_x[k]) * h->qw_y[l];
-            r.went[k + NX * l] = (((0.25 * h->hx) * h->hy) * h->qw_x[k]) * h->qw_y[l];
-        }
+    record_base(r, cfg, S);
     // rows N-1 and 0 of the powers of D, in long double (OEDG face traces)
     auto drows = [](const std::vector<double> &D, int N, double (*out)[MAXN][MAXN]) {
         long double Dl[MAXN][MAXN], Pw[MAXN][MAXN], T[MAXN][MAXN];
@@ -1073,6 +1087,114 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     }
     ENG_TRY(hipStreamSynchronize(stream));
     return rc;
+}
+
+
+// ------------------------------------------------------------------ residual without allocation
+//
+// ppes_semidiscretize_noalloc works in a fixed device workspace: one instance
+// record (the caller's buffer pointers, sizes, tables and the dt_lim cell),
+// then the residual's block map, at most NOALLOC_MAX_BLOCKS workgroups. The
+// array is `scratch`; each device holds its own copy until the process ends.
+// Host code never uses the array's name as an address: hipGetSymbolAddress
+// gives each device's address once, kept in scratch_dev.
+constexpr long long NOALLOC_MAX_BLOCKS = PPES_NOALLOC_MAX_BLOCKS;
+constexpr size_t NOALLOC_REC_BYTES = (sizeof(Inst) + 255) / 256 * 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_REC_BYTES + (size_t)NOALLOC_MAX_BLOCKS * sizeof(Blk);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static unsigned char *scratch_dev[NOALLOC_MAX_DEVICES];
+
+// the current device's address of `scratch`, looked up on the first call on
+// that device
+static int scratch_address(unsigned char **out)
+{
+    int dev = 0;
+    ENG_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return PPES_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "ppesdg: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev, hipGetErrorString(e));
+            return PPES_HIP_ERROR;
+        }
+        scratch_dev[dev] = (unsigned char *)p;
+    }
+    *out = scratch_dev[dev];
+    return PPES_OK;
+}
+
+// residual block map of one instance of n workgroups, in the engine's XCD
+// order (xcd_order of the identity map)
+__global__ void k_noalloc_map(Blk *map, long long n)
+{
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const long long q = n / N_XCD, r = n % N_XCD;
+    long long b = w;
+    if (q > 0) {
+        const long long x = w % N_XCD;
+        b = x * q + (x < r ? x : r) + w / N_XCD;
+    }
+    map[w] = Blk{0, (int)b};
+}
+
+int residual_noalloc(const ppes_config &cfg, int flux, int arith, const double *u, double *ut, double *dt_lim,
+                     hipStream_t stream)
+{
+    const int sid = shape_id(cfg.px, cfg.py);
+    if (sid < 0) return PPES_UNSUPPORTED;
+    if (!u || !ut || !dt_lim || cfg.nx < 1 || cfg.ny < 1) return PPES_BAD_ARG;
+    if (!(cfg.right > cfg.left) || !(cfg.top > cfg.bottom) || !(cfg.gamma > 1.0)) return PPES_BAD_ARG;
+    if (flux < FLUX_ESPP_HLL || flux > FLUX_EC_LF) return PPES_BAD_ARG;
+    if (arith != PPES_ARITH_FAST && arith != PPES_ARITH_EXACT) return PPES_BAD_ARG;
+    const int NN = (cfg.px + 1) * (cfg.py + 1);
+    if ((long long)(cfg.nx + 2) * (cfg.ny + 2) * NN >= (1LL << 31)) return PPES_UNSUPPORTED;
+    // the default residual size: every flux and arithmetic variant exists for it
+    const int tb = resid_tb(NN);
+    const long long nblk = cdiv((long long)cfg.nx * cfg.ny, (long long)(tb / NN));
+    if (nblk > NOALLOC_MAX_BLOCKS) {
+        fprintf(stderr,
+                "ppesdg: ppes_semidiscretize_noalloc: %d x %d elements of degree (%d, %d) need %lld workgroups; "
+                "the fixed workspace holds at most %lld\n",
+                cfg.nx, cfg.ny, cfg.px, cfg.py, nblk, NOALLOC_MAX_BLOCKS);
+        return PPES_BAD_ARG;
+    }
+    unsigned char *ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    Inst *d_recs = reinterpret_cast<Inst *>(ws);
+    Blk *d_map = reinterpret_cast<Blk *>(ws + NOALLOC_REC_BYTES);
+
+    // the record starts each call afresh, dt_lim cell included
+    Inst rec;
+    memset(&rec, 0, sizeof rec);
+    record_base(rec, cfg, 1);
+    rec.u[0] = const_cast<double *>(u);
+    rec.ut[1] = ut;
+    rec.ctl.state = ST_ACTIVE;
+    rec.ctl.dtlim_bits = 0x7FF0000000000000ULL;
+    ENG_TRY(hipMemcpyAsync(d_recs, &rec, sizeof(Inst), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_noalloc_map, dim3((unsigned)cdiv(nblk, BLOCK)), dim3(BLOCK), 0, stream, d_map, nblk);
+    ENG_TRY(hipGetLastError());
+    const Blk *m_ = d_map;
+    switch (flux) {
+    case FLUX_PP_HLL: RESIDUAL_DISPATCH_F(sid, FLUX_PP_HLL, tb, nblk, d_recs, m_, 0, 1, 0); break;
+    case FLUX_ESPP_LF: RESIDUAL_DISPATCH_F(sid, FLUX_ESPP_LF, tb, nblk, d_recs, m_, 0, 1, 0); break;
+    case FLUX_EC_LF: RESIDUAL_DISPATCH_F(sid, FLUX_EC_LF, tb, nblk, d_recs, m_, 0, 1, 0); break;
+    default: RESIDUAL_DISPATCH_F(sid, FLUX_ESPP_HLL, tb, nblk, d_recs, m_, 0, 1, 0); break;
+    }
+    unsigned long long bits = 0;
+    ENG_TRY(hipMemcpyAsync(&bits, &d_recs->ctl.dtlim_bits, sizeof bits, hipMemcpyDeviceToHost, stream));
+    ENG_TRY(hipStreamSynchronize(stream));
+    double d;
+    memcpy(&d, &bits, sizeof d);
+    *dt_lim = d;
+    return PPES_OK;
 }
 
 }  // namespace ppes

@@ -176,4 +176,10 @@ enum ApiCall { API_BC = 0, API_POST = 1, API_RESID = 2, API_COMBINE = 3, API_GHO
 // PPES_BC_FIXED state.
 int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g);
 
+// DG_semidiscretize of a caller-owned device state into a caller-owned
+// residual, with no allocation: the record and block map live in a fixed
+// static device workspace (ppes_semidiscretize_noalloc). Synchronous.
+int residual_noalloc(const ppes_config &cfg, int flux, int arith, const double *u, double *ut, double *dt_lim,
+                     hipStream_t stream);
+
 }  // namespace ppes

@@ -596,6 +596,13 @@ extern "C" int ppes_init_problem(ppes_solver *s, int prob, const double *pert, i
 
 extern "C" int ppes_sync(ppes_solver *s) { return s ? s->eng.sync() : PPES_BAD_ARG; }
 
+extern "C" int ppes_semidiscretize_noalloc(const ppes_config *cfg, int flux, int arith, const double *u, double *ut,
+                                           double *dt_lim, void *stream)
+{
+    if (!cfg) return PPES_BAD_ARG;
+    return residual_noalloc(*cfg, flux, arith, u, ut, dt_lim, (hipStream_t)stream);
+}
+
 extern "C" int ppes_apply_bc(ppes_solver *s, int slot)
 {
     if (!s || slot < 0 || slot > s->eng.S) return PPES_BAD_ARG;

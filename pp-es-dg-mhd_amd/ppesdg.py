@@ -115,6 +115,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.ppes_semidiscretize_noalloc.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.c_int, vp, vp, dp, vp]
+        L.ppes_semidiscretize_noalloc.restype = ctypes.c_int
         L.ppes_instance_bytes.argtypes = [ctypes.POINTER(Config)]
         L.ppes_instance_bytes.restype = ctypes.c_longlong
         L.ppes_status_string.argtypes = [ctypes.c_int]
@@ -160,6 +162,27 @@ def make_config(nx, ny, p, domain, gamma, bc="periodic", bc_state=None, rk=3, py
         for v in range(8):
             cfg.bc_state[v] = float(bc_state[v])
     return cfg
+
+
+def to_device_layout(a):
+    """Fortran state (8, NX, NY, nx+2, ny+2) or residual (8, NX, NY, nx, ny)
+    -> the device layout [v][ey][ex][k + NX l], C-contiguous."""
+    return np.ascontiguousarray(np.asarray(a).transpose(0, 4, 3, 2, 1))
+
+
+def from_device_layout(d):
+    """Inverse of to_device_layout: (8, ey, ex, l, k) -> Fortran (8, k, l, ex, ey)."""
+    return np.asfortranarray(np.asarray(d).transpose(0, 4, 3, 2, 1))
+
+
+def semidiscretize_noalloc(cfg, u_dev, ut_dev, flux="espp_hll", arith="fast", stream=None):
+    """ppes_semidiscretize_noalloc on device buffers the caller owns (device
+    addresses as integers, arrays in the device layout) -> dt_lim."""
+    dtl = ctypes.c_double()
+    _check(lib().ppes_semidiscretize_noalloc(ctypes.byref(cfg), FLUX[flux], ARITH[arith], ctypes.c_void_p(u_dev),
+                                              ctypes.c_void_p(ut_dev), ctypes.byref(dtl), stream),
+           "ppes_semidiscretize_noalloc")
+    return dtl.value
 
 
 def instance_bytes(nx, ny, p, rk=3, py=None):
