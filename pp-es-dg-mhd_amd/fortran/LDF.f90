@@ -1,6 +1,6 @@
 ! Locally divergence-free projector on Legendre modal coefficients of
 ! (Bx, By) on the reference element (drop-in for type(LDF_projector),
-! LDF.f90:94-269). The library supplies the projector as one dense modal
+! LDF.f90:94-269; apply and div are pure, as there). The library supplies the projector as one dense modal
 ! operator (the unique M-orthogonal projection onto divergence-free Q_p^2).
 module LDF_m
 	use ppesdg_capi
@@ -15,6 +15,7 @@ module LDF_m
 		procedure, pass :: init
 		procedure, pass :: apply
 		procedure, pass :: div
+		final :: destroy
 	end type LDF_projector
 contains
 	subroutine init(this, px, py)
@@ -32,14 +33,14 @@ contains
 		this%ready = .true.
 	end subroutine init
 
-	subroutine apply(this, uu)
+	pure subroutine apply(this, uu)
 		class(LDF_projector), intent(in) :: this
 		real, dimension(this%px+1, this%py+1, 2), intent(inout) :: uu
 		if (.not. this%ready) error stop 'LDF_projector: not ready!'
 		uu = reshape(matmul(this%op, reshape(uu, [size(uu)])), shape(uu))
 	end subroutine apply
 
-	subroutine div(this, uu, udiv)
+	pure subroutine div(this, uu, udiv)
 		use gauss_m, only: gauss_legendre_matrix
 		class(LDF_projector), intent(in) :: this
 		real, dimension(this%px+1, this%py+1, 2), intent(in) :: uu
@@ -49,4 +50,11 @@ contains
 		call gauss_legendre_matrix(this%px, m, minv, Dm, tl, tr)
 		udiv = matmul(Dm, uu(:,:,1)) + matmul(uu(:,:,2), transpose(Dm))
 	end subroutine div
+
+	! release the operator (LDF.f90:262-269)
+	subroutine destroy(this)
+		type(LDF_projector), intent(inout) :: this
+		if (allocated(this%op)) deallocate(this%op)
+		this%ready = .false.
+	end subroutine destroy
 end module LDF_m

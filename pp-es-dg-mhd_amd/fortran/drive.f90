@@ -5,9 +5,10 @@
 ! whole SSP-RK loop (stage combines, BCs, post-processing, residuals, the dt
 ! rule, retries on an inadmissible average, the commit) then runs on the
 ! device, one host synchronisation per step, which returns (t, entropy) for
-! the entropy log. The three device boundary kinds are chosen when
-! BC_handle points at BC_periodic, BC_YeeSjogreen or BC_rotor; any other
-! BC_handle is called on the host between device calls (slower, same result).
+! the entropy log. A device boundary kind is chosen when BC_handle points at
+! one of the six shipped fillers (BC_periodic, BC_YeeSjogreen, BC_rotor,
+! BC_BrioWu, BC_cloud_shock, BC_jet1_cell_ext); any other BC_handle is called
+! on the host between device calls (slower, same result).
 module drive_m
 	implicit none
 

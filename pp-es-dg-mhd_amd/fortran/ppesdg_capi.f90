@@ -142,13 +142,15 @@ module ppesdg_capi
 			type(ppes_run_stats), intent(out) :: stats
 		end function ppes_advance
 
-		integer(c_int) function ppes_tables(p, x, w, D, Pmn, Pnm) bind(C, name='ppes_tables')
+		! host-only table builders without side effects: pure, so that the
+		! reference's pure routines (gauss_lobatto, LDF_projector%apply) stay pure
+		pure integer(c_int) function ppes_tables(p, x, w, D, Pmn, Pnm) bind(C, name='ppes_tables')
 			import :: c_int, c_double
 			integer(c_int), value :: p
 			real(c_double), intent(out) :: x(*), w(*), D(*), Pmn(*), Pnm(*)
 		end function ppes_tables
 
-		integer(c_int) function ppes_ldf_modal_operator(p, op) bind(C, name='ppes_ldf_modal_operator')
+		pure integer(c_int) function ppes_ldf_modal_operator(p, op) bind(C, name='ppes_ldf_modal_operator')
 			import :: c_int, c_double
 			integer(c_int), value :: p
 			real(c_double), intent(out) :: op(*)

@@ -18,6 +18,12 @@ module problems
 
 	procedure(fun_type), pointer, public :: init_pri_fun => null()
 	procedure(fun_type), pointer, public :: init_con_fun => null()
+
+	! left and right primitive states of a fixed-state boundary, and the
+	! primitive function of a function-defined boundary (problems.f90:16-20):
+	! public for programs that set them; no shipped filler reads them
+	real, dimension(4), public :: u_pri_L, u_pri_R
+	procedure(fun_type), pointer, public :: BC_pri_fun => null()
 contains
 
 	subroutine init_by_fun_pri(u_con, xx, yy, nx, ny, px, py)
@@ -56,7 +62,7 @@ contains
 	end subroutine init_by_fun_con
 
 	! Orszag-Tang vortex on [0,1]^2 (problems.f90:155-168)
-	function OrszagTang_init_pri(x, y) result(u_pri)
+	pure function OrszagTang_init_pri(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		u_pri = [25.0/(36.0*pi), -sin(2*pi*y), sin(2*pi*x), 0.0, -sin(2*pi*y)/sqrt(4.0*pi), &
@@ -65,7 +71,7 @@ contains
 
 	! Orszag-Tang vortex on [0,2pi]^2 with gamma from MHD_PDE_m: rho = gamma^2,
 	! p = gamma, u = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0)
-	function OrszagTang2pi_init_pri(x, y) result(u_pri)
+	pure function OrszagTang2pi_init_pri(x, y) result(u_pri)
 		use MHD_PDE_m, only: gamma
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
@@ -73,7 +79,7 @@ contains
 	end function OrszagTang2pi_init_pri
 
 	! rotor (problems.f90:170-199)
-	function Rotor_init_pri(x, y) result(u_pri)
+	pure function Rotor_init_pri(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		real, parameter :: r0 = 0.1, r1 = 0.115, u0 = 2.0
@@ -93,7 +99,7 @@ contains
 	end function Rotor_init_pri
 
 	! blast wave on [0,1]^2 (problems.f90:201-218)
-	function Blast_init_pri(x, y) result(u_pri)
+	pure function Blast_init_pri(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		u_pri = [1.0, 0.0, 0.0, 0.0, 100.0/sqrt(4*pi), 0.0, 0.0, 0.1]
@@ -101,7 +107,7 @@ contains
 	end function Blast_init_pri
 
 	! blast wave on [-0.5,0.5]^2 with a diagonal field B = (100/sqrt(4pi))(1,1,0)/sqrt(2)
-	function BlastDiag_init_pri(x, y) result(u_pri)
+	pure function BlastDiag_init_pri(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		real :: b0
@@ -111,7 +117,7 @@ contains
 	end function BlastDiag_init_pri
 
 	! smooth Alfven wave along the direction pi/6 (problems.f90:74-98)
-	function smooth_alfven_init_fun(x, y) result(u_pri)
+	pure function smooth_alfven_init_fun(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		real :: ca, sa, xp, vp, vz
@@ -124,7 +130,7 @@ contains
 	end function smooth_alfven_init_fun
 
 	! Kelvin-Helmholtz shear layers at y = +-0.5 (problems.f90:100-113)
-	function KHI_init_pri(x, y) result(u_pri)
+	pure function KHI_init_pri(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		u_pri = [1.0, 5.0*((tanh(20*(y + 0.5)) - tanh(20*(y - 0.5))) - 1.0), &
@@ -132,7 +138,7 @@ contains
 	end function KHI_init_pri
 
 	! Brio-Wu shock tube, discontinuity at x = 0 (problems.f90:115-138)
-	function Brio_Wu_init_fun(x, y) result(u_pri)
+	pure function Brio_Wu_init_fun(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		if (x < 0.0) then
@@ -143,14 +149,14 @@ contains
 	end function Brio_Wu_init_fun
 
 	! smooth density wave advected in x (problems.f90:140-153)
-	function smooth_1Dwave_init_fun(x, y) result(u_pri)
+	pure function smooth_1Dwave_init_fun(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		u_pri = [1.0 + 0.2*sin(x), 1.0, 0.0, 0.0, 0.5, 1.0, 1.5, 2.0]
 	end function smooth_1Dwave_init_fun
 
 	! shock (x < 0.6) hitting a dense cloud at (0.8, 0.5) (problems.f90:220-247)
-	function cloud_shock_init_pri(x, y) result(u_pri)
+	pure function cloud_shock_init_pri(x, y) result(u_pri)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
 		if (x < 0.6) then
@@ -161,13 +167,13 @@ contains
 		end if
 	end function cloud_shock_init_pri
 
-	function cloud_inflow_pri() result(u_pri)
+	pure function cloud_inflow_pri() result(u_pri)
 		real, dimension(8) :: u_pri
 		u_pri = [1.0, -11.2536, 0.0, 0.0, 0.0, 0.56418958, 0.56418958, 1.0]
 	end function cloud_inflow_pri
 
 	! astrophysical jet: the ambient state (problems.f90:249-257)
-	function jet1_init_pri(x, y) result(u_pri)
+	pure function jet1_init_pri(x, y) result(u_pri)
 		use MHD_PDE_m, only: gamma
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_pri
@@ -175,7 +181,7 @@ contains
 	end function jet1_init_pri
 
 	! 2D Riemann problem of Yee and Sjogreen, conservative (problems.f90:259-308)
-	function YeeSjogreenInitCon(x, y) result(u_con)
+	pure function YeeSjogreenInitCon(x, y) result(u_con)
 		real, intent(in) :: x, y
 		real, dimension(8) :: u_con
 		if (y >= 0.0 .and. x >= 0.0) then
@@ -218,7 +224,7 @@ contains
 	end subroutine BC_YeeSjogreen
 
 	! the rotor's fixed ambient state
-	function rotor_ghost_state() result(u_con_)
+	pure function rotor_ghost_state() result(u_con_)
 		use MHD_PDE_m, only: MHD_pri2con
 		real, dimension(8) :: u_con_
 		u_con_ = MHD_pri2con([1.0, 0.0, 0.0, 0.0, 5.0/sqrt(4*pi), 0.0, 0.0, 1.0])
@@ -278,7 +284,7 @@ contains
 		u_con(:,:,:,1:nx,ny+1) = u_con(:,:,:,1:nx,ny)
 	end subroutine BC_cloud_shock
 
-	function jet_nozzle_state() result(u_con_)
+	pure function jet_nozzle_state() result(u_con_)
 		use MHD_PDE_m, only: gamma, MHD_pri2con
 		real, dimension(8) :: u_con_
 		u_con_ = MHD_pri2con([gamma, 0.0, 800.0, 0.0, 0.0, sqrt(200.0), 0.0, 1.0])
