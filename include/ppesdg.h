@@ -223,6 +223,16 @@ int ppes_batch_start(ppes_batch *b, int id, double t_final, long long max_steps,
  * active if nsteps <= 0); *n_active = instances still active afterwards */
 int ppes_batch_step(ppes_batch *b, long long nsteps, int use_graph, long long *done_steps, int *n_active);
 int ppes_batch_status(ppes_batch *b, int id, int *state, ppes_run_stats *stats);
+/* Positivity record (no reference counterpart: the check that density and
+ * pressure stay positive at every GLL node after every stage). The smallest
+ * density and pressure at any interior node of any post-processed stage
+ * since the instance's solve started (ppes_solve, ppes_solve_begin,
+ * ppes_batch_start) or, before that, since creation; per-call post-processes
+ * count too. Kept on the device (a wavefront minimum and one atomic per
+ * wavefront in the kernels that store the limited state). +inf: no node
+ * recorded yet; a NaN node reads as -inf. */
+int ppes_positivity(ppes_solver *s, double *min_rho, double *min_p);
+int ppes_batch_positivity(ppes_batch *b, int id, double *min_rho, double *min_p);
 /* device bytes held by an instance / needed by a configuration */
 int ppes_batch_memory(ppes_batch *b, int id, long long *bytes);
 long long ppes_instance_bytes(const ppes_config *cfg);

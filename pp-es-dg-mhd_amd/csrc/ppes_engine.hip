@@ -422,6 +422,7 @@ int Engine::add(const ppes_config &cfg, int *id)
     r.nblk_slot = cdiv((long long)(nx + 2) * (ny + 2), E);
     r.ctl.state = ST_IDLE;
     r.ctl.dtlim_bits = 0x7FF0000000000000ULL;
+    r.ctl.min_rho_key = r.ctl.min_p_key = POS_KEY_NONE;
 
     auto alloc = [&](double **p, size_t n) -> bool {
         if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) { *p = nullptr; return false; }

@@ -598,6 +598,32 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
     }
 }
 
+// The smallest density and pressure over the nodes of a workgroup that
+// carry a final limited state (on = true), recorded in the instance's
+// positivity keys: a butterfly of lane exchanges across each 64-lane
+// wavefront (elements may straddle wavefronts; the minimum does not care),
+// then one atomic per wavefront and quantity. A NaN enters as -inf, below
+// every number. Called by every thread of the workgroup.
+__device__ __forceinline__ void record_positivity(Inst &I, bool on, const double q[8])
+{
+    double r = DBL_HUGE, p = DBL_HUGE;
+    if (on) {
+        r = q[0];
+        p = prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], I.g);
+        r = r == r ? r : -__builtin_inf();
+        p = p == p ? p : -__builtin_inf();
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        r = dmin(r, __shfl_xor(r, m));
+        p = dmin(p, __shfl_xor(p, m));
+    }
+    if ((threadIdx.x & 63) == 0 && (r < DBL_HUGE || p < DBL_HUGE)) {
+        atomicMin(&I.ctl.min_rho_key, pos_key(r));
+        atomicMin(&I.ctl.min_p_key, pos_key(p));
+    }
+}
+
 // Cell entropy of one element summed in the reference's node order, then the
 // workgroup's sum over its counted elements in element order; thread 0
 // receives the workgroup partial (DG_solver.f90:635-649).
@@ -1018,6 +1044,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     if (valid && ok)
 #pragma unroll
         for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
+    record_positivity(I, lim, q);
 }
 
 // Two-level deterministic reduction of the per-workgroup (sum, min, max)
@@ -1267,6 +1294,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
 #pragma unroll
         for (int v = 0; v < 8; ++v) u[v * vs + base] = q[v];
     }
+    record_positivity(I, valid, q);
 }
 
 // total entropy of each instance: one workgroup per instance, contiguous

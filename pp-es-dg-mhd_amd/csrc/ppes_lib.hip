@@ -390,6 +390,7 @@ static int begin_instance(Engine &e, int id, double t_final, long long max_steps
     c.cfl_eff = ((e.S <= 3) ? 1.0 : 1.508) * cfl;  // CFL_RK (drive.f90:44-56)
     c.max_steps = max_steps;
     c.dtlim_bits = 0x7FF0000000000000ULL;
+    c.min_rho_key = c.min_p_key = POS_KEY_NONE;
     if ((rc = e.push())) return rc;
     e.invalidate_plan();
     if ((rc = e.launch_startup(std::vector<int>{id}))) return rc;
@@ -893,6 +894,31 @@ extern "C" int ppes_batch_status(ppes_batch *b, int id, int *state, ppes_run_sta
     const Inst &r = b->eng.rec(id);  // host mirror, current after the last batch call
     if (state) *state = r.ctl.state;
     if (stats) fill_stats(r, stats);
+    return PPES_OK;
+}
+
+// the positivity keys of an instance record as numbers (+inf: no node yet)
+static void read_positivity(const Inst &r, double *min_rho, double *min_p)
+{
+    if (min_rho) *min_rho = r.ctl.min_rho_key == POS_KEY_NONE ? HUGE_VAL : pos_unkey(r.ctl.min_rho_key);
+    if (min_p) *min_p = r.ctl.min_p_key == POS_KEY_NONE ? HUGE_VAL : pos_unkey(r.ctl.min_p_key);
+}
+
+extern "C" int ppes_batch_positivity(ppes_batch *b, int id, double *min_rho, double *min_p)
+{
+    if (!b || !b->eng.valid(id)) return PPES_BAD_ARG;
+    int rc = b->eng.pull();
+    if (rc) return rc;
+    read_positivity(b->eng.rec(id), min_rho, min_p);
+    return PPES_OK;
+}
+
+extern "C" int ppes_positivity(ppes_solver *s, double *min_rho, double *min_p)
+{
+    if (!s) return PPES_BAD_ARG;
+    int rc = s->eng.pull();
+    if (rc) return rc;
+    read_positivity(s->eng.rec(s->id), min_rho, min_p);
     return PPES_OK;
 }
 

@@ -65,7 +65,30 @@ struct Ctl {
     int state, status, flag_last, retry_residual;
     int oedg_mode, use_ldf, ent_offset, pad1;
     long long steps, retries, negcells, stages, max_steps;
+    // smallest density and pressure at any GLL node of the interior after any
+    // stage's post-process since the last reset, as order-preserving keys
+    // (pos_key); a NaN counts as below every number
+    unsigned long long min_rho_key, min_p_key;
 };
+
+// Order-preserving map of a double onto an unsigned key (x < y <=> key(x) <
+// key(y)), NaN mapped below everything, and its inverse.
+__host__ __device__ inline unsigned long long pos_key(double x)
+{
+    if (!(x == x)) return 0ULL;
+    unsigned long long b;
+    __builtin_memcpy(&b, &x, sizeof b);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double pos_unkey(unsigned long long k)
+{
+    if (k == 0ULL) return __builtin_nan("");
+    const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
+    double x;
+    __builtin_memcpy(&x, &b, sizeof x);
+    return x;
+}
+constexpr unsigned long long POS_KEY_NONE = ~0ULL;  // no node recorded yet
 
 struct Inst {
     int nx, ny, px, py, S, pad_;

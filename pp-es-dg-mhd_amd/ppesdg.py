@@ -108,6 +108,8 @@ def lib():
             "ppes_batch_step": [vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ip],
             "ppes_batch_status": [vp, ctypes.c_int, ip, ctypes.POINTER(RunStats)],
             "ppes_batch_memory": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)],
+            "ppes_positivity": [vp, dp, dp],
+            "ppes_batch_positivity": [vp, ctypes.c_int, dp, dp],
             "ppes_batch_set_timing": [vp, ctypes.c_int],
             "ppes_batch_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
         }
@@ -301,6 +303,13 @@ class Solver:
         _check(lib().ppes_semidiscretize(self.h, ut_slot, u_slot, ctypes.byref(dtl)), "ppes_semidiscretize")
         return dtl.value
 
+    def positivity(self):
+        """(min density, min pressure) over every interior GLL node of every
+        post-processed stage since the solve started."""
+        r, p = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ppes_positivity(self.h, ctypes.byref(r), ctypes.byref(p)), "ppes_positivity")
+        return r.value, p.value
+
     def rk_combine(self, stage, dt):
         _check(lib().ppes_rk_combine(self.h, stage, float(dt)), "ppes_rk_combine")
 
@@ -415,6 +424,13 @@ class Batch:
         d = _stats(st)
         d["state"] = INST_STATE[stt.value]
         return d
+
+    def positivity(self, i):
+        """(min density, min pressure) of instance i over every interior GLL
+        node of every post-processed stage since its start."""
+        r, p = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ppes_batch_positivity(self.h, int(i), ctypes.byref(r), ctypes.byref(p)), "ppes_batch_positivity")
+        return r.value, p.value
 
     def memory(self, i):
         b = ctypes.c_longlong()

@@ -1,6 +1,9 @@
 """The benchmark configurations at their full sizes, where the oracle cannot
 follow, through the properties the method guarantees whatever the size:
-exact positivity of density and pressure at every GLL node, mass
+exact positivity of density and pressure at every GLL node of every RK
+stage (the device's positivity record: the minimum over every interior node
+of every post-processed stage, checked against host minima in
+test_gpu_positivity.py), mass
 conservation to round-off on a periodic domain, and an instance giving the
 same bits alone and inside the 64-instance batch.
 
@@ -44,13 +47,16 @@ def test_ot_perturbed_p3_1024_mass_and_positivity():
     m0 = mass(s.download_state(0), p, h, h)
     st = s.solve(-1.0, 200, 0.2, 1, True)
     u = s.download_state(0)
+    srho, sp = s.positivity()
     s.close()
     m1 = mass(u, p, h, h)
     rmin, pmin = rho_prs_min(u, g)
-    print(f"OT 1024^2 P3: mass {m0!r} -> {m1!r} (rel {(m1 - m0) / m0:.2e}), min rho {rmin}, min p {pmin}")
+    print(f"OT 1024^2 P3: mass {m0!r} -> {m1!r} (rel {(m1 - m0) / m0:.2e}), min rho {rmin}, min p {pmin}; "
+          f"over all 600 stages: min rho {srho}, min p {sp}")
     assert st["status"] == 0 and st["steps"] == 200
     assert abs(m1 - m0) <= 1e-13 * abs(m0)
     assert rmin > 0.0 and pmin > 0.0
+    assert 0.0 < srho <= rmin and 0.0 < sp <= pmin
 
 
 def test_blast_p2_400_positivity_to_final_time():
@@ -62,18 +68,21 @@ def test_blast_p2_400_positivity_to_final_time():
     s.init_problem("blast")
     st = s.solve(0.01, 0, 0.15, 2, True)
     u = s.download_state(0)
+    srho, sp = s.positivity()
     s.close()
     rmin, pmin = rho_prs_min(u, g)
     print(f"blast 400^2 P2: {st['steps']} steps, {st['retries']} retries, {st['negcells']} limited cells, "
-          f"min rho {rmin}, min p {pmin}")
+          f"final min rho {rmin}, min p {pmin}; over every stage: min rho {srho}, min p {sp}")
     assert st["status"] == 0 and abs(st["t"] - 0.01) <= 1e-15
     assert rmin >= PP_EPS and pmin > 0.0
+    # every stage of every step: density at least the limiter's floor, pressure positive
+    assert srho >= PP_EPS and sp > 0.0
 
 
 def test_mixed_batch_full_size_alone_equals_in_batch():
-    """The 64 uneven instances of the batched benchmark, 20 steps together:
-    every node of every instance positive; the largest P4 instance and the
-    smallest instance give the same bits alone."""
+    """The 64 uneven instances of the batched benchmark, their full 100 steps
+    together: every node of every instance positive after every stage; the
+    largest P4 instance and the smallest instance give the same bits alone."""
     mix = W.mixed_batch(64, seed=1)
     b = ppesdg.Batch(3)
     ids = []
@@ -81,12 +90,16 @@ def test_mixed_batch_full_size_alone_equals_in_batch():
         dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[m["prob"]]
         i = b.add(m["nx"], m["ny"], m["p"], dom, g, bc, bcs)
         b.init_problem(i, m["prob"])
-        b.start(i, -1.0, 20, cfl, oedg, True)
+        b.start(i, -1.0, 100, cfl, oedg, True)
         ids.append(i)
-    b.step(20, True)
+    b.step(100, True)
     for m, i in zip(mix, ids):
+        st = b.status(i)
+        assert st["state"] == "done" and st["steps"] == 100, (m, st)
         rmin, pmin = rho_prs_min(b.download_state(i, 0), W.PROBLEMS[m["prob"]][1])
+        srho, sp = b.positivity(i)
         assert rmin > 0.0 and pmin > 0.0, (m, rmin, pmin)
+        assert 0.0 < srho <= rmin and 0.0 < sp <= pmin, (m, srho, sp)
     big = max(range(len(mix)), key=lambda q: (mix[q]["p"], W.nodes(mix[q])))
     small = min(range(len(mix)), key=lambda q: W.nodes(mix[q]))
     for q in (big, small):
@@ -94,7 +107,7 @@ def test_mixed_batch_full_size_alone_equals_in_batch():
         dom, g, bc, bcs, cfl, oedg = W.PROBLEMS[m["prob"]]
         s = ppesdg.Solver(m["nx"], m["ny"], m["p"], dom, g, bc, bcs)
         s.init_problem(m["prob"])
-        s.solve(-1.0, 20, cfl, oedg, True)
+        s.solve(-1.0, 100, cfl, oedg, True)
         alone = s.download_state(0)[:, :, :, 1:-1, 1:-1]
         s.close()
         assert np.array_equal(alone, b.download_state(ids[q], 0)[:, :, :, 1:-1, 1:-1]), m
