@@ -133,6 +133,48 @@ constexpr int resid_waves(int p, bool fast) { return !fast ? 1 : (p == 4 ? PPES_
 #endif
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Maximum over the NN consecutive lanes of one element (node n of the
+// element at wavefront lane `lane`), returned to every lane of the element;
+// the element must lie inside one wavefront. A power-of-two NN is an aligned
+// group: a butterfly of lane exchanges. Otherwise a window doubling by
+// rotation inside the element's lanes: after the exchange at offset o each
+// lane holds the maximum over 2o cyclically consecutive nodes, so offsets
+// 1, 2, 4, ... below NN cover the element. Exact (max) in any order.
+template <int NN>
+__device__ __forceinline__ double seg_max(double x, int lane, int n)
+{
+    if constexpr ((NN & (NN - 1)) == 0) {
+#pragma unroll
+        for (int m = 1; m < NN; m <<= 1) x = dmax(x, __shfl_xor(x, m));
+    } else {
+        const int base = lane - n;
+#pragma unroll
+        for (int off = 1; off < NN; off <<= 1) {
+            const int r = n + off >= NN ? n + off - NN : n + off;
+            x = dmax(x, __shfl(x, base + r));
+        }
+    }
+    return x;
+}
+
+// Minimum of a non-negative double over each wavefront (a butterfly of lane
+// exchanges), folded into *bits (the double's bit pattern, which orders
+// non-negative doubles) by one atomic per wavefront, issued only when it is
+// below the value read first: one address per instance would otherwise
+// serialise every workgroup's atomic at the L2, while a value that only
+// falls makes almost every later wavefront skip it (a stale read costs an
+// extra atomic, never a wrong result). Called by every lane of the wavefront.
+__device__ __forceinline__ void wave_min_atomic(unsigned long long *bits, double x)
+{
+    if (!(x == x)) x = DBL_HUGE;  // a NaN never lowers the minimum (as its bit pattern would not)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = dmin(x, __shfl_xor(x, m));
+    if ((threadIdx.x & 63) == 0 && x < DBL_HUGE) {
+        const unsigned long long k = (unsigned long long)__double_as_longlong(x);
+        if (k < *bits) atomicMin(bits, k);
+    }
+}
+
 template <int PX, int PY, bool FAST, int TB = resid_tb((PX + 1) * (PY + 1)), int FLUX = FLUX_ESPP_HLL>
 __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_waves(PX == PY ? PX : 0, FAST)))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
@@ -141,7 +183,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     constexpr int XP = S::XP, HX = S::HX, HY = S::HY, ROUNDS = S::ROUNDS;
     __shared__ Prim sprim[EPB][NN];
     __shared__ double sbuf[EPB][(FAST && PPES_FAST_ROUND) ? S::BUF_ROUND : S::BUF];
-    __shared__ double sred[TB][2];
+    // the element maxima of the signal speeds go by lane exchanges when every
+    // element lies inside one wavefront, else through LDS
+    constexpr bool SEG = (TB == 64) || (64 % NN == 0);
+    __shared__ double sred[SEG ? 1 : TB][2];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -510,16 +555,24 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 #pragma unroll
         for (int v = 0; v < 8; ++v) (ut + v * vsi)[ob] = acc[v];
     }
-    // ---- element dt_lim (DG_solver.f90:806); max is exact in any order
-    sred[t][0] = ax;
-    sred[t][1] = ay;
-    __syncthreads();
-    if (valid && n == 0) {
-        double mx_ = 0.0, my_ = 0.0;
-        for (int m = 0; m < NN; ++m) { mx_ = dmax(mx_, sred[el * NN + m][0]); my_ = dmax(my_, sred[el * NN + m][1]); }
-        const double dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
-        atomicMin(&I.ctl.dtlim_bits, (unsigned long long)__double_as_longlong(dtl));
+    // ---- element dt_lim (DG_solver.f90:806): the element's maximum signal
+    // speeds, its dt, the wavefront's minimum dt, one atomic per wavefront
+    // (max and min are exact in any order, so dt_lim is the reference's)
+    double dtl = DBL_HUGE;
+    if constexpr (SEG) {
+        const double mx_ = seg_max<NN>(ax, t & 63, n), my_ = seg_max<NN>(ay, t & 63, n);
+        if (valid && n == 0) dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
+    } else {
+        sred[t][0] = ax;
+        sred[t][1] = ay;
+        __syncthreads();
+        if (valid && n == 0) {
+            double mx_ = 0.0, my_ = 0.0;
+            for (int m = 0; m < NN; ++m) { mx_ = dmax(mx_, sred[el * NN + m][0]); my_ = dmax(my_, sred[el * NN + m][1]); }
+            dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
+        }
     }
+    wave_min_atomic(&I.ctl.dtlim_bits, dtl);
 }
 
 // --------------------------------------------------------------------------
@@ -602,8 +655,12 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
 // carry a final limited state (on = true), recorded in the instance's
 // positivity keys: a butterfly of lane exchanges across each 64-lane
 // wavefront (elements may straddle wavefronts; the minimum does not care),
-// then one atomic per wavefront and quantity. A NaN enters as -inf, below
-// every number. Called by every thread of the workgroup.
+// then an atomic minimum per wavefront and quantity, issued only when the
+// wavefront's value is below the record it reads first: the record only
+// falls, so after the first few wavefronts of a launch almost none issue
+// one (every workgroup of an instance updating one address serialises at
+// the L2 otherwise; a stale read only costs an extra atomic). A NaN enters as
+// -inf, below every number. Called by every thread of the workgroup.
 __device__ __forceinline__ void record_positivity(Inst &I, bool on, const double q[8])
 {
     double r = DBL_HUGE, p = DBL_HUGE;
@@ -619,8 +676,9 @@ __device__ __forceinline__ void record_positivity(Inst &I, bool on, const double
         p = dmin(p, __shfl_xor(p, m));
     }
     if ((threadIdx.x & 63) == 0 && (r < DBL_HUGE || p < DBL_HUGE)) {
-        atomicMin(&I.ctl.min_rho_key, pos_key(r));
-        atomicMin(&I.ctl.min_p_key, pos_key(p));
+        const unsigned long long kr = pos_key(r), kp = pos_key(p);
+        if (kr < I.ctl.min_rho_key) atomicMin(&I.ctl.min_rho_key, kr);
+        if (kp < I.ctl.min_p_key) atomicMin(&I.ctl.min_p_key, kp);
     }
 }
 
