@@ -427,27 +427,129 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
                 if (ll > l) sub_y(cy * DY(ll), &sbuf[el][(XP + k * HY + ll * (ll - 1) / 2 + l) * REC]);
         }
     } else {
-        // one round's records at a time (half the LDS of the ordered form):
-        // after round c every node adds the pairs of that round it belongs to
+        // Statically scheduled pair rounds (one record per lane in LDS at a
+        // time). In an x-round at distance d, lane (k, l) evaluates the pair
+        // (k, (k + d) mod NX) of its row; its node then adds that record and
+        // the one lane ((k - d) mod NX, l) evaluated, the pair
+        // ((k - d) mod NX, k). Distances 1 .. (NX - 1) / 2 cover every pair
+        // of a row once; y-rounds likewise along columns. An even node count
+        // leaves the pairs at distance N / 2, which half the lanes can
+        // evaluate: with NX = NY even, lanes k < NX / 2 take the x-pairs
+        // (k, k + NX / 2) of their row and lanes k >= NX / 2 the y-pairs
+        // (k - NX / 2, k) of column l, in one round. Every node reads exactly
+        // the two records it belongs to (x and y in the merged round): no
+        // search over partners. Each pair is evaluated with the higher node
+        // first, as in the other forms; the node sums its records with fused
+        // multiply-adds, round by round.
+        constexpr int XD = (NX - 1) / 2, YD = NY > 1 ? (NY - 1) / 2 : 0;
+        constexpr bool XHALF = NX % 2 == 0, YHALF = NY > 1 && NY % 2 == 0;
+        static_assert(!YHALF || (XHALF && NX == NY), "an even y node count comes with the same even x count");
+        // row entry r of Dx / Dy for a lane-dependent r (a select over the
+        // row held in registers, or the LDS copy)
+        auto dx_at = [&](int r) -> double {
+            if constexpr (DREG) {
+                double v = dxr[0];
 #pragma unroll
-        for (int c = 0; c < ROUNDS; ++c) {
-            if (valid && n + c * NN < S::PAIRS) pair_flux(n + c * NN, &sbuf[el][n * REC]);
+                for (int q = 1; q < NX; ++q) v = (q == r) ? dxr[q] : v;
+                return v;
+            } else {
+                return sdrow[k + NX * r];
+            }
+        };
+        auto dy_at = [&](int s2) -> double {
+            if constexpr (DREG) {
+                double v = dyr[0];
+#pragma unroll
+                for (int q = 1; q < NY; ++q) v = (q == s2) ? dyr[q] : v;
+                return v;
+            } else {
+                return sdrow[NX * NX + l + NY * s2];
+            }
+        };
+        // the record of the pair of nodes nA (the higher one along the pair's
+        // direction) and nB; a y-pair is the x-flux of the x/y-swapped
+        // primitives, swapped back (x-record (0,1,2,3,5,6,7), y-record (0,1,2,3,4,6,7))
+        auto pair_rec = [&](int nA, int nB, bool ydir, double *o) {
+            Prim A = sprim[el][nA], B = sprim[el][nB];
+            if (ydir) {
+                double x;
+                x = A.vx; A.vx = A.vy; A.vy = x;
+                x = A.bx; A.bx = A.by; A.by = x;
+                x = B.vx; B.vx = B.vy; B.vy = x;
+                x = B.bx; B.bx = B.by; B.by = x;
+            }
+            double ec[8];
+            ec_flux_fast<0>(A, B, inv_gm1, ec);
+            o[0] = ec[0];
+            o[1] = ydir ? ec[2] : ec[1];
+            o[2] = ydir ? ec[1] : ec[2];
+            o[3] = ec[3];
+            o[4] = ec[5];
+            o[5] = ec[6];
+            o[6] = ec[7];
+        };
+        auto fsub_x = [&](double a, const double *o) {
+            acc[0] = __builtin_fma(-a, o[0], acc[0]); acc[1] = __builtin_fma(-a, o[1], acc[1]);
+            acc[2] = __builtin_fma(-a, o[2], acc[2]); acc[3] = __builtin_fma(-a, o[3], acc[3]);
+            acc[5] = __builtin_fma(-a, o[4], acc[5]); acc[6] = __builtin_fma(-a, o[5], acc[6]);
+            acc[7] = __builtin_fma(-a, o[6], acc[7]);
+        };
+        auto fsub_y = [&](double a, const double *o) {
+            acc[0] = __builtin_fma(-a, o[0], acc[0]); acc[1] = __builtin_fma(-a, o[1], acc[1]);
+            acc[2] = __builtin_fma(-a, o[2], acc[2]); acc[3] = __builtin_fma(-a, o[3], acc[3]);
+            acc[4] = __builtin_fma(-a, o[4], acc[4]); acc[6] = __builtin_fma(-a, o[5], acc[6]);
+            acc[7] = __builtin_fma(-a, o[6], acc[7]);
+        };
+        const int hx2 = NX / 2, hy2 = NY / 2;
+        int nround = 0;
+        // full x-rounds
+#pragma unroll
+        for (int d = 1; d <= XD; ++d) {
+            if (nround++) __syncthreads();
+            const int kp = k + d < NX ? k + d : k + d - NX, km = k >= d ? k - d : k - d + NX;
+            if (valid) pair_rec((kp > k ? kp : k) + NX * l, (kp > k ? k : kp) + NX * l, false, &sbuf[el][n * REC]);
             __syncthreads();
             if (valid) {
-#pragma unroll
-                for (int r = 0; r < NX; ++r) {
-                    if (r == k) continue;
-                    const int qp = l * HX + (r > k ? r * (r - 1) / 2 + k : k * (k - 1) / 2 + r);
-                    if (qp / NN == c) sub_x(cx * DX(r), &sbuf[el][(qp - c * NN) * REC]);
-                }
-#pragma unroll
-                for (int s2 = 0; s2 < NY; ++s2) {
-                    if (s2 == l) continue;
-                    const int qp = XP + k * HY + (s2 > l ? s2 * (s2 - 1) / 2 + l : l * (l - 1) / 2 + s2);
-                    if (qp / NN == c) sub_y(cy * DY(s2), &sbuf[el][(qp - c * NN) * REC]);
-                }
+                fsub_x(cx * dx_at(kp), &sbuf[el][n * REC]);
+                fsub_x(cx * dx_at(km), &sbuf[el][(km + NX * l) * REC]);
             }
-            if (c + 1 < ROUNDS) __syncthreads();
+        }
+        // full y-rounds
+#pragma unroll
+        for (int d = 1; d <= YD; ++d) {
+            if (nround++) __syncthreads();
+            const int lp = l + d < NY ? l + d : l + d - NY, lm = l >= d ? l - d : l - d + NY;
+            if (valid) pair_rec(k + NX * (lp > l ? lp : l), k + NX * (lp > l ? l : lp), true, &sbuf[el][n * REC]);
+            __syncthreads();
+            if (valid) {
+                fsub_y(cy * dy_at(lp), &sbuf[el][n * REC]);
+                fsub_y(cy * dy_at(lm), &sbuf[el][(k + NX * lm) * REC]);
+            }
+        }
+        if constexpr (XHALF && YHALF) {
+            // the merged half round: x-pairs (k, k + h) on lanes k < h, y-pairs
+            // (k - h, k) of column l on lanes k >= h
+            if (nround++) __syncthreads();
+            // the pair chosen per lane, then one flux evaluation for both kinds
+            const bool yd = k >= hx2;
+            const int nA = yd ? l + NX * k : (k + hx2) + NX * l;
+            const int nB = yd ? l + NX * (k - hx2) : k + NX * l;
+            if (valid) pair_rec(nA, nB, yd, &sbuf[el][n * REC]);
+            __syncthreads();
+            if (valid) {
+                // x: the pair (min, min + h) of row l was evaluated by lane (min, l)
+                fsub_x(cx * dx_at(k < hx2 ? k + hx2 : k - hx2), &sbuf[el][((k < hx2 ? k : k - hx2) + NX * l) * REC]);
+                // y: the pair (lo, lo + h) of column k was evaluated by lane (h + lo, k)
+                const int lo = l < hy2 ? l : l - hy2;
+                fsub_y(cy * dy_at(l < hy2 ? l + hy2 : l - hy2), &sbuf[el][((hy2 + lo) + NX * k) * REC]);
+            }
+        } else if constexpr (XHALF) {
+            // one row (NY = 1): the x-pairs (k, k + h) on lanes k < h
+            if (nround++) __syncthreads();
+            if (valid && k < hx2) pair_rec((k + hx2) + NX * l, k + NX * l, false, &sbuf[el][n * REC]);
+            __syncthreads();
+            if (valid)
+                fsub_x(cx * dx_at(k < hx2 ? k + hx2 : k - hx2), &sbuf[el][((k < hx2 ? k : k - hx2) + NX * l) * REC]);
         }
     }
     __syncthreads();  // pair records consumed; the buffer now holds face data
