@@ -277,8 +277,9 @@ int ppes_ldf_modal_operator(int p, double *op);
  * flux: PPES_FLUX_*; arith: PPES_ARITH_*. *dt_lim (host) receives dt_lim.
  * Runs on `stream` (a hipStream_t; NULL: the null stream) and waits for it.
  * Its only device memory beyond u and ut is a fixed workspace per GPU (one
- * instance record and the residual block map, PPES_NOALLOC_MAX_BLOCKS
- * workgroups at most: grids that need more return PPES_BAD_ARG), held until
+ * instance record, the residual block map of PPES_NOALLOC_MAX_BLOCKS
+ * workgroups at most -- grids that need more return PPES_BAD_ARG -- and the
+ * 32 cells of the dt_lim minimum, 2.1 MiB in all), held until
  * the process ends. It calls no hipMalloc or hipFree. Calls on one GPU share
  * that workspace, so calls that overlap on one GPU race. */
 #define PPES_NOALLOC_MAX_BLOCKS 262144

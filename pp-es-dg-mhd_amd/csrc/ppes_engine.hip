@@ -443,6 +443,9 @@ int Engine::add(const ppes_config &cfg, int *id)
     ok = ok && alloc(&r.entpart, (size_t)std::max(r.nblk_el, r.nblk_slot));
     ok = ok && alloc(&r.ghost0, 8 * (size_t)(2 * nx + 2 * ny) * NN);
     ok = ok && alloc(&r.cent, 2 * (size_t)(nx + 2) * (ny + 2));
+    double *dts = nullptr;
+    ok = ok && alloc(&dts, (size_t)DT_SLOTS * DT_SLOT_STRIDE);
+    r.dtslot = reinterpret_cast<unsigned long long *>(dts);
     hosts[slot] = h;
     if (!ok) {
         remove(slot);
@@ -452,6 +455,10 @@ int Engine::add(const ppes_config &cfg, int *id)
     host_ldf_nodal(h->px, h->py, h->hx, h->hy, op.data());
     ENG_TRY(hipMemcpyAsync(r.ldf, op.data(), op.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     for (int k = 0; k <= S; ++k) ENG_TRY(hipMemsetAsync(r.u[k], 0, nstate * sizeof(double), stream));
+    {
+        const std::vector<unsigned long long> none((size_t)DT_SLOTS * DT_SLOT_STRIDE, DT_NONE);
+        ENG_TRY(hipMemcpy(r.dtslot, none.data(), none.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    }
     ENG_TRY(hipMemsetAsync(r.uavg, 0, 8 * (size_t)(nx + 2) * (ny + 2) * sizeof(double), stream));
     rc = push();
     if (rc) return rc;
@@ -518,6 +525,33 @@ static void xcd_order(std::vector<Blk> &v)
     v.swap(out);
 }
 
+// Residual blocks of one instance (nres blocks of epb elements in row-major
+// element order) in column-strip order: strips of sw elements, sw sized so
+// that one strip row of state and residual (nn nodes of 64 B each, read and
+// written) is about 256 KB, each strip walked row by row, blocks in a strip
+// row left to right (a block belongs to the strip row of its first element).
+// The residual reads its neighbours' face traces; within a strip the row
+// above comes back while it is still in the XCD's L2, where a whole grid row
+// (1024 P3 elements: 2 MB) between a row and its neighbour would have left it.
+// A permutation of the blocks: results do not depend on it.
+static void strip_order(std::vector<Blk> &out, int id, long long nres, int nx, int ny, int epb, int nn)
+{
+    const long long sw = std::max<long long>(epb, (256LL * 1024) / ((long long)nn * 64 * 2));
+    if (nx <= sw) {
+        for (long long b = 0; b < nres; ++b) out.push_back({id, (int)b});
+        return;
+    }
+    const long long E = (long long)nx * ny;
+    for (long long s0 = 0; s0 < nx; s0 += sw) {
+        const long long s1 = std::min<long long>(s0 + sw, nx);
+        for (long long j = 0; j < ny; ++j) {
+            // blocks whose first element b * epb lies in [j nx + s0, j nx + s1)
+            const long long b0 = cdiv(j * nx + s0, epb), b1 = cdiv(std::min(j * nx + s1, E), epb);
+            for (long long b = b0; b < b1; ++b) out.push_back({id, (int)b});
+        }
+    }
+}
+
 int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool)
 {
     g.sid = sid;
@@ -531,7 +565,7 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapP
         const int NN = (sid_px(sid) + 1) * (sid_py(sid) + 1);
         for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
         const long long nres = cdiv((long long)r.nx * r.ny, (long long)(g.res_tb / NN));
-        for (long long b = 0; b < nres; ++b) rsf[hosts[id]->flux].push_back({id, (int)b});
+        strip_order(rsf[hosts[id]->flux], id, nres, r.nx, r.ny, g.res_tb / NN, NN);
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
         // x-face workgroups, then y-face ones (the kernel's split)
         const long long nf = cdiv((long long)(r.nx + 1) * r.ny * 8, BLOCK) + cdiv((long long)r.nx * (r.ny + 1) * 8, BLOCK);
@@ -860,6 +894,14 @@ int Engine::autotune(int reps, int *tb_out)
     if (rc) return rc;
     if (arith != PPES_ARITH_FAST || n_all == 0) return PPES_OK;
     reps = std::max(1, reps);
+    // The timing runs recompute L(u^n) into residual slot 1 and fold their
+    // dt_lim into the records. Between committed steps both equal what the
+    // last stage left; an instance with a retry pending would instead have
+    // its retry residual and dt_lim disturbed: then tune nothing. The records
+    // (dt_lim included) are put back afterwards.
+    if ((rc = pull())) return rc;
+    for (size_t i = 0; i < hosts.size(); ++i)
+        if (hosts[i] && h_recs[i].ctl.state == ST_ACTIVE && h_recs[i].ctl.retry_residual) return PPES_OK;
     ENG_TRY(hipStreamSynchronize(stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ENG_TRY(hipEventCreate(&e0));
@@ -878,7 +920,7 @@ int Engine::autotune(int reps, int *tb_out)
             for (int id : g.members) {
                 const Inst &r = h_recs[id];
                 const long long nres = cdiv((long long)r.nx * r.ny, (long long)(tb / NN));
-                for (long long b2 = 0; b2 < nres; ++b2) rs.push_back({id, (int)b2});
+                strip_order(rs, id, nres, r.nx, r.ny, tb / NN, NN);
             }
             xcd_order(rs);
             Blk *d = nullptr;
@@ -910,6 +952,7 @@ int Engine::autotune(int reps, int *tb_out)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (!err) err = push();  // the records as they were before the timing runs
     // the next launch rebuilds the plan with the chosen sizes
     built_key.clear();
     plan_ok = false;
@@ -1013,6 +1056,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
         for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         LAUNCH1(k_pre_residual, n, d_recs, (const int *)d_ids, n, 0);
         for (auto &g : tmp) RESIDUAL_DISPATCH(g, 0, 1, 0);
+        LAUNCH1(k_fold_dt, n, d_recs, (const int *)d_ids, n, 0);
     }
     (void)hipStreamSynchronize(stream);  // (tmp's maps live in start_pools)
     (void)hipFree(d_ids);
@@ -1062,6 +1106,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
         RESIDUAL_DISPATCH(g, slot, ut_slot, 0);
+        LAUNCH1(k_fold_dt, 1, d_recs, (const int *)d_ids, 1, 0);
         break;
     case API_POST: {
         const bool eo = h_recs[id].ctl.ent_offset != 0;
@@ -1095,13 +1140,15 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
 //
 // ppes_semidiscretize_noalloc works in a fixed device workspace: one instance
 // record (the caller's buffer pointers, sizes, tables and the dt_lim cell),
-// then the residual's block map, at most NOALLOC_MAX_BLOCKS workgroups. The
+// then the residual's block map, at most NOALLOC_MAX_BLOCKS workgroups, then
+// the dt_lim slots (reset by a kernel before each residual). The
 // array is `scratch`; each device holds its own copy until the process ends.
 // Host code never uses the array's name as an address: hipGetSymbolAddress
 // gives each device's address once, kept in scratch_dev.
 constexpr long long NOALLOC_MAX_BLOCKS = PPES_NOALLOC_MAX_BLOCKS;
 constexpr size_t NOALLOC_REC_BYTES = (sizeof(Inst) + 255) / 256 * 256;
-constexpr size_t NOALLOC_BYTES = NOALLOC_REC_BYTES + (size_t)NOALLOC_MAX_BLOCKS * sizeof(Blk);
+constexpr size_t NOALLOC_MAP_BYTES = (size_t)NOALLOC_MAX_BLOCKS * sizeof(Blk);
+constexpr size_t NOALLOC_BYTES = NOALLOC_REC_BYTES + NOALLOC_MAP_BYTES + (size_t)DT_SLOTS * DT_SLOT_STRIDE * 8;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 constexpr int NOALLOC_MAX_DEVICES = 64;
@@ -1170,6 +1217,7 @@ int residual_noalloc(const ppes_config &cfg, int flux, int arith, const double *
     if (rc) return rc;
     Inst *d_recs = reinterpret_cast<Inst *>(ws);
     Blk *d_map = reinterpret_cast<Blk *>(ws + NOALLOC_REC_BYTES);
+    unsigned long long *d_slots = reinterpret_cast<unsigned long long *>(ws + NOALLOC_REC_BYTES + NOALLOC_MAP_BYTES);
 
     // the record starts each call afresh, dt_lim cell included
     Inst rec;
@@ -1178,9 +1226,12 @@ int residual_noalloc(const ppes_config &cfg, int flux, int arith, const double *
     rec.u[0] = const_cast<double *>(u);
     rec.ut[1] = ut;
     rec.ctl.state = ST_ACTIVE;
-    rec.ctl.dtlim_bits = 0x7FF0000000000000ULL;
+    rec.ctl.dtlim_bits = DT_NONE;
+    rec.dtslot = d_slots;
     ENG_TRY(hipMemcpyAsync(d_recs, &rec, sizeof(Inst), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_noalloc_map, dim3((unsigned)cdiv(nblk, BLOCK)), dim3(BLOCK), 0, stream, d_map, nblk);
+    ENG_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_pre_residual, dim3(1), dim3(64), 0, stream, d_recs, (const int *)nullptr, 1, 0);
     ENG_TRY(hipGetLastError());
     const Blk *m_ = d_map;
     switch (flux) {
@@ -1189,6 +1240,8 @@ int residual_noalloc(const ppes_config &cfg, int flux, int arith, const double *
     case FLUX_EC_LF: RESIDUAL_DISPATCH_F(sid, FLUX_EC_LF, tb, nblk, d_recs, m_, 0, 1, 0); break;
     default: RESIDUAL_DISPATCH_F(sid, FLUX_ESPP_HLL, tb, nblk, d_recs, m_, 0, 1, 0); break;
     }
+    hipLaunchKernelGGL(k_fold_dt, dim3(1), dim3(64), 0, stream, d_recs, (const int *)nullptr, 1, 0);
+    ENG_TRY(hipGetLastError());
     unsigned long long bits = 0;
     ENG_TRY(hipMemcpyAsync(&bits, &d_recs->ctl.dtlim_bits, sizeof bits, hipMemcpyDeviceToHost, stream));
     ENG_TRY(hipStreamSynchronize(stream));

@@ -115,7 +115,7 @@ __device__ __forceinline__ void rk_combine_node(const StageSrc &c, long long os,
 // Build-time shape of the fast residual: waves per SIMD asked of the register
 // allocator, early trace loads, one-round pair buffer.
 #ifndef PPES_FAST_WAVES
-#define PPES_FAST_WAVES 4  // P4 (4 waves with a short spill beat 3 without one)
+#define PPES_FAST_WAVES 3  // P4: 3 waves without a spill (4 waves spill the face flux's registers and run slower)
 #endif
 #ifndef PPES_FAST_WAVES_LOW
 #define PPES_FAST_WAVES_LOW 1  // P1-P3
@@ -160,10 +160,10 @@ __device__ __forceinline__ double seg_max(double x, int lane, int n)
 // Minimum of a non-negative double over each wavefront (a butterfly of lane
 // exchanges), folded into *bits (the double's bit pattern, which orders
 // non-negative doubles) by one atomic per wavefront, issued only when it is
-// below the value read first: one address per instance would otherwise
-// serialise every workgroup's atomic at the L2, while a value that only
-// falls makes almost every later wavefront skip it (a stale read costs an
-// extra atomic, never a wrong result). Called by every lane of the wavefront.
+// below the cell's current value, read first past the L1 (a cached copy
+// would be stale): a minimum that only falls makes almost every later
+// wavefront skip the atomic (a stale read costs an extra atomic, never a
+// wrong result). Called by every lane of the wavefront.
 __device__ __forceinline__ void wave_min_atomic(unsigned long long *bits, double x)
 {
     if (!(x == x)) x = DBL_HUGE;  // a NaN never lowers the minimum (as its bit pattern would not)
@@ -171,7 +171,7 @@ __device__ __forceinline__ void wave_min_atomic(unsigned long long *bits, double
     for (int m = 32; m >= 1; m >>= 1) x = dmin(x, __shfl_xor(x, m));
     if ((threadIdx.x & 63) == 0 && x < DBL_HUGE) {
         const unsigned long long k = (unsigned long long)__double_as_longlong(x);
-        if (k < *bits) atomicMin(bits, k);
+        if (k < __hip_atomic_load(bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(bits, k);
     }
 }
 
@@ -233,15 +233,20 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     // in global memory, so an indexed read inside the pair loops would wait
     // on memory at every use). They stay in registers, or, for elements of
     // more than 16 nodes, whose registers are all taken, in LDS.
+    // The LDS copy of Dx and Dy also serves the fast pair rounds' entries of
+    // a lane-dependent column (read from registers by a select, the compiler
+    // makes a stack array indexed in scratch memory).
     constexpr bool DREG = NN <= 16;
-    __shared__ double sdrow[DREG ? 1 : NX * NX + NY * NY];
+    constexpr bool DLDS = !DREG || (FAST && PPES_FAST_ROUND);
+    __shared__ double sdrow[DLDS ? NX * NX + NY * NY : 1];
     double dxr[DREG ? NX : 1], dyr[DREG ? NY : 1];
     if constexpr (DREG) {
 #pragma unroll
         for (int r = 0; r < NX; ++r) dxr[r] = I.Dx[k + NX * r];
 #pragma unroll
         for (int s = 0; s < NY; ++s) dyr[s] = I.Dy[l + NY * s];
-    } else {
+    }
+    if constexpr (DLDS) {
         for (int a = t; a < NX * NX + NY * NY; a += TB) sdrow[a] = a < NX * NX ? I.Dx[a] : I.Dy[a - NX * NX];
     }
     auto DX = [&](int r) -> double {
@@ -444,28 +449,9 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         constexpr int XD = (NX - 1) / 2, YD = NY > 1 ? (NY - 1) / 2 : 0;
         constexpr bool XHALF = NX % 2 == 0, YHALF = NY > 1 && NY % 2 == 0;
         static_assert(!YHALF || (XHALF && NX == NY), "an even y node count comes with the same even x count");
-        // row entry r of Dx / Dy for a lane-dependent r (a select over the
-        // row held in registers, or the LDS copy)
-        auto dx_at = [&](int r) -> double {
-            if constexpr (DREG) {
-                double v = dxr[0];
-#pragma unroll
-                for (int q = 1; q < NX; ++q) v = (q == r) ? dxr[q] : v;
-                return v;
-            } else {
-                return sdrow[k + NX * r];
-            }
-        };
-        auto dy_at = [&](int s2) -> double {
-            if constexpr (DREG) {
-                double v = dyr[0];
-#pragma unroll
-                for (int q = 1; q < NY; ++q) v = (q == s2) ? dyr[q] : v;
-                return v;
-            } else {
-                return sdrow[NX * NX + l + NY * s2];
-            }
-        };
+        // row entry r of Dx / Dy for a lane-dependent r, from the LDS copy
+        auto dx_at = [&](int r) -> double { return sdrow[k + NX * r]; };
+        auto dy_at = [&](int s2) -> double { return sdrow[NX * NX + l + NY * s2]; };
         // the record of the pair of nodes nA (the higher one along the pair's
         // direction) and nB; a y-pair is the x-flux of the x/y-swapped
         // primitives, swapped back (x-record (0,1,2,3,5,6,7), y-record (0,1,2,3,4,6,7))
@@ -674,7 +660,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
         }
     }
-    wave_min_atomic(&I.ctl.dtlim_bits, dtl);
+    wave_min_atomic(&I.dtslot[(blockIdx.x % DT_SLOTS) * DT_SLOT_STRIDE], dtl);
 }
 
 // --------------------------------------------------------------------------
@@ -779,8 +765,10 @@ __device__ __forceinline__ void record_positivity(Inst &I, bool on, const double
     }
     if ((threadIdx.x & 63) == 0 && (r < DBL_HUGE || p < DBL_HUGE)) {
         const unsigned long long kr = pos_key(r), kp = pos_key(p);
-        if (kr < I.ctl.min_rho_key) atomicMin(&I.ctl.min_rho_key, kr);
-        if (kp < I.ctl.min_p_key) atomicMin(&I.ctl.min_p_key, kp);
+        if (kr < __hip_atomic_load(&I.ctl.min_rho_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&I.ctl.min_rho_key, kr);
+        if (kp < __hip_atomic_load(&I.ctl.min_p_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&I.ctl.min_p_key, kp);
     }
 }
 
@@ -1612,17 +1600,46 @@ __global__ void k_step_begin(Inst *insts, const int *list, int n)
     rk_times(c, I.S);
 }
 
-// reset the dt_lim minimum before a residual (left alone when the stage
-// failed, so that it keeps the last completed residual's value, as dt_lim
-// does in drive.f90)
+__device__ __forceinline__ void reset_dt_slots(Inst &I)
+{
+#pragma unroll 4
+    for (int s = 0; s < DT_SLOTS; ++s) I.dtslot[s * DT_SLOT_STRIDE] = DT_NONE;
+}
+
+__device__ __forceinline__ unsigned long long fold_dt_slots(const Inst &I)
+{
+    unsigned long long m = DT_NONE;
+#pragma unroll 4
+    for (int s = 0; s < DT_SLOTS; ++s) {
+        const unsigned long long v = I.dtslot[s * DT_SLOT_STRIDE];
+        m = v < m ? v : m;
+    }
+    return m;
+}
+
+// reset the dt_lim slots before a residual (left alone when the stage
+// failed, so that they keep the last completed residual's minima, as dt_lim
+// does in drive.f90); list == nullptr: instance q
 __global__ void k_pre_residual(Inst *insts, const int *list, int n, int mode)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    Inst &I = insts[list[q]];
+    Inst &I = insts[list ? list[q] : q];
     if (!inst_live(I)) return;
     if (mode == 1 && !I.ctl.retry_residual) return;
-    I.ctl.dtlim_bits = 0x7FF0000000000000ULL;
+    reset_dt_slots(I);
+}
+
+// dt_lim of the residual the slots hold (drive.f90:104,296), for the
+// instances k_pre_residual reset with the same mode; list == nullptr: instance q
+__global__ void k_fold_dt(Inst *insts, const int *list, int n, int mode)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    Inst &I = insts[list ? list[q] : q];
+    if (!inst_live(I)) return;
+    if (mode == 1 && !I.ctl.retry_residual) return;
+    I.ctl.dtlim_bits = fold_dt_slots(I);
 }
 
 // End of an RK step (drive.f90:285-312): an inadmissible average schedules a
@@ -1635,6 +1652,9 @@ __global__ void k_step_end(Inst *insts, const int *list, int n)
     Inst &I = insts[list[q]];
     Ctl &c = I.ctl;
     if (c.state != ST_ACTIVE) return;
+    // dt_lim of the step's last completed residual (all of its stages', or
+    // up to the stage that failed)
+    c.dtlim_bits = fold_dt_slots(I);
     if (c.neg_avg) { c.retry_residual = 1; return; }
     if (c.pp_fail) { c.state = ST_FAILED; c.status = S_PP_FAIL; return; }
     double *tmp = I.u[0];
@@ -1655,7 +1675,10 @@ __global__ void k_reset_flags(Inst *insts, const int *list, int n, int mask)
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     Ctl &c = insts[list[q]].ctl;
-    if (mask & 1) c.dtlim_bits = 0x7FF0000000000000ULL;
+    if (mask & 1) {
+        c.dtlim_bits = DT_NONE;
+        reset_dt_slots(insts[list[q]]);
+    }
     if (mask & 2) { c.num_negcell = 0; c.tot_ent = 0.0; }
     if (mask & 4) { c.neg_avg = 0; c.pp_fail = 0; }
 }

@@ -488,7 +488,8 @@ extern "C" long long ppes_instance_bytes(const ppes_config *cfg)
     const long long nblk_el = cdiv(nx * ny, E), nblk_slot = cdiv((nx + 2) * (ny + 2), E);
     const long long doubles = 8 * vs * (S + 1) + 8 * vsi * S + 8 * (nx + 2) * (ny + 2) + 8 * (nx + 1) * ny +
                               8 * nx * (ny + 1) + 4 * NN * NN + 24 * nblk_slot + 24 * cdiv(nblk_slot, BLOCK) +
-                              std::max(nblk_el, nblk_slot) + 8 * (2 * nx + 2 * ny) * NN + 2 * (nx + 2) * (ny + 2);
+                              std::max(nblk_el, nblk_slot) + 8 * (2 * nx + 2 * ny) * NN + 2 * (nx + 2) * (ny + 2) +
+                              (long long)DT_SLOTS * DT_SLOT_STRIDE;
     return 8 * doubles;
 }
 

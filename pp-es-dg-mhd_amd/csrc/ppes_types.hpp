@@ -29,6 +29,14 @@ namespace ppes {
 
 constexpr int BLOCK = 256;
 
+// The residual's dt_lim minimum is gathered in DT_SLOTS cells per instance,
+// 4 KB apart (workgroup b updates cell b mod DT_SLOTS), and folded into
+// Ctl::dtlim_bits afterwards: all the workgroups of a large instance
+// updating one address would serialise their atomics on it.
+constexpr int DT_SLOTS = 32;
+constexpr int DT_SLOT_STRIDE = 512;  // unsigned long longs
+constexpr unsigned long long DT_NONE = 0x7FF0000000000000ULL;  // +inf
+
 // Device global memory seen through an explicit address space. The state
 // arrays are reached through pointers stored in the instance records, so the
 // compiler cannot infer that they are global; through these casts their
@@ -115,6 +123,7 @@ struct Inst {
     double *ut[MAXS + 1];            // residual slots 1..S
     double *uavg, *sigx, *sigy, *ldf, *part, *part2, *entpart, *ghost0;
     double *cent;                    // per slot element: cell entropy before LDF (NaN: a node inadmissible), entropy of the mean
+    unsigned long long *dtslot;      // DT_SLOTS partial minima of the residual's dt_lim, DT_SLOT_STRIDE apart
     long long nblk_el, nblk_slot;    // workgroups of the element / slot grids
     Ctl ctl;
 };
