@@ -764,11 +764,12 @@ __device__ __forceinline__ void record_positivity(Inst &I, bool on, const double
         p = dmin(p, __shfl_xor(p, m));
     }
     if ((threadIdx.x & 63) == 0 && (r < DBL_HUGE || p < DBL_HUGE)) {
+        // a plain (L1-cached) read: the record is lowered rarely after a
+        // solve's first stages, and the L1 copy is at most as old as the
+        // launch (never below the record), so it only ever costs an extra atomic
         const unsigned long long kr = pos_key(r), kp = pos_key(p);
-        if (kr < __hip_atomic_load(&I.ctl.min_rho_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMin(&I.ctl.min_rho_key, kr);
-        if (kp < __hip_atomic_load(&I.ctl.min_p_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMin(&I.ctl.min_p_key, kp);
+        if (kr < I.ctl.min_rho_key) atomicMin(&I.ctl.min_rho_key, kr);
+        if (kp < I.ctl.min_p_key) atomicMin(&I.ctl.min_p_key, kp);
     }
 }
 
