@@ -308,7 +308,7 @@ def run_serving_config5(window, cap, budget=200e9):
                "admitted_work_remaining_est": need_run,
                **({"p50": serve.percentiles(lat)["p50"], "p99": serve.percentiles(lat)["p99"]} if lat else {})}
     say(f"config 5 batched: {batched['finished']} finished, {batched['value']:.3e} DOF-stage/s")
-    seq, busy = serve.run_sequential(reqs, wall_cap=cap)
+    seq, busy = serve.run_sequential(reqs, wall_cap=cap, progress=say)
     fin = [r for r in seq if "t_done" in r]
     lat_seq = [r["t_done"] - r["arrival"] for r in fin]
     work_seq = sum(W.nodes(r) * r["stats"]["stages"] for r in seq)

@@ -138,15 +138,17 @@ def run_stream(requests, budget_bytes=200e9, use_graph=True, keep_results=False,
     return done, wall, srv.rejected
 
 
-def run_sequential(requests, wall_cap=None, chunk=50):
+def run_sequential(requests, wall_cap=None, chunk=50, progress=None):
     """The plain path on the same stream: one request at a time through the
     single-instance solver, first come first served. Returns per-request
     (start, finish) times on the stream's clock and the busy time. With
     `wall_cap`, the stream's clock stops there: the request then running
-    reports its progress (stats, no t_done) and later ones are not started."""
+    reports its progress (stats, no t_done) and later ones are not started.
+    `progress(msg)` is called about every 30 s."""
     out = []
     free_at = 0.0
     busy = 0.0
+    last = time.perf_counter()
     for r in requests:
         start = max(r["arrival"], free_at)
         if wall_cap is not None and start >= wall_cap:
@@ -167,6 +169,10 @@ def run_sequential(requests, wall_cap=None, chunk=50):
                 # fewer steps than asked: the solve reached its end (or failed)
                 finished = st["status"] != 0 or st["steps"] < steps + chunk
                 steps = st["steps"]
+                if progress is not None and time.perf_counter() - last >= 30.0:
+                    last = time.perf_counter()
+                    progress(f"sequential t={start + (last - t_start):.0f}s: {len(out)} done, request "
+                             f"{len(out)} at step {steps}")
                 if finished or start + (time.perf_counter() - t_start) >= wall_cap:
                     break
         s.close()
