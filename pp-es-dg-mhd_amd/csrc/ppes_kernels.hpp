@@ -667,8 +667,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 // Minimum over the NN lanes of each element, returned to every lane of the
 // element. When NN divides the wavefront (P1, P3) the element's lanes are one
 // aligned group of a wavefront and a butterfly of lane exchanges does it with
-// no barrier; otherwise through LDS (called by every thread of the workgroup).
-// min is exact, so both give the value of the reference's loop.
+// no barrier; otherwise (elements straddle wavefronts) through LDS: the
+// element's first lane reduces its NN values and the others read the result
+// (called by every thread of the workgroup). min is exact, so both give the
+// value of the reference's loop.
 template <int NN>
 __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, int el, bool valid)
 {
@@ -679,9 +681,13 @@ __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, i
     } else {
         sred[t][0] = x;
         __syncthreads();
-        double r = DBL_HUGE;
-        if (valid)
+        if (valid && t == el * NN) {
+            double r = DBL_HUGE;
             for (int m = 0; m < NN; ++m) r = dmin(r, sred[el * NN + m][0]);
+            sred[t][1] = r;
+        }
+        __syncthreads();
+        const double r = valid ? sred[el * NN][1] : DBL_HUGE;
         __syncthreads();
         return r;
     }
@@ -1369,6 +1375,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     __shared__ int scnt[EPB];
     __shared__ double sth[EO ? EPB : 1];
     __shared__ double ssig[EPB][16];
+    __shared__ double soth[EPB];  // OEDG damping factor per element (LDS path)
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -1399,14 +1406,11 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
             ssig[el][a] = a < 8 ? sxL[a] + sxL[8 + a] : syB[a - 8] + syB[8 * (long long)I.nx + a - 8];
     }
     {
-        double sx = 0.0, sy = 0.0;
-        if (valid) node_speeds(dmax(q[0], PP_EPS), q[1], q[2], q[3], q[4], q[5], q[6], q[7], g, sx, sy);
-        sred[t][0] = sx;
-        sred[t][1] = sy;
-        __syncthreads();
-        if (valid) {
-            double mex = 0.0, mey = 0.0;
-            for (int m = 0; m < NN; ++m) { mex = dmax(mex, sred[el * NN + m][0]); mey = dmax(mey, sred[el * NN + m][1]); }
+        // the element's damping factor (DG_solver.f90:448-524) from its
+        // largest signal speeds: by lane exchanges when the element is an
+        // aligned lane group of one wavefront (P1, P3), else by its first lane
+        // from LDS, which also forms theta once for the element
+        auto theta_of = [&](double mex, double mey) {
             double dm = 0.0;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
@@ -1418,7 +1422,28 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
                     delta = 0.0;
                 dm = (v == 0) ? delta : dmax(dm, delta);
             }
-            const double theta = exp(-stage_oedg_dt(I, st) * dm);
+            return exp(-stage_oedg_dt(I, st) * dm);
+        };
+        double sx = 0.0, sy = 0.0;
+        if (valid) node_speeds(dmax(q[0], PP_EPS), q[1], q[2], q[3], q[4], q[5], q[6], q[7], g, sx, sy);
+        double theta = 1.0;
+        if constexpr (64 % NN == 0) {
+            const double mex = seg_max<NN>(sx, t & 63, n), mey = seg_max<NN>(sy, t & 63, n);
+            __syncthreads();  // the face indicator sums are in LDS
+            if (valid) theta = theta_of(mex, mey);
+        } else {
+            sred[t][0] = sx;
+            sred[t][1] = sy;
+            __syncthreads();
+            if (valid && n == 0) {
+                double mex = 0.0, mey = 0.0;
+                for (int m = 0; m < NN; ++m) { mex = dmax(mex, sred[el * NN + m][0]); mey = dmax(mey, sred[el * NN + m][1]); }
+                soth[el] = theta_of(mex, mey);
+            }
+            __syncthreads();
+            if (valid) theta = soth[el];
+        }
+        if (valid) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta * (q[v] - avg[v]);
         }
