@@ -170,7 +170,7 @@ def run_single(ppesdg, steps, warmup):
            "residual_fp64_frac": ALG_FLOPS_PER_NODE[p]["residual"] * nodes / (r[0] / max(r[1], 1) / 1e3) / 1e12
            / FP64_PEAK_TFLOPS,
            "min_rho": float(u[0].min()), "min_p": float(prs.min()), "retries": after["retries"],
-           "residual_workgroup_lanes": tuned}
+           "residual_workgroup_lanes": tuned, "oedg_face_workgroup_lanes": s.face_tb()}
     s.close()
     return res
 
@@ -420,7 +420,8 @@ def main():
                                f"sides 32-256, P1-P4), SSP-RK3, LDF + PP every stage, OEDG final stage (blast: every "
                                f"stage), per-instance CFL dt; batched path, hipGraph step",
                    "instances": len(mix), "nodes_total": nodes_total, "dof_per_node": 8, "path": "batched",
-                   "retries": retries, "failed": len(failed), "residual_workgroup_lanes": tuned},
+                   "retries": retries, "failed": len(failed), "residual_workgroup_lanes": tuned,
+                   "oedg_face_workgroup_lanes": batch.face_tb()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(dominant),
                      "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": bytes_per_launch,

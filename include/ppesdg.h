@@ -252,6 +252,16 @@ int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long lon
  * ppes_solve_begin). */
 int ppes_batch_autotune(ppes_batch *b, int reps, int *tb_out);
 int ppes_autotune(ppes_solver *s, int reps, int *tb_out);
+/* The same call also times the OEDG face sums (DG_solver.f90:295-422, one
+ * thread per face and variable, each independent of the others, so results
+ * do not depend on the size) at 64, 128 and 256 threads per workgroup for
+ * every degree present, whatever the face flux, and keeps the fastest; these
+ * report the sizes in use by shape id as above (0: not tuned, 256 lanes).
+ * The stage and OEDG-limit kernels keep 256 lanes: they fold per-workgroup
+ * partial sums (entropy, the OEDG normalisation) whose order a different size
+ * would change. */
+int ppes_batch_face_tb(ppes_batch *b, int *tb_out);
+int ppes_face_tb(ppes_solver *s, int *tb_out);
 
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
  * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major),

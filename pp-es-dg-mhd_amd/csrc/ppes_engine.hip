@@ -63,6 +63,18 @@ namespace ppes {
         }                                                                                                        \
     } while (0)
 
+// the OEDG face sums over a group (or sub-plan) at its workgroup size
+#define FACES_DISPATCH(g, ...)                                                                                    \
+    do {                                                                                                          \
+        if ((g).face_tb == 64) {                                                                                  \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 64>), (g).n_face, 64, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+        } else if ((g).face_tb == 128) {                                                                          \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 128>), (g).n_face, 128, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+        } else {                                                                                                  \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 256>), (g).n_face, 256, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+        }                                                                                                         \
+    } while (0)
+
 // one case per shape id, with PX, PY and the node count NN_ as constants
 // (CALL is substituted in place: it expands to launch syntax whose commas
 // must not pass through another macro)
@@ -559,6 +571,7 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapP
     g.eo = false;
     for (int id : members) g.eo = g.eo || hosts[id]->ent_offset;
     g.res_tb = residual_tb(sid, members);
+    g.face_tb = face_tb(sid);
     std::vector<Blk> el, sl, fc, bc, r1, r2, rs, rsf[4];
     for (int id : members) {
         const Inst &r = h_recs[id];
@@ -568,7 +581,7 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapP
         strip_order(rsf[hosts[id]->flux], id, nres, r.nx, r.ny, g.res_tb / NN, NN);
         for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
         // x-face workgroups, then y-face ones (the kernel's split)
-        const long long nf = cdiv((long long)(r.nx + 1) * r.ny * 8, BLOCK) + cdiv((long long)r.nx * (r.ny + 1) * 8, BLOCK);
+        const long long nf = cdiv((long long)(r.nx + 1) * r.ny * 8, g.face_tb) + cdiv((long long)r.nx * (r.ny + 1) * 8, g.face_tb);
         for (long long b = 0; b < nf; ++b) fc.push_back({id, (int)b});
         const long long nb = (2LL * r.nx + 2LL * r.ny) * NN;
         for (long long b = 0; b < cdiv(nb, BLOCK); ++b) bc.push_back({id, (int)b});
@@ -796,7 +809,7 @@ int Engine::launch_step(bool all_kernels)
                 }
                 {
                     Timed tm(this, KT_OEDG);
-                    ENG_DISPATCH(g.sid, LAUNCH((k_oedg_faces<PX, PY>), g.n_face, d_recs, (const Blk *)g.d_face, st, 0));
+                    FACES_DISPATCH(g, st, 0);
                 }
                 {
                     Timed tm(this, KT_LIMIT);
@@ -861,7 +874,7 @@ int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
             const Group &o = *og;
             LAUNCH(k_reduce_global, o.n_red1, d_recs, (const Blk *)o.d_red1, st, 0, 0);
             LAUNCH(k_reduce_global, (long long)o.members.size(), d_recs, (const Blk *)o.d_red2, st, 0, 1);
-            ENG_DISPATCH(o.sid, LAUNCH((k_oedg_faces<PX, PY>), o.n_face, d_recs, (const Blk *)o.d_face, st, 0));
+            FACES_DISPATCH(o, st, 0);
             if (o.eo) {
                 ENG_DISPATCH(o.sid, LAUNCH((k_oedg_limit<PX, PY, true>), o.n_el, d_recs, (const Blk *)o.d_el, st, 0));
             } else {
@@ -949,6 +962,44 @@ int Engine::autotune(int reps, int *tb_out)
         if (err) break;
         tuned_tb[g.sid] = best;
         if (tb_out) tb_out[g.sid - 1] = best;
+    }
+    // the OEDG face sums of u^n (written to the face indicator buffers only,
+    // which every OEDG stage recomputes before its limiter reads them), at
+    // 64, 128 and 256 threads per workgroup
+    for (const Group &g : groups) {
+        if (err) break;
+        int best = 0;
+        float best_ms = 0.0f;
+        for (int tb : {64, 128, 256}) {
+            std::vector<Blk> fc;
+            for (int id : g.members) {
+                const Inst &r = h_recs[id];
+                const long long nf = cdiv((long long)(r.nx + 1) * r.ny * 8, tb) + cdiv((long long)r.nx * (r.ny + 1) * 8, tb);
+                for (long long b2 = 0; b2 < nf; ++b2) fc.push_back({id, (int)b2});
+            }
+            Blk *d = nullptr;
+            if (hipMalloc((void **)&d, std::max<size_t>(fc.size(), 1) * sizeof(Blk)) != hipSuccess) { err = PPES_OOM; break; }
+            Group t = g;
+            t.face_tb = tb;
+            t.d_face = d;
+            t.n_face = (long long)fc.size();
+            float ms = 0.0f;
+            auto run = [&]() -> int {
+                ENG_TRY(hipMemcpy(d, fc.data(), fc.size() * sizeof(Blk), hipMemcpyHostToDevice));
+                FACES_DISPATCH(t, 0, (int)F_FORCE_OEDG);  // warm-up
+                ENG_TRY(hipEventRecord(e0, stream));
+                for (int r = 0; r < reps; ++r) FACES_DISPATCH(t, 0, (int)F_FORCE_OEDG);
+                ENG_TRY(hipEventRecord(e1, stream));
+                ENG_TRY(hipEventSynchronize(e1));
+                ENG_TRY(hipEventElapsedTime(&ms, e0, e1));
+                return PPES_OK;
+            };
+            err = run();
+            (void)hipFree(d);
+            if (err) break;
+            if (best == 0 || ms < best_ms) { best = tb; best_ms = ms; }
+        }
+        if (!err) tuned_face_tb[g.sid] = best;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1083,7 +1134,8 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     Group &g = api_groups[id];
     // (rebuilt when the residual size its map was made for is no longer the
     // one the instance's mode and flux select)
-    if (g.d_el && g.res_tb != residual_tb(hosts[id]->sid, std::vector<int>{id})) free_group(g);
+    if (g.d_el && (g.res_tb != residual_tb(hosts[id]->sid, std::vector<int>{id}) || g.face_tb != face_tb(hosts[id]->sid)))
+        free_group(g);
     if (!g.d_el) {
         int rc = build_group(g, std::vector<int>{id}, hosts[id]->sid);
         if (rc) { free_group(g); return rc; }
@@ -1118,7 +1170,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         if (flags & F_FORCE_OEDG) {
             LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, slot, flags, 0);
             LAUNCH(k_reduce_global, 1, d_recs, (const Blk *)g.d_red2, slot, flags, 1);
-            ENG_DISPATCH(g.sid, LAUNCH((k_oedg_faces<PX, PY>), g.n_face, d_recs, (const Blk *)g.d_face, slot, flags));
+            FACES_DISPATCH(g, slot, flags);
             if (eo) {
                 ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
             } else {

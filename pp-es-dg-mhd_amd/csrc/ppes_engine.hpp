@@ -40,6 +40,7 @@ struct Group {
     long long res_off[5] = {0, 0, 0, 0, 0};  // blocks of flux kind f: [res_off[f], res_off[f + 1])
     int *d_list = nullptr;
     int res_tb = 0;  // workgroup size of the residual launches (the residual map's elements per block follow)
+    int face_tb = 0;  // workgroup size of the OEDG face launches (the face map's blocks follow)
     long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
     bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
     // OEDG participation: some member applies it on the final stage; the
@@ -112,6 +113,8 @@ public:
     int autotune(int reps, int *tb_out);
     int tuned_tb[9] = {};  // by shape id; 0: the default size
     int residual_tb(int sid, const std::vector<int> &members) const;
+    int tuned_face_tb[9] = {};  // OEDG face workgroup size by shape id; 0: BLOCK
+    int face_tb(int sid) const { return tuned_face_tb[sid] ? tuned_face_tb[sid] : BLOCK; }
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
     int set_flux(int id, int kind);                                      // face flux of one instance (FluxKind)
 

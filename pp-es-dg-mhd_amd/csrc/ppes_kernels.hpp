@@ -1321,19 +1321,21 @@ __device__ __forceinline__ double oedg_face_sum(const double *A, const double *B
 }
 
 // The face direction is uniform over a workgroup: an instance's x-face
-// workgroups (cdiv(8 (nx+1) ny, BLOCK) of them) come first, then its y-face
+// workgroups (cdiv(8 (nx+1) ny, TBF) of them) come first, then its y-face
 // ones, so every table the sums read is one uniform address.
-template <int PX, int PY>
-__global__ void __launch_bounds__(BLOCK) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
+// TBF threads per workgroup (64, 128 or 256, chosen by the autotuner: each
+// thread's sum is independent of the others, so results do not depend on it).
+template <int PX, int PY, int TBF = BLOCK>
+__global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
     constexpr int NX = PX + 1, NY = PY + 1, NN = NX * NY, MXP = PX > PY ? PX : PY;
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const long long nxf = (long long)(I.nx + 1) * I.ny * 8, nyf = (long long)I.nx * (I.ny + 1) * 8;
-    const long long nbx = (nxf + BLOCK - 1) / BLOCK;
+    const long long nbx = (nxf + TBF - 1) / TBF;
     const bool xface = bm.local < nbx;
-    const long long tt = (xface ? (long long)bm.local : bm.local - nbx) * BLOCK + threadIdx.x;
+    const long long tt = (xface ? (long long)bm.local : bm.local - nbx) * TBF + threadIdx.x;
     if (tt >= (xface ? nxf : nyf)) return;
     const gdbl *__restrict__ u = gin(I.u[st]);
     const int v = (int)(tt % 8);

@@ -950,6 +950,25 @@ extern "C" int ppes_batch_autotune(ppes_batch *b, int reps, int *tb_out)
     return b->eng.autotune(reps, tb_out);
 }
 
+static void face_sizes(const Engine &e, int *tb_out)
+{
+    for (int sid = 1; sid <= 8; ++sid) tb_out[sid - 1] = e.tuned_face_tb[sid];
+}
+
+extern "C" int ppes_batch_face_tb(ppes_batch *b, int *tb_out)
+{
+    if (!b || !tb_out) return PPES_BAD_ARG;
+    face_sizes(b->eng, tb_out);
+    return PPES_OK;
+}
+
+extern "C" int ppes_face_tb(ppes_solver *s, int *tb_out)
+{
+    if (!s || !tb_out) return PPES_BAD_ARG;
+    face_sizes(s->eng, tb_out);
+    return PPES_OK;
+}
+
 extern "C" int ppes_autotune(ppes_solver *s, int reps, int *tb_out)
 {
     if (!s || !s->begun) return PPES_BAD_ARG;

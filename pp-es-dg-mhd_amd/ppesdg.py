@@ -93,6 +93,8 @@ def lib():
             "ppes_set_arith": [vp, ctypes.c_int],
             "ppes_batch_set_arith": [vp, ctypes.c_int],
             "ppes_autotune": [vp, ctypes.c_int, ip],
+            "ppes_face_tb": [vp, ip],
+            "ppes_batch_face_tb": [vp, ip],
             "ppes_batch_autotune": [vp, ctypes.c_int, ip],
             "ppes_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
             "ppes_batch_create": [ctypes.c_int, ctypes.POINTER(vp)],
@@ -342,6 +344,10 @@ class Solver:
         {"P<px>x<py>": lanes} for the tuned shape."""
         return _tuned(lambda out: lib().ppes_autotune(self.h, int(reps), out), "ppes_autotune")
 
+    def face_tb(self):
+        """OEDG face workgroup size per tuned shape, {"P<px>x<py>": lanes}."""
+        return _tuned(lambda out: lib().ppes_face_tb(self.h, out), "ppes_face_tb")
+
     def set_flux(self, kind):
         """face flux: espp_hll (default), pp_hll, espp_lf or ec_lf (ESPP_HLL_flux_x,
         PP_HLL_flux_x, ESPP_LF_flux_x, EC_LF_flux_x of DG_solver.f90:894-1497)"""
@@ -447,6 +453,10 @@ class Batch:
         """Residual workgroup size per degree among the active instances, by
         timing; returns {"P<px>x<py>": lanes} for the tuned shapes."""
         return _tuned(lambda out: lib().ppes_batch_autotune(self.h, int(reps), out), "ppes_batch_autotune")
+
+    def face_tb(self):
+        """OEDG face workgroup size per tuned shape, {"P<px>x<py>": lanes}."""
+        return _tuned(lambda out: lib().ppes_batch_face_tb(self.h, out), "ppes_batch_face_tb")
 
     def set_flux(self, i, kind):
         """face flux of instance i (see Solver.set_flux)"""

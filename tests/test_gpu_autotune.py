@@ -1,7 +1,8 @@
-"""The residual autotuner (ppes_batch_autotune / ppes_autotune): it picks a
-workgroup size from the candidates for every degree present, and the solves
-continue to the same bits as without it (an element's arithmetic does not
-depend on how many elements share its workgroup)."""
+"""The autotuner (ppes_batch_autotune / ppes_autotune): it picks a residual
+and an OEDG-face workgroup size from the candidates for every degree present,
+and the solves continue to the same bits as without it (an element's
+arithmetic does not depend on how many elements share its workgroup, nor a
+face's on how many faces share it)."""
 import numpy as np
 import pytest
 
@@ -25,18 +26,20 @@ def run_batch(tune):
         ids.append(i)
     b.step(3, True)
     tuned = b.autotune(3) if tune else {}
+    faces = b.face_tb()
     b.step(5, True)
     out = [interior(b.download_state(i, 0)) for i in ids]
     b.close()
-    return out, tuned
+    return out, tuned, faces
 
 
 def test_batch_autotune_keeps_results():
-    ref, _ = run_batch(False)
-    got, tuned = run_batch(True)
-    print("tuned residual workgroups:", tuned)
-    assert set(tuned) == {"P1x1", "P2x2", "P3x3", "P4x4"}
-    assert all(v in (64, 128, 256) for v in tuned.values())
+    ref, _, faces0 = run_batch(False)
+    got, tuned, faces = run_batch(True)
+    print("tuned residual workgroups:", tuned, "OEDG face workgroups:", faces)
+    assert faces0 == {}
+    assert set(tuned) == {"P1x1", "P2x2", "P3x3", "P4x4"} and set(faces) == set(tuned)
+    assert all(v in (64, 128, 256) for v in list(tuned.values()) + list(faces.values()))
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
 
@@ -50,7 +53,7 @@ def test_solver_autotune_keeps_results():
         g.solve_begin(True)
         g.advance(-1.0, 2, 0.2, 1)
         if tune:
-            assert set(g.autotune(2)) == {"P3x3"}
+            assert set(g.autotune(2)) == {"P3x3"} and set(g.face_tb()) == {"P3x3"}
         g.advance(-1.0, 3, 0.2, 1)
         out.append(interior(g.download_state(0)))
         g.close()
