@@ -181,7 +181,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     using S = Shape<PX, PY>;
     constexpr int NX = S::NX, NY = S::NY, NN = S::NN, EPB = TB / NN, REC = S::REC;
     constexpr int XP = S::XP, HX = S::HX, HY = S::HY, ROUNDS = S::ROUNDS;
-    __shared__ Prim sprim[EPB][NN];
+    // node primitives; the fast form adds their logs (PrimL), and keeps
+    // ln rho and ln p of every node for the face phase
+    using PT = std::conditional_t<FAST, PrimL, Prim>;
+    __shared__ PT sprim[EPB][NN];
+    __shared__ double slog[FAST ? EPB : 1][FAST ? NN : 1][2];
     __shared__ double sbuf[EPB][(FAST && PPES_FAST_ROUND) ? S::BUF_ROUND : S::BUF];
     // the element maxima of the signal speeds go by lane exchanges when every
     // element lies inside one wavefront, else through LDS
@@ -296,9 +300,18 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double coef = 4.0 * snc_sq * inv_rho;
         msx = fabs(vx) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (bx * bx), 0.0)), 0.0));
         msy = fabs(vy) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (by * by), 0.0)), 0.0));
-        Prim pr;
+        PT pr;
         pr.rho = rho; pr.vx = vx; pr.vy = vy; pr.vz = vz; pr.bx = bx; pr.by = by; pr.bz = bz;
         pr.beta = 0.5 * rho / p_fld;
+        if constexpr (FAST) {
+            // pfld_sym is p_fld's expression: the face phase of the other
+            // element of a face forms the same logs from this node's trace
+            const double lnr = ln_pos(rho), lnp = ln_pos(pfld_sym(q, g, inv_rho));
+            pr.lnr = lnr;
+            pr.lnb = (lnr - lnp) - LN2;
+            slog[el][n][0] = lnr;
+            slog[el][n][1] = lnp;
+        }
         sprim[el][n] = pr;
     }
     __syncthreads();
@@ -361,7 +374,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             const int lo = a - hi * (hi - 1) / 2;
             const int nA = ydir ? line + NX * hi : hi + NX * line;
             const int nB2 = ydir ? line + NX * lo : lo + NX * line;
-            Prim A = sprim[el][nA], B = sprim[el][nB2];
+            PT A = sprim[el][nA], B = sprim[el][nB2];
             if (ydir) {
                 double x;
                 x = A.vx; A.vx = A.vy; A.vy = x;
@@ -456,7 +469,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         // direction) and nB; a y-pair is the x-flux of the x/y-swapped
         // primitives, swapped back (x-record (0,1,2,3,5,6,7), y-record (0,1,2,3,4,6,7))
         auto pair_rec = [&](int nA, int nB, bool ydir, double *o) {
-            Prim A = sprim[el][nA], B = sprim[el][nB];
+            PT A = sprim[el][nA], B = sprim[el][nB];
             if (ydir) {
                 double x;
                 x = A.vx; A.vx = A.vy; A.vy = x;
@@ -566,7 +579,15 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             if constexpr (FLUX == FLUX_PP_HLL) pp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
             else if constexpr (FLUX == FLUX_ESPP_LF) espp_lf_flux_x(wl, wr, g, tf, nB, sl, sr);
             else if constexpr (FLUX == FLUX_EC_LF) ec_lf_flux_x(wl, wr, g, tf, nB, sl, sr);
-            else if constexpr (FAST) espp_hll_flux_x_fast(wl, wr, g, inv_gm1, tf, nB, sl, sr);
+            else if constexpr (FAST) {
+                // the element's own trace node: the right state of its left
+                // and bottom faces, the left state of its right and top faces
+                const int side = f < 2 * NY ? f / NY : 2 + (f - 2 * NY) / NX;
+                const int a = f < 2 * NY ? f - side * NY : f - 2 * NY - (side - 2) * NX;
+                const int no = side == 0 ? NX * a : side == 1 ? (NX - 1) + NX * a : side == 2 ? a : a + NX * (NY - 1);
+                espp_hll_flux_x_fast(wl, wr, g, inv_gm1, side == 1 || side == 3, slog[el][no][0], slog[el][no][1], tf,
+                                     nB, sl, sr);
+            }
             else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
             double *o = &sbuf[el][f * S::FREC];
             swap_xy_if(yf, tf, o);

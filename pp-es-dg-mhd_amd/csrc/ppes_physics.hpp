@@ -7,6 +7,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "ppes_math.hpp"
+
 namespace ppes {
 
 constexpr double PP_EPS = 1.0e-11;    // DG_solver.f90:17
@@ -299,27 +301,76 @@ __device__ __forceinline__ void ec_lf_flux_x(const double wl[8], const double wr
 }
 
 // ---------------------------------------------------------------------------
-// Shared-reciprocal forms of the two flux functions (the library's fast
-// arithmetic mode): the same mathematics with one division per distinct
-// denominator -- (zeta - 1) / (zeta + 1) as (L - R) / (L + R), x / p as
-// x * (1 / p), / (g - 1) as * (1 / (g - 1)) -- and a * b + c contracted to a
-// fused multiply-add within each statement (`fp contract(on)`). Contraction
-// is fixed per source expression, not left to instruction scheduling, so
-// every inlined copy rounds alike and both elements sharing a face still
-// compute bit-identical face fluxes. Results differ from the reference's
-// rounding by a few ulp.
+// Shared-reciprocal, shared-logarithm forms of the two flux functions (the
+// library's fast arithmetic mode): the same mathematics with one division
+// per distinct denominator -- x / p as x * (1 / p), / (g - 1) as
+// * (1 / (g - 1)) --, the logarithms of rho and p formed once per node (by
+// ln_pos, ppes_math.hpp) and reused by every pair and face that node enters,
+// the logarithmic means from those logs (log_mean_parts), and a * b + c
+// contracted to a fused multiply-add within each statement
+// (`fp contract(on)`). Contraction is fixed per source expression, not left
+// to instruction scheduling, so every inlined copy rounds alike and both
+// elements sharing a face still compute bit-identical face fluxes. Results
+// differ from the reference's rounding by a few ulp.
 
-__device__ __forceinline__ double log_avg_divisor_fast(double LL, double RR)
+// Node primitives with their logarithms (fast mode): ln rho and
+// ln beta = ln rho - ln p - ln 2, formed once per node, so that no pair or
+// face evaluates a logarithm its element already holds.
+struct PrimL : Prim {
+    double lnr, lnb;
+};
+
+constexpr double LN2 = 0.69314718055994530942;
+
+// Field pressure max((g - 1) (E - |m|^2 / (2 rho) - |B|^2 / 2), eps) of a
+// conservative state (inv_rho = 1 / rho), in the node primitives' order of
+// operations and without contraction: the squares are summed as
+// (x^2 + y^2) + z^2, so a state and its swap_xy image give the same bits.
+__device__ __forceinline__ double pfld_sym(const double w[8], double g, double inv_rho)
+{
+#pragma clang fp contract(off)
+    const double m2 = (w[1] * w[1] + w[2] * w[2]) + w[3] * w[3];
+    const double b2 = (w[4] * w[4] + w[5] * w[5]) + w[6] * w[6];
+    return dmax((g - 1.0) * (w[7] - (0.5 * inv_rho * m2 + 0.5 * b2)), PP_EPS);
+}
+
+// The logarithmic mean (a - b) / (ln a - ln b) of a, b > 0 from their logs
+// la, lb, as a quotient whose parts the caller selects: with
+// f = (a - b) / (a + b) and u = f^2,
+//   u <  1e-2: ((a + b) / 2) / F(u), F(u) = atanh(f) / f = sum_k u^k / (2k + 1)
+//              to k = 7 (the remainder is below u^8 / 17 / (1 - u) < 6e-18);
+//   u >= 1e-2: (a - b) / (la - lb), where |la - lb| = 2 atanh|f| >= 0.2, so
+//              the difference of the logs loses a few ulp of max(|la|, |lb|)
+//              at most.
+// This is the mean the reference's log_avg_divisor (DG_solver.f90:1653-1681)
+// forms through one log of a / b per pair. u needs far less than full
+// precision (its relative error enters F scaled by u / 3 < 3.4e-3): 1 / (a + b)
+// is the hardware reciprocal refined by two Newton steps.
+struct LogMean {
+    bool small;
+    double F, d, dl;
+};
+
+__device__ __forceinline__ LogMean log_mean_parts(double a, double b, double la, double lb)
 {
 #pragma clang fp contract(on)
-    const double f = (LL - RR) / (LL + RR);
+    LogMean m;
+    const double s = a + b;
+    m.d = a - b;
+    m.dl = la - lb;
+    double r = __builtin_amdgcn_rcp(s);
+    r = __builtin_fma(r, __builtin_fma(-s, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-s, r, 1.0), r);
+    const double f = m.d * r;
     const double u = f * f;
-    if (u < 1.4e-4) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
-    return (0.5 * log(LL / RR)) / f;
+    m.small = u < 1e-2;
+    m.F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0 + u * (1.0 / 11.0 +
+          u * (1.0 / 13.0 + u * (1.0 / 15.0)))))));
+    return m;
 }
 
 template <int DIR>
-__device__ __forceinline__ void ec_flux_fast(const Prim &L, const Prim &R, double inv_gm1, double f[8])
+__device__ __forceinline__ void ec_flux_fast(const PrimL &L, const PrimL &R, double inv_gm1, double f[8])
 {
 #pragma clang fp contract(on)
     const double rho_bar = 0.5 * (L.rho + R.rho);
@@ -336,9 +387,12 @@ __device__ __forceinline__ void ec_flux_fast(const Prim &L, const Prim &R, doubl
     const double betavy_bar = 0.5 * (L.beta * L.vy + R.beta * R.vy);
     const double betavz_bar = 0.5 * (L.beta * L.vz + R.beta * R.vz);
     const double inv_bb = 1.0 / beta_bar;
-    const double rho_hat = rho_bar / log_avg_divisor_fast(L.rho, R.rho);
-    // 1 / ((g - 1) beta_hat) with beta_hat = beta_bar / divisor
-    const double e_coef = log_avg_divisor_fast(L.beta, R.beta) * inv_bb * inv_gm1;
+    // rho_hat, the logarithmic mean of rho, and 1 / ((g - 1) beta_hat): one
+    // division each, of the operands the mean's branch selects
+    const LogMean mr = log_mean_parts(L.rho, R.rho, L.lnr, R.lnr);
+    const LogMean mb = log_mean_parts(L.beta, R.beta, L.lnb, R.lnb);
+    const double rho_hat = (mr.small ? rho_bar : mr.d) / (mr.small ? mr.F : mr.dl);
+    const double e_coef = ((mb.small ? mb.F : mb.dl) / (mb.small ? beta_bar : mb.d)) * inv_gm1;
     if (DIR == 0) {
         f[0] = rho_hat * vx_bar;
         f[1] = vx_bar * f[0] - bx_bar * bx_bar + 0.5 * (b_sq_bar + rho_bar * inv_bb);
@@ -364,8 +418,14 @@ __device__ __forceinline__ void ec_flux_fast(const Prim &L, const Prim &R, doubl
     }
 }
 
+// The two states' logs: the element's own trace node's (lnr_o = ln rho,
+// lnp_o = ln p, on the left when own_l) come from the node primitives; the
+// other element's are formed here, from pfld_sym, as that element forms its
+// own (and swap_xy leaves them unchanged): both elements of a face still
+// compute the same bits.
 __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const double wr[8], double g, double inv_gm1,
-                                                     double nflux[8], double &nBn, double &sl_out, double &sr_out)
+                                                     bool own_l, double lnr_o, double lnp_o, double nflux[8],
+                                                     double &nBn, double &sl_out, double &sr_out)
 {
 #pragma clang fp contract(on)
     const double rho_l = wl[0], mx_l = wl[1], my_l = wl[2], mz_l = wl[3], bx_l = wl[4], by_l = wl[5], bz_l = wl[6], e_l = wl[7];
@@ -381,8 +441,8 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     const double b_sq_l = bx_l * bx_l + by_l * by_l + bz_l * bz_l;
     const double b_sq_r = bx_r * bx_r + by_r * by_r + bz_r * bz_r;
     const double p_mag_l = 0.5 * b_sq_l, p_mag_r = 0.5 * b_sq_r;
-    const double p_fld_l = dmax((g - 1.0) * (e_l - (e_knt_l + p_mag_l)), PP_EPS);
-    const double p_fld_r = dmax((g - 1.0) * (e_r - (e_knt_r + p_mag_r)), PP_EPS);
+    const double p_fld_l = pfld_sym(wl, g, inv_rho_l);
+    const double p_fld_r = pfld_sym(wr, g, inv_rho_r);
     const double inv_p_l = 1.0 / p_fld_l, inv_p_r = 1.0 / p_fld_r;
     const double p_tot_l = p_fld_l + p_mag_l, p_tot_r = p_fld_r + p_mag_r;
     const double b_dot_u_l = bx_l * vx_l + by_l * vy_l + bz_l * vz_l;
@@ -392,8 +452,9 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     const double cfx_l = sqrt(0.5 * (sum_sq_l + sqrt(dmax(sum_sq_l * sum_sq_l - 4.0 * snc_sq_l * inv_rho_l * (bx_l * bx_l), 0.0))));
     const double cfx_r = sqrt(0.5 * (sum_sq_r + sqrt(dmax(sum_sq_r * sum_sq_r - 4.0 * snc_sq_r * inv_rho_r * (bx_r * bx_r), 0.0))));
     const double inv_theta_l = rho_l * inv_p_l, inv_theta_r = rho_r * inv_p_r;
-    const double sp_ent_l = log(p_fld_l) - g * log(rho_l);
-    const double sp_ent_r = log(p_fld_r) - g * log(rho_r);
+    const double lnr_x = ln_pos(own_l ? rho_r : rho_l), lnp_x = ln_pos(own_l ? p_fld_r : p_fld_l);
+    const double sp_ent_l = (own_l ? lnp_o : lnp_x) - g * (own_l ? lnr_o : lnr_x);
+    const double sp_ent_r = (own_l ? lnp_x : lnp_o) - g * (own_l ? lnr_x : lnr_o);
     const double ent_flux_l = -(rho_l * sp_ent_l) * inv_gm1 * vx_l;
     const double ent_flux_r = -(rho_r * sp_ent_r) * inv_gm1 * vx_r;
     double fl[8], fr[8];
