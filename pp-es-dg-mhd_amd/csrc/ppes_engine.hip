@@ -63,15 +63,18 @@ namespace ppes {
         }                                                                                                        \
     } while (0)
 
-// the OEDG face sums over a group (or sub-plan) at its workgroup size
+// the OEDG face sums over a group (or sub-plan) at its workgroup size, in the
+// fast form in the fast arithmetic mode (the exact form always at BLOCK)
 #define FACES_DISPATCH(g, ...)                                                                                    \
     do {                                                                                                          \
-        if ((g).face_tb == 64) {                                                                                  \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 64>), (g).n_face, 64, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+        if (arith != PPES_ARITH_FAST) {                                                                           \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, BLOCK, false>), (g).n_face, BLOCK, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+        } else if ((g).face_tb == 64) {                                                                           \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 64, true>), (g).n_face, 64, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
         } else if ((g).face_tb == 128) {                                                                          \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 128>), (g).n_face, 128, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 128, true>), (g).n_face, 128, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
         } else {                                                                                                  \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 256>), (g).n_face, 256, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
+            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 256, true>), (g).n_face, 256, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
         }                                                                                                         \
     } while (0)
 

@@ -114,7 +114,7 @@ public:
     int tuned_tb[9] = {};  // by shape id; 0: the default size
     int residual_tb(int sid, const std::vector<int> &members) const;
     int tuned_face_tb[9] = {};  // OEDG face workgroup size by shape id; 0: BLOCK
-    int face_tb(int sid) const { return tuned_face_tb[sid] ? tuned_face_tb[sid] : BLOCK; }
+    int face_tb(int sid) const { return arith == PPES_ARITH_FAST && tuned_face_tb[sid] ? tuned_face_tb[sid] : BLOCK; }
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
     int set_flux(int id, int kind);                                      // face flux of one instance (FluxKind)
 

@@ -1341,15 +1341,69 @@ __device__ __forceinline__ double oedg_face_sum(const double *A, const double *B
     return total;
 }
 
+// The same sums in the fast arithmetic mode: every a * b + c a fused
+// multiply-add, the order's scale (2/h)^o folded into the (o, q) weights
+// (|s x| = s |x| for s > 0), the last transverse derivative of each order,
+// which no later term reads, not formed, and the tables read from LDS (one
+// copy per workgroup; as scalar loads they outgrow the scalar registers and
+// spill). Results differ from the exact form by a few ulp.
+template <int PN, int PT, int MXP, bool XF, int NX>
+__device__ __forceinline__ double oedg_face_sum_fast(const double *A, const double *B, const double *tab, double h)
+{
+    constexpr int NNO = PN + 1, NT = PT + 1;
+    // tab: rows of D^o (side 0 then side 1), (2/h_t) D_t, the (o, q) weights,
+    // half the transverse quadrature weights
+    const double *r0 = tab, *r1 = tab + MAXN * MAXN, *d2t = tab + 2 * MAXN * MAXN;
+    const double *coef = d2t + MAXN * MAXN, *hq = coef + MAXN * MAXN;
+    double total = 0.0, scale = 1.0;
+#pragma unroll
+    for (int o = 0; o <= PN; ++o) {
+        double jump[NT], jt[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) jump[t] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NNO; ++a) {
+            const double c1 = r1[o * MAXN + a], c0 = r0[o * MAXN + a];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int m = XF ? a + NX * t : t + NX * a;
+                jump[t] = __builtin_fma(c1, B[m], __builtin_fma(-c0, A[m], jump[t]));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q <= MXP - o; ++q) {
+            double jl1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) jl1 = __builtin_fma(hq[t], fabs(jump[t]), jl1);
+            total = __builtin_fma(coef[o * MAXN + q] * scale, jl1, total);
+            if (q < PT && q < MXP - o) {
+#pragma unroll
+                for (int a = 0; a < NT; ++a) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) acc = __builtin_fma(d2t[a + NT * b], jump[b], acc);
+                    jt[a] = acc;
+                }
+#pragma unroll
+                for (int a = 0; a < NT; ++a) jump[a] = jt[a];
+            }
+        }
+        if (o < PN) scale = scale * (2.0 / h);
+    }
+    return total;
+}
+
 // The face direction is uniform over a workgroup: an instance's x-face
 // workgroups (cdiv(8 (nx+1) ny, TBF) of them) come first, then its y-face
 // ones, so every table the sums read is one uniform address.
 // TBF threads per workgroup (64, 128 or 256, chosen by the autotuner: each
 // thread's sum is independent of the others, so results do not depend on it).
-template <int PX, int PY, int TBF = BLOCK>
+template <int PX, int PY, int TBF = BLOCK, bool FAST = false>
 __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
     constexpr int NX = PX + 1, NY = PY + 1, NN = NX * NY, MXP = PX > PY ? PX : PY;
+    constexpr int TAB = 4 * MAXN * MAXN + MAXN;
+    __shared__ double stab[FAST ? TAB : 1];
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
@@ -1357,10 +1411,13 @@ __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map,
     const long long nbx = (nxf + TBF - 1) / TBF;
     const bool xface = bm.local < nbx;
     const long long tt = (xface ? (long long)bm.local : bm.local - nbx) * TBF + threadIdx.x;
-    if (tt >= (xface ? nxf : nyf)) return;
+    // the fast form keeps every thread to the barrier below; the others
+    // leave here
+    const bool act = tt < (xface ? nxf : nyf);
+    if (!FAST && !act) return;
     const gdbl *__restrict__ u = gin(I.u[st]);
     const int v = (int)(tt % 8);
-    const long long f = tt / 8;
+    const long long f = act ? tt / 8 : 0;
     long long bA, bB;
     if (xface) {
         const int i = (int)(f % (I.nx + 1)) + 1, j = (int)(f / (I.nx + 1)) + 1;
@@ -1376,7 +1433,27 @@ __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map,
     double A[NN], B[NN];
 #pragma unroll
     for (int m = 0; m < NN; ++m) { A[m] = uA[m]; B[m] = uB[m]; }
-    if (xface) {
+    if constexpr (FAST) {
+        // the face direction's tables (the direction is uniform over the
+        // workgroup), copied once per workgroup while the element values
+        // are in flight: rows of D^o (side 0, then side 1), (2/h_t) D_t, the
+        // (o, q) weights, half the transverse quadrature weights
+        const int d = xface ? 0 : 1;
+        const double *r = &(xface ? I.drow_x : I.drow_y)[0][0][0];
+        const double *qw = xface ? I.qw_y : I.qw_x;
+        for (int a = threadIdx.x; a < TAB; a += TBF) {
+            double x;
+            if (a < 2 * MAXN * MAXN) x = r[a];
+            else if (a < 3 * MAXN * MAXN) x = I.oedg_d2[d][a - 2 * MAXN * MAXN];
+            else if (a < 4 * MAXN * MAXN) x = (&I.oedg_coef[d][0][0])[a - 3 * MAXN * MAXN];
+            else x = 0.5 * qw[a - 4 * MAXN * MAXN];
+            stab[a] = x;
+        }
+        __syncthreads();
+        if (!act) return;
+        if (xface) gout(I.sigx)[f * 8 + v] = oedg_face_sum_fast<PX, PY, MXP, true, NX>(A, B, stab, I.hx);
+        else gout(I.sigy)[f * 8 + v] = oedg_face_sum_fast<PY, PX, MXP, false, NX>(A, B, stab, I.hy);
+    } else if (xface) {
         const double total = oedg_face_sum<PX, PY, MXP, true, NX>(A, B, I.drow_x, I.oedg_d2[0], I.oedg_coef[0], I.qw_y, I.hx);
         gout(I.sigx)[f * 8 + v] = total;
     } else {

@@ -952,7 +952,7 @@ extern "C" int ppes_batch_autotune(ppes_batch *b, int reps, int *tb_out)
 
 static void face_sizes(const Engine &e, int *tb_out)
 {
-    for (int sid = 1; sid <= 8; ++sid) tb_out[sid - 1] = e.tuned_face_tb[sid];
+    for (int sid = 1; sid <= 8; ++sid) tb_out[sid - 1] = e.arith == PPES_ARITH_FAST ? e.tuned_face_tb[sid] : 0;
 }
 
 extern "C" int ppes_batch_face_tb(ppes_batch *b, int *tb_out)
