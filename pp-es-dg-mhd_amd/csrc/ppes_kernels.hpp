@@ -175,6 +175,18 @@ __device__ __forceinline__ void wave_min_atomic(unsigned long long *bits, double
     }
 }
 
+// LDS atomic minimum / maximum of a double (ds_min_f64 / ds_max_f64): min
+// and max are exact in any order, so a cell reduced by all lanes of an
+// element holds what a loop over them would (for NaN-free values)
+__device__ __forceinline__ void lds_min(double *cell, double x)
+{
+    __hip_atomic_fetch_min(cell, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_max(double *cell, double x)
+{
+    __hip_atomic_fetch_max(cell, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <int PX, int PY, bool FAST, int TB = resid_tb((PX + 1) * (PY + 1)), int FLUX = FLUX_ESPP_HLL>
 __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_waves(PX == PY ? PX : 0, FAST)))) k_residual(Inst *insts, const Blk *map, int u_slot, int ut_slot, int mode)
 {
@@ -190,7 +202,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     // the element maxima of the signal speeds go by lane exchanges when every
     // element lies inside one wavefront, else through LDS
     constexpr bool SEG = (TB == 64) || (64 % NN == 0);
-    __shared__ double sred[SEG ? 1 : TB][2];
+    __shared__ double sred[SEG ? 1 : EPB][2];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -212,6 +224,8 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     const double g = I.g, hx = I.hx, hy = I.hy;
     const double inv_gm1 = 1.0 / (g - 1.0);
     const long long vs = I.vs;
+    if constexpr (!SEG)
+        if (valid && n == 0) sred[el][0] = sred[el][1] = 0.0;
     const gdbl *up[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) up[v] = u + v * vs;
@@ -672,14 +686,14 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double mx_ = seg_max<NN>(ax, t & 63, n), my_ = seg_max<NN>(ay, t & 63, n);
         if (valid && n == 0) dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
     } else {
-        sred[t][0] = ax;
-        sred[t][1] = ay;
-        __syncthreads();
-        if (valid && n == 0) {
-            double mx_ = 0.0, my_ = 0.0;
-            for (int m = 0; m < NN; ++m) { mx_ = dmax(mx_, sred[el * NN + m][0]); my_ = dmax(my_, sred[el * NN + m][1]); }
-            dtl = 1.0 / ((2 * PX + 1) * mx_ / hx + (2 * PY + 1) * my_ / hy);
+        // one LDS atomic maximum per lane into the element's cells (zeroed
+        // at the start of the kernel; the speeds are >= 0)
+        if (valid) {
+            lds_max(&sred[el][0], ax);
+            lds_max(&sred[el][1], ay);
         }
+        __syncthreads();
+        if (valid && n == 0) dtl = 1.0 / ((2 * PX + 1) * sred[el][0] / hx + (2 * PY + 1) * sred[el][1] / hy);
     }
     wave_min_atomic(&I.dtslot[(blockIdx.x % DT_SLOTS) * DT_SLOT_STRIDE], dtl);
 }
@@ -688,10 +702,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 // Minimum over the NN lanes of each element, returned to every lane of the
 // element. When NN divides the wavefront (P1, P3) the element's lanes are one
 // aligned group of a wavefront and a butterfly of lane exchanges does it with
-// no barrier; otherwise (elements straddle wavefronts) through LDS: the
-// element's first lane reduces its NN values and the others read the result
-// (called by every thread of the workgroup). min is exact, so both give the
-// value of the reference's loop.
+// no barrier; otherwise (elements straddle wavefronts) through LDS: one
+// atomic minimum per lane into the element's cell, which the element's
+// first lane has set to DBL_HUGE (called by every thread of the workgroup).
+// min is exact, so both give the value of the reference's loop.
 template <int NN>
 __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, int el, bool valid)
 {
@@ -700,13 +714,9 @@ __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, i
         for (int m = 1; m < NN; m <<= 1) x = dmin(x, __shfl_xor(x, m));
         return x;
     } else {
-        sred[t][0] = x;
+        if (valid && t == el * NN) sred[t][1] = DBL_HUGE;
         __syncthreads();
-        if (valid && t == el * NN) {
-            double r = DBL_HUGE;
-            for (int m = 0; m < NN; ++m) r = dmin(r, sred[el * NN + m][0]);
-            sred[t][1] = r;
-        }
+        if (valid) lds_min(&sred[el * NN][1], x);
         __syncthreads();
         const double r = valid ? sred[el * NN][1] : DBL_HUGE;
         __syncthreads();
@@ -1475,7 +1485,7 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     __shared__ int scnt[EPB];
     __shared__ double sth[EO ? EPB : 1];
     __shared__ double ssig[EPB][16];
-    __shared__ double soth[EPB];  // OEDG damping factor per element (LDS path)
+    __shared__ double smax[64 % NN == 0 ? 1 : EPB][2];  // element's largest signal speeds (LDS path)
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -1504,6 +1514,8 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
         const gdbl *syB = gin(I.sigy) + 8 * ((long long)(i - 1) + (long long)I.nx * (j - 1));
         for (int a = n; a < 16; a += NN)
             ssig[el][a] = a < 8 ? sxL[a] + sxL[8 + a] : syB[a - 8] + syB[8 * (long long)I.nx + a - 8];
+        if constexpr (64 % NN != 0)
+            if (n == 0) smax[el][0] = smax[el][1] = 0.0;
     }
     {
         // the element's damping factor (DG_solver.f90:448-524) from its
@@ -1532,16 +1544,15 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
             __syncthreads();  // the face indicator sums are in LDS
             if (valid) theta = theta_of(mex, mey);
         } else {
-            sred[t][0] = sx;
-            sred[t][1] = sy;
+            // one LDS atomic maximum per lane into the element's cells
+            // (zeroed with the face sums' load; the speeds are >= 0)
             __syncthreads();
-            if (valid && n == 0) {
-                double mex = 0.0, mey = 0.0;
-                for (int m = 0; m < NN; ++m) { mex = dmax(mex, sred[el * NN + m][0]); mey = dmax(mey, sred[el * NN + m][1]); }
-                soth[el] = theta_of(mex, mey);
+            if (valid) {
+                lds_max(&smax[el][0], sx);
+                lds_max(&smax[el][1], sy);
             }
             __syncthreads();
-            if (valid) theta = soth[el];
+            if (valid) theta = theta_of(smax[el][0], smax[el][1]);
         }
         if (valid) {
 #pragma unroll
