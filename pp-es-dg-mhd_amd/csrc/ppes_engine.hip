@@ -292,6 +292,23 @@ int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g)
     return PPES_OK;
 }
 
+// The residual's per-instance constants (Inst::r2hx .. cface), from hx, hy,
+// g and the GLL weights: written as the kernels wrote them, so that each is
+// the same double.
+void record_constants(Inst &r)
+{
+    const int NX = r.px + 1, NY = r.py + 1;
+    r.r2hx = 2.0 / r.hx;
+    r.r2hy = 2.0 / r.hy;
+    r.cx = 2.0 * r.r2hx;
+    r.cy = 2.0 * r.r2hy;
+    r.inv_gm1 = 1.0 / (r.g - 1.0);
+    r.cface[0] = r.r2hx / r.qw_x[0];
+    r.cface[1] = r.r2hx / r.qw_x[NX - 1];
+    r.cface[2] = r.r2hy / r.qw_y[0];
+    r.cface[3] = r.r2hy / r.qw_y[NY - 1];
+}
+
 // Sizes, geometry, gamma and the 1D tables of an instance record: every
 // field the residual reads apart from the buffer pointers and the drive-loop
 // state (DG_solver.f90:47-106). Stack arrays only.
@@ -316,6 +333,7 @@ static void record_base(Inst &r, const ppes_config &cfg, int S)
     for (int i = 0; i < NY; ++i) r.qw_y[i] = qw_y[i];
     for (int i = 0; i < NX * NX; ++i) r.Dx[i] = Dx[i];
     for (int i = 0; i < NY * NY; ++i) r.Dy[i] = Dy[i];
+    record_constants(r);
     for (int l = 0; l < NY; ++l)
         for (int k = 0; k < NX; ++k) {
             r.wavg[k + NX * l] = (0.25 * qw_x[k]) * qw_y[l];

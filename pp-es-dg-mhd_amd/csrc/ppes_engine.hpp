@@ -179,6 +179,10 @@ enum ApiCall { API_BC = 0, API_POST = 1, API_RESID = 2, API_COMBINE = 3, API_GHO
 // PPES_BC_FIXED state.
 int set_bc_rules(Inst &r, int kind, const double *state, int nx, double g);
 
+// The residual's constants of an instance record (Inst::r2hx .. cface) from
+// its hx, hy, g and GLL weights; called again whenever g changes.
+void record_constants(Inst &r);
+
 // DG_semidiscretize of a caller-owned device state into a caller-owned
 // residual, with no allocation: the record and block map live in a fixed
 // static device workspace (ppes_semidiscretize_noalloc). Synchronous.

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     const int i = valid ? (int)(eg % nx) + 1 : 1;
     const int j = valid ? (int)(eg / nx) + 1 : 1;
     const double g = I.g, hx = I.hx, hy = I.hy;
-    const double inv_gm1 = 1.0 / (g - 1.0);
+    const double inv_gm1 = I.inv_gm1;
     const long long vs = I.vs;
     if constexpr (!SEG)
         if (valid && n == 0) sred[el][0] = sred[el][1] = 0.0;
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     double msx = 0.0, msy = 0.0;
     if (valid) {
         const double rho = q[0], mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
-        const double inv_rho = 1.0 / rho;
+        const double inv_rho = recip<FAST>(rho);
         const double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
         const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
         const double b_sq = bx * bx + by * by + bz * bz;
@@ -316,7 +316,8 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         msy = fabs(vy) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (by * by), 0.0)), 0.0));
         PT pr;
         pr.rho = rho; pr.vx = vx; pr.vy = vy; pr.vz = vz; pr.bx = bx; pr.by = by; pr.bz = bz;
-        pr.beta = 0.5 * rho / p_fld;
+        if constexpr (FAST) pr.beta = (0.5 * rho) * rcp_fast(p_fld);
+        else pr.beta = 0.5 * rho / p_fld;
         if constexpr (FAST) {
             // pfld_sym is p_fld's expression: the face phase of the other
             // element of a face forms the same logs from this node's trace
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     // lanes); the component that is identically zero (4 in x, 5 in y) is not
     // stored
     double acc[8];
-    const double cx = 2.0 * (2.0 / hx), cy = 2.0 * (2.0 / hy);
+    const double cx = I.cx, cy = I.cy;
     if (valid) {
         double divB;
         {
@@ -344,12 +345,12 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             for (int r = 0; r < NX; ++r) a += DX(r) * sprim[el][r + NX * l].bx;
 #pragma unroll
             for (int s = 0; s < NY; ++s) b += sprim[el][k + NX * s].by * DY(s);
-            divB = (2.0 / hx) * a + (2.0 / hy) * b;
+            divB = I.r2hx * a + I.r2hy * b;
         }
         const Prim &pr = sprim[el][n];
         const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
         const double vx = pr.vx, vy = pr.vy, vz = pr.vz;
-        const double inv_rho = 1.0 / q[0];
+        const double inv_rho = recip<FAST>(q[0]);
         const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
         const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
         const double b_sq = bx * bx + by * by + bz * bz;
@@ -623,7 +624,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = up[v][base];
         const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];
-        const double inv_rho = 1.0 / q[0];
+        const double inv_rho = recip<FAST>(q[0]);
         const double vx = inv_rho * mx, vy = inv_rho * my, vz = inv_rho * mz;
         const double e_knt = 0.5 * inv_rho * (mx * mx + my * my + mz * mz);
         const double ex = by * vz - bz * vy, ey = bz * vx - bx * vz, ez = bx * vy - by * vx;
@@ -640,14 +641,14 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
                                  0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
             if (k == 0) {
                 const double *o = &sbuf[el][l * S::FREC];
-                const double c = (2.0 / hx) / I.qw_x[0];
+                const double c = I.cface[0];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (F[v] - o[v] + GP[v] * (bx - o[8]));
                 ax = dmax(ax, o[10]);  // sr at the left face
             }
             if (k == NX - 1) {
                 const double *o = &sbuf[el][(NY + l) * S::FREC];
-                const double c = (2.0 / hx) / I.qw_x[NX - 1];
+                const double c = I.cface[1];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (F[v] - o[v] + GP[v] * (bx - o[8]));
                 ax = dmax(ax, -o[9]);  // -sl at the right face
@@ -658,14 +659,14 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
                                  ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
             if (l == 0) {
                 const double *o = &sbuf[el][(2 * NY + k) * S::FREC];
-                const double c = (2.0 / hy) / I.qw_y[0];
+                const double c = I.cface[2];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (G[v] - o[v] + GP[v] * (by - o[8]));
                 ay = dmax(ay, o[10]);
             }
             if (l == NY - 1) {
                 const double *o = &sbuf[el][(2 * NY + NX + k) * S::FREC];
-                const double c = (2.0 / hy) / I.qw_y[NY - 1];
+                const double c = I.cface[3];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (G[v] - o[v] + GP[v] * (by - o[8]));
                 ay = dmax(ay, -o[9]);

@@ -533,6 +533,7 @@ extern "C" int ppes_set_gamma(ppes_solver *s, double gamma)
     Inst &r = s->eng.rec(s->id);
     HostInst &h = s->eng.host(s->id);
     r.g = gamma;
+    record_constants(r);
     h.cfg.gamma = gamma;
     // the built-in ghost states of the cloud-shock and jet fillers depend on gamma
     if (r.bc_kind == PPES_BC_CLOUD || r.bc_kind == PPES_BC_JET) set_bc_rules(r, r.bc_kind, nullptr, r.nx, gamma);

@@ -346,6 +346,28 @@ __device__ __forceinline__ double pfld_sym(const double w[8], double g, double i
 // forms through one log of a / b per pair. u needs far less than full
 // precision (its relative error enters F scaled by u / 3 < 3.4e-3): 1 / (a + b)
 // is the hardware reciprocal refined by two Newton steps.
+// 1 / b for a positive normal b, in the fast arithmetic mode: the hardware
+// reciprocal estimate refined by two Newton steps r <- r + r (1 - b r), each
+// of which squares the relative error, so the result is within about 1 ulp
+// of the correctly rounded quotient; 5 instructions where the IEEE division
+// takes 11 (two scalings, the estimate, five fused multiply-adds, a
+// multiply, the fix-ups).
+__device__ __forceinline__ double rcp_fast(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    return r;
+}
+
+// 1 / b: the correctly rounded division (exact mode) or rcp_fast (fast mode)
+template <bool FAST>
+__device__ __forceinline__ double recip(double b)
+{
+    if constexpr (FAST) return rcp_fast(b);
+    else return 1.0 / b;
+}
+
 struct LogMean {
     bool small;
     double F, d, dl;
@@ -358,10 +380,7 @@ __device__ __forceinline__ LogMean log_mean_parts(double a, double b, double la,
     const double s = a + b;
     m.d = a - b;
     m.dl = la - lb;
-    double r = __builtin_amdgcn_rcp(s);
-    r = __builtin_fma(r, __builtin_fma(-s, r, 1.0), r);
-    r = __builtin_fma(r, __builtin_fma(-s, r, 1.0), r);
-    const double f = m.d * r;
+    const double f = m.d * rcp_fast(s);
     const double u = f * f;
     m.small = u < 1e-2;
     m.F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0 + u * (1.0 / 11.0 +
@@ -386,13 +405,13 @@ __device__ __forceinline__ void ec_flux_fast(const PrimL &L, const PrimL &R, dou
     const double betavx_bar = 0.5 * (L.beta * L.vx + R.beta * R.vx);
     const double betavy_bar = 0.5 * (L.beta * L.vy + R.beta * R.vy);
     const double betavz_bar = 0.5 * (L.beta * L.vz + R.beta * R.vz);
-    const double inv_bb = 1.0 / beta_bar;
+    const double inv_bb = rcp_fast(beta_bar);
     // rho_hat, the logarithmic mean of rho, and 1 / ((g - 1) beta_hat): one
     // division each, of the operands the mean's branch selects
     const LogMean mr = log_mean_parts(L.rho, R.rho, L.lnr, R.lnr);
     const LogMean mb = log_mean_parts(L.beta, R.beta, L.lnb, R.lnb);
-    const double rho_hat = (mr.small ? rho_bar : mr.d) / (mr.small ? mr.F : mr.dl);
-    const double e_coef = ((mb.small ? mb.F : mb.dl) / (mb.small ? beta_bar : mb.d)) * inv_gm1;
+    const double rho_hat = (mr.small ? rho_bar : mr.d) * rcp_fast(mr.small ? mr.F : mr.dl);
+    const double e_coef = ((mb.small ? mb.F : mb.dl) * rcp_fast(mb.small ? beta_bar : mb.d)) * inv_gm1;
     if (DIR == 0) {
         f[0] = rho_hat * vx_bar;
         f[1] = vx_bar * f[0] - bx_bar * bx_bar + 0.5 * (b_sq_bar + rho_bar * inv_bb);
@@ -430,7 +449,7 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
 #pragma clang fp contract(on)
     const double rho_l = wl[0], mx_l = wl[1], my_l = wl[2], mz_l = wl[3], bx_l = wl[4], by_l = wl[5], bz_l = wl[6], e_l = wl[7];
     const double rho_r = wr[0], mx_r = wr[1], my_r = wr[2], mz_r = wr[3], bx_r = wr[4], by_r = wr[5], bz_r = wr[6], e_r = wr[7];
-    const double inv_rho_l = 1.0 / rho_l, inv_rho_r = 1.0 / rho_r;
+    const double inv_rho_l = rcp_fast(rho_l), inv_rho_r = rcp_fast(rho_r);
     const double vx_l = inv_rho_l * mx_l, vx_r = inv_rho_r * mx_r;
     const double vy_l = inv_rho_l * my_l, vy_r = inv_rho_r * my_r;
     const double vz_l = inv_rho_l * mz_l, vz_r = inv_rho_r * mz_r;
@@ -443,7 +462,7 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     const double p_mag_l = 0.5 * b_sq_l, p_mag_r = 0.5 * b_sq_r;
     const double p_fld_l = pfld_sym(wl, g, inv_rho_l);
     const double p_fld_r = pfld_sym(wr, g, inv_rho_r);
-    const double inv_p_l = 1.0 / p_fld_l, inv_p_r = 1.0 / p_fld_r;
+    const double inv_p_l = rcp_fast(p_fld_l), inv_p_r = rcp_fast(p_fld_r);
     const double p_tot_l = p_fld_l + p_mag_l, p_tot_r = p_fld_r + p_mag_r;
     const double b_dot_u_l = bx_l * vx_l + by_l * vy_l + bz_l * vz_l;
     const double b_dot_u_r = bx_r * vx_r + by_r * vy_r + bz_r * vz_r;
@@ -465,7 +484,7 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     fr[3] = vx_r * mz_r - bz_r * bx_r; fr[4] = 0.0; fr[5] = -ez_r; fr[6] = ey_r;
     fr[7] = vx_r * (e_r + p_tot_r) - bx_r * b_dot_u_r;
     const double sqrt_rho_l = sqrt(rho_l), sqrt_rho_r = sqrt(rho_r);
-    const double inv_ssr = 1.0 / (sqrt_rho_l + sqrt_rho_r);
+    const double inv_ssr = rcp_fast(sqrt_rho_l + sqrt_rho_r);
     const double vx_roe = (vx_l * sqrt_rho_l + vx_r * sqrt_rho_r) * inv_ssr;
     const double db_aux = sqrt((bx_l - bx_r) * (bx_l - bx_r) + (by_l - by_r) * (by_l - by_r) + (bz_l - bz_r) * (bz_l - bz_r)) * inv_ssr;
     const double sl_pp = dmin(vx_l, vx_roe) - cfx_l - db_aux;
@@ -488,7 +507,7 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     term(5, inv_theta_l * by_l, inv_theta_r * by_r);
     term(6, inv_theta_l * bz_l, inv_theta_r * bz_r);
     term(7, -inv_theta_l, -inv_theta_r);
-    const double inv_a = 1.0 / (dmax(dot_a, 0.0) + 1e-8);
+    const double inv_a = rcp_fast(dmax(dot_a, 0.0) + 1e-8);
     const double dflux = ent_flux_r - ent_flux_l;
     const double bl_form = dflux - dot_bl - (inv_theta_l * b_dot_u_l) * (bx_r - bx_l);
     const double br_form = dflux - dot_br - (inv_theta_r * b_dot_u_r) * (bx_r - bx_l);
@@ -497,7 +516,7 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     const double aa_form = sqrt(al_form * ar_form);
     const double sl = dmin(dmin(-(ar_form + aa_form), sl_pp), 0.0);
     const double sr = dmax(dmax(al_form + aa_form, sr_pp), 0.0);
-    const double inv_d = 1.0 / (sr - sl);
+    const double inv_d = rcp_fast(sr - sl);
     const double wei_l = sr * inv_d, wei_r = -sl * inv_d, c = (sl * sr) * inv_d;
 #pragma unroll
     for (int i = 0; i < 8; ++i) nflux[i] = wei_l * fl[i] + wei_r * fr[i] + c * (wr[i] - wl[i]);

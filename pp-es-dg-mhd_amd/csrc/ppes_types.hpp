@@ -102,6 +102,12 @@ struct Inst {
     int nx, ny, px, py, S, pad_;
     long long vs, vsi;               // variable strides of state / residual arrays
     double hx, hy, g;
+    // constants of the residual, each rounded once on the host (an IEEE
+    // division gives the same bits there as on the device): 2 / hx, 2 / hy,
+    // 2 (2 / hx), 2 (2 / hy), 1 / (g - 1), and the surface coefficients
+    // (2 / hx) / w_0, (2 / hx) / w_(NX-1), (2 / hy) / w_0, (2 / hy) / w_(NY-1)
+    double r2hx, r2hy, cx, cy, inv_gm1;
+    double cface[4];
     double qw_x[MAXN], qw_y[MAXN];   // GLL weights in x (NX) and y (NY)
     double Dx[MAXN * MAXN];          // Dx[k + NX r] = l_r'(x_k)
     double Dy[MAXN * MAXN];          // Dy[l + NY s]
