@@ -312,8 +312,13 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double alfven_sq = b_sq * inv_rho;
         const double sum_sq = snc_sq + alfven_sq;
         const double coef = 4.0 * snc_sq * inv_rho;
-        msx = fabs(vx) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (bx * bx), 0.0)), 0.0));
-        msy = fabs(vy) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (by * by), 0.0)), 0.0));
+        if constexpr (FAST) {
+            msx = fabs(vx) + sqrt_fast(0.5 * dmax(sum_sq + sqrt_fast(dmax(sum_sq * sum_sq - coef * (bx * bx), 0.0)), 0.0));
+            msy = fabs(vy) + sqrt_fast(0.5 * dmax(sum_sq + sqrt_fast(dmax(sum_sq * sum_sq - coef * (by * by), 0.0)), 0.0));
+        } else {
+            msx = fabs(vx) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (bx * bx), 0.0)), 0.0));
+            msy = fabs(vy) + sqrt(0.5 * dmax(sum_sq + sqrt(dmax(sum_sq * sum_sq - coef * (by * by), 0.0)), 0.0));
+        }
         PT pr;
         pr.rho = rho; pr.vx = vx; pr.vy = vy; pr.vz = vz; pr.bx = bx; pr.by = by; pr.bz = bz;
         if constexpr (FAST) pr.beta = (0.5 * rho) * rcp_fast(p_fld);
