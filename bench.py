@@ -170,7 +170,8 @@ def run_single(ppesdg, steps, warmup):
            "residual_fp64_frac": ALG_FLOPS_PER_NODE[p]["residual"] * nodes / (r[0] / max(r[1], 1) / 1e3) / 1e12
            / FP64_PEAK_TFLOPS,
            "min_rho": float(u[0].min()), "min_p": float(prs.min()), "retries": after["retries"],
-           "residual_workgroup_lanes": tuned, "oedg_face_workgroup_lanes": s.face_tb()}
+           "residual_workgroup_lanes": tuned, "oedg_face_workgroup_lanes": s.face_tb(),
+           "stage_workgroup_lanes": {s.shape: s.tiles("stage")}, "limit_workgroup_lanes": {s.shape: s.tiles("limit")}}
     s.close()
     return res
 
@@ -402,6 +403,10 @@ def main():
     resid_flops_step = 3 * sum(W.nodes(m) * ALG_FLOPS_PER_NODE[m["p"]]["residual"] for m in mix)
     resid_s_step = kt["residual"][0] / a.timing_steps / 1e3
     stage_flops = sum(W.nodes(m) * ALG_FLOPS_PER_NODE[m["p"]]["node_stage"] for m in mix) / nodes_total
+    shapes = {f"P{m['p']}x{m['p']}" for m in mix}
+
+    def present(d):
+        return {k: v for k, v in d.items() if k in shapes}
 
     out = {
         "metric": METRIC,
@@ -421,7 +426,9 @@ def main():
                                f"stage), per-instance CFL dt; batched path, hipGraph step",
                    "instances": len(mix), "nodes_total": nodes_total, "dof_per_node": 8, "path": "batched",
                    "retries": retries, "failed": len(failed), "residual_workgroup_lanes": tuned,
-                   "oedg_face_workgroup_lanes": batch.face_tb()},
+                   "oedg_face_workgroup_lanes": batch.face_tb(),
+                   "stage_workgroup_lanes": present(batch.tiles("stage")),
+                   "limit_workgroup_lanes": present(batch.tiles("limit"))},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(dominant),
                      "avg_launch_ms": avg_s * 1e3, "alg_bytes_per_launch": bytes_per_launch,

@@ -257,11 +257,29 @@ int ppes_autotune(ppes_solver *s, int reps, int *tb_out);
  * do not depend on the size) at 64, 128 and 256 threads per workgroup for
  * every degree present, whatever the face flux, and keeps the fastest; these
  * report the sizes in use by shape id as above (0: not tuned, 256 lanes).
- * The stage and OEDG-limit kernels keep 256 lanes: they fold per-workgroup
- * partial sums (entropy, the OEDG normalisation) whose order a different size
- * would change. */
+ * It also times the stage post-process (post_process, DG_solver.f90:128-661,
+ * with the RK combine) and the OEDG limiter (DG_solver.f90:428-656) at tiles
+ * of one, two, four and eight reduction chunks per workgroup (a chunk: the
+ * elements of one wavefront, of two for P4 elements; at most 512 lanes) and
+ * keeps the fastest per degree. Their partial sums (entropy, the OEDG
+ * normalisation) are formed per chunk in element order, so results do not
+ * depend on the tile either. Solves with ent_offset keep the 256-lane tile. */
 int ppes_batch_face_tb(ppes_batch *b, int *tb_out);
 int ppes_face_tb(ppes_solver *s, int *tb_out);
+/* Workgroup sizes in use (no reference counterpart): lanes_out[sid - 1]
+ * receives, per shape id as above, the lanes per workgroup at which the step
+ * launches kernel class `kernel` (PPES_K_RESIDUAL, PPES_K_STAGE, PPES_K_OEDG,
+ * PPES_K_LIMIT): the autotuner's choice, a size set below, or the default. */
+int ppes_batch_tiles(ppes_batch *b, int kernel, int *lanes_out);
+int ppes_tiles(ppes_solver *s, int kernel, int *lanes_out);
+/* Sets that size for degree pair (px, py) (a solver: its own pair) by hand:
+ * 64, 128 or 256 for the residual (at least the element's node count) and
+ * the OEDG face sums; one of the tile sizes above for the stage and limiter
+ * kernels (P1, P3, P1-P3 x P0: 64, 128, 256, 512; P2: 64, 128, 256, 512;
+ * P4: 128, 256, 512). PPES_BAD_ARG for a size that is not built. Results do
+ * not depend on it. */
+int ppes_batch_set_tile(ppes_batch *b, int kernel, int px, int py, int lanes);
+int ppes_set_tile(ppes_solver *s, int kernel, int lanes);
 
 /* Tables (host only, no device needed): gauss_lobatto / gauss_lobatto_matrix
  * (gauss.f90:17-71,134-331): x(N), w(N), D, Pmn, Pnm (N x N column-major),

@@ -78,6 +78,71 @@ namespace ppes {
         }                                                                                                         \
     } while (0)
 
+// The stage kernel over a group's slot map at the group's tile (stage_m
+// chunks per workgroup), or its ent_offset variant at the default tile; the
+// limiter kernel likewise over the element map (limit_m). Tiles wider than
+// TILE_LANES_MAX lanes are not built.
+template <int PX, int PY, bool EO, int MV>
+static int launch_stage_k(hipStream_t stream, Inst *d_recs, const Group &g, int st, int flags)
+{
+    constexpr int NN = (PX + 1) * (PY + 1);
+    if constexpr (tile_lanes(NN, MV) > TILE_LANES_MAX) {
+        return PPES_BAD_ARG;
+    } else {
+        LAUNCH_TB((k_stage_pp<PX, PY, EO, MV>), g.n_slot, tile_lanes(NN, MV), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, flags);
+        return PPES_OK;
+    }
+}
+template <int PX, int PY>
+static int launch_stage(hipStream_t stream, Inst *d_recs, const Group &g, bool eo, int st, int flags)
+{
+    constexpr int MD = tile_mult_default((PX + 1) * (PY + 1));
+    if (eo) return g.stage_m == MD ? launch_stage_k<PX, PY, true, MD>(stream, d_recs, g, st, flags) : PPES_BAD_ARG;
+    switch (g.stage_m) {
+    case 1: return launch_stage_k<PX, PY, false, 1>(stream, d_recs, g, st, flags);
+    case 2: return launch_stage_k<PX, PY, false, 2>(stream, d_recs, g, st, flags);
+    case 4: return launch_stage_k<PX, PY, false, 4>(stream, d_recs, g, st, flags);
+    case 8: return launch_stage_k<PX, PY, false, 8>(stream, d_recs, g, st, flags);
+    default: return PPES_BAD_ARG;
+    }
+}
+template <int PX, int PY, bool EO, int MV>
+static int launch_limit_k(hipStream_t stream, Inst *d_recs, const Group &g, int st, int flags)
+{
+    constexpr int NN = (PX + 1) * (PY + 1);
+    if constexpr (tile_lanes(NN, MV) > TILE_LANES_MAX) {
+        return PPES_BAD_ARG;
+    } else {
+        LAUNCH_TB((k_oedg_limit<PX, PY, EO, MV>), g.n_el, tile_lanes(NN, MV), d_recs, (const Blk *)g.d_el, st, flags);
+        return PPES_OK;
+    }
+}
+template <int PX, int PY>
+static int launch_limit(hipStream_t stream, Inst *d_recs, const Group &g, bool eo, int st, int flags)
+{
+    constexpr int MD = tile_mult_default((PX + 1) * (PY + 1));
+    if (eo) return g.limit_m == MD ? launch_limit_k<PX, PY, true, MD>(stream, d_recs, g, st, flags) : PPES_BAD_ARG;
+    switch (g.limit_m) {
+    case 1: return launch_limit_k<PX, PY, false, 1>(stream, d_recs, g, st, flags);
+    case 2: return launch_limit_k<PX, PY, false, 2>(stream, d_recs, g, st, flags);
+    case 4: return launch_limit_k<PX, PY, false, 4>(stream, d_recs, g, st, flags);
+    case 8: return launch_limit_k<PX, PY, false, 8>(stream, d_recs, g, st, flags);
+    default: return PPES_BAD_ARG;
+    }
+}
+#define STAGE_DISPATCH(g, eo, st, flags)                                                                    \
+    do {                                                                                                  \
+        int rc_ = PPES_OK;                                                                                \
+        ENG_DISPATCH((g).sid, (rc_ = launch_stage<PX, PY>(stream, d_recs, (g), (eo), (st), (int)(flags)))); \
+        if (rc_) return rc_;                                                                              \
+    } while (0)
+#define LIMIT_DISPATCH(g, eo, st, flags)                                                                    \
+    do {                                                                                                  \
+        int rc_ = PPES_OK;                                                                                \
+        ENG_DISPATCH((g).sid, (rc_ = launch_limit<PX, PY>(stream, d_recs, (g), (eo), (st), (int)(flags)))); \
+        if (rc_) return rc_;                                                                              \
+    } while (0)
+
 // one case per shape id, with PX, PY and the node count NN_ as constants
 // (CALL is substituted in place: it expands to launch syntax whose commas
 // must not pass through another macro)
@@ -99,8 +164,9 @@ namespace ppes {
 
 static long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
-// elements per workgroup of the stage kernels: BLOCK / NN
-static long long epb_of(int sid) { return BLOCK / ((sid_px(sid) + 1) * (sid_py(sid) + 1)); }
+// elements per reduction chunk of a shape (chunk_elems)
+static long long chunk_of(int sid) { return chunk_elems((sid_px(sid) + 1) * (sid_py(sid) + 1)); }
+static int nn_of(int sid) { return (sid_px(sid) + 1) * (sid_py(sid) + 1); }
 
 // ------------------------------------------------------------------ timers
 
@@ -450,9 +516,9 @@ int Engine::add(const ppes_config &cfg, int *id)
         delete h;
         return PPES_BAD_ARG;
     }
-    const long long E = epb_of(h->sid);
-    r.nblk_el = cdiv((long long)nx * ny, E);
-    r.nblk_slot = cdiv((long long)(nx + 2) * (ny + 2), E);
+    const long long E = chunk_of(h->sid);
+    r.nchk_el = cdiv((long long)nx * ny, E);
+    r.nchk_slot = cdiv((long long)(nx + 2) * (ny + 2), E);
     r.ctl.state = ST_IDLE;
     r.ctl.dtlim_bits = 0x7FF0000000000000ULL;
     r.ctl.min_rho_key = r.ctl.min_p_key = POS_KEY_NONE;
@@ -471,9 +537,9 @@ int Engine::add(const ppes_config &cfg, int *id)
     ok = ok && alloc(&r.sigx, 8 * (size_t)(nx + 1) * ny);
     ok = ok && alloc(&r.sigy, 8 * (size_t)nx * (ny + 1));
     ok = ok && alloc(&r.ldf, (size_t)4 * NN * NN);
-    ok = ok && alloc(&r.part, 24 * (size_t)r.nblk_slot);
-    ok = ok && alloc(&r.part2, 24 * (size_t)cdiv(r.nblk_slot, BLOCK));
-    ok = ok && alloc(&r.entpart, (size_t)std::max(r.nblk_el, r.nblk_slot));
+    ok = ok && alloc(&r.part, 24 * (size_t)r.nchk_slot);
+    ok = ok && alloc(&r.part2, 24 * (size_t)cdiv(r.nchk_slot, BLOCK));
+    ok = ok && alloc(&r.entpart, (size_t)std::max(r.nchk_el, r.nchk_slot));
     ok = ok && alloc(&r.ghost0, 8 * (size_t)(2 * nx + 2 * ny) * NN);
     ok = ok && alloc(&r.cent, 2 * (size_t)(nx + 2) * (ny + 2));
     double *dts = nullptr;
@@ -585,7 +651,7 @@ static void strip_order(std::vector<Blk> &out, int id, long long nres, int nx, i
     }
 }
 
-int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool)
+int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool, bool tuned)
 {
     g.sid = sid;
     g.members = members;
@@ -593,20 +659,23 @@ int Engine::build_group(Group &g, const std::vector<int> &members, int sid, MapP
     for (int id : members) g.eo = g.eo || hosts[id]->ent_offset;
     g.res_tb = residual_tb(sid, members);
     g.face_tb = face_tb(sid);
+    g.stage_m = tuned ? stage_mult(sid, g.eo) : tile_mult_default(nn_of(sid));
+    g.limit_m = tuned ? limit_mult(sid, g.eo) : tile_mult_default(nn_of(sid));
+    const long long est = g.stage_m * chunk_of(sid), elt = g.limit_m * chunk_of(sid);
     std::vector<Blk> el, sl, fc, bc, r1, r2, rs, rsf[4];
     for (int id : members) {
         const Inst &r = h_recs[id];
         const int NN = (sid_px(sid) + 1) * (sid_py(sid) + 1);
-        for (long long b = 0; b < r.nblk_el; ++b) el.push_back({id, (int)b});
+        for (long long b = 0; b < cdiv((long long)r.nx * r.ny, elt); ++b) el.push_back({id, (int)b});
         const long long nres = cdiv((long long)r.nx * r.ny, (long long)(g.res_tb / NN));
         strip_order(rsf[hosts[id]->flux], id, nres, r.nx, r.ny, g.res_tb / NN, NN);
-        for (long long b = 0; b < r.nblk_slot; ++b) sl.push_back({id, (int)b});
+        for (long long b = 0; b < cdiv((long long)(r.nx + 2) * (r.ny + 2), est); ++b) sl.push_back({id, (int)b});
         // x-face workgroups, then y-face ones (the kernel's split)
         const long long nf = cdiv((long long)(r.nx + 1) * r.ny * 8, g.face_tb) + cdiv((long long)r.nx * (r.ny + 1) * 8, g.face_tb);
         for (long long b = 0; b < nf; ++b) fc.push_back({id, (int)b});
         const long long nb = (2LL * r.nx + 2LL * r.ny) * NN;
         for (long long b = 0; b < cdiv(nb, BLOCK); ++b) bc.push_back({id, (int)b});
-        for (long long b = 0; b < cdiv(r.nblk_slot, BLOCK); ++b) r1.push_back({id, (int)b});
+        for (long long b = 0; b < cdiv(r.nchk_slot, BLOCK); ++b) r1.push_back({id, (int)b});
         r2.push_back({id, 0});
     }
     // (the OEDG face map keeps dispatch order: measured slower when reordered)
@@ -812,11 +881,7 @@ int Engine::launch_step(bool all_kernels)
     for (int st = 1; st <= S; ++st) {
         for (auto &g : groups) {
             Timed tm(this, KT_STAGE);
-            if (g.eo) {
-                ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, true>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
-            } else {
-                ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
-            }
+            STAGE_DISPATCH(g, g.eo, st, F_COMBINE);
         }
         if (any_oedg_stage[st]) {
             for (auto &g0 : groups) {
@@ -834,11 +899,7 @@ int Engine::launch_step(bool all_kernels)
                 }
                 {
                     Timed tm(this, KT_LIMIT);
-                    if (g.eo) {
-                        ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
-                    } else {
-                        ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, false>), g.n_el, d_recs, (const Blk *)g.d_el, st, 0));
-                    }
+                    LIMIT_DISPATCH(g, g.eo, st, 0);
                 }
             }
         }
@@ -886,21 +947,13 @@ int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
         RESIDUAL_DISPATCH(g, 0, 1, 1);
     }
     for (int st = 1; st <= S; ++st) {
-        if (g.eo) {
-            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, true>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
-        } else {
-            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, (int)F_COMBINE));
-        }
+        STAGE_DISPATCH(g, g.eo, st, F_COMBINE);
         if (const Group *og = oedg_group(g, st)) {
             const Group &o = *og;
             LAUNCH(k_reduce_global, o.n_red1, d_recs, (const Blk *)o.d_red1, st, 0, 0);
             LAUNCH(k_reduce_global, (long long)o.members.size(), d_recs, (const Blk *)o.d_red2, st, 0, 1);
             FACES_DISPATCH(o, st, 0);
-            if (o.eo) {
-                ENG_DISPATCH(o.sid, LAUNCH((k_oedg_limit<PX, PY, true>), o.n_el, d_recs, (const Blk *)o.d_el, st, 0));
-            } else {
-                ENG_DISPATCH(o.sid, LAUNCH((k_oedg_limit<PX, PY, false>), o.n_el, d_recs, (const Blk *)o.d_el, st, 0));
-            }
+            LIMIT_DISPATCH(o, o.eo, st, 0);
         }
         if (st == S) LAUNCH(k_reduce_ent, (long long)nm, d_recs, dl, st, 0);
         ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, st));
@@ -918,6 +971,57 @@ int Engine::residual_tb(int sid, const std::vector<int> &members) const
     for (int id : members)
         if (hosts[id]->flux != FLUX_ESPP_HLL) return def;
     return tuned_tb[sid];
+}
+
+static bool tile_built(int sid, int m) { return (m == 1 || m == 2 || m == 4 || m == 8) && tile_lanes(nn_of(sid), m) <= TILE_LANES_MAX; }
+
+int Engine::stage_mult(int sid, bool eo) const
+{
+    return (!eo && tuned_stage_m[sid]) ? tuned_stage_m[sid] : tile_mult_default(nn_of(sid));
+}
+
+int Engine::limit_mult(int sid, bool eo) const
+{
+    return (!eo && tuned_limit_m[sid]) ? tuned_limit_m[sid] : tile_mult_default(nn_of(sid));
+}
+
+int Engine::tile_of(int kernel, int sid) const
+{
+    if (sid < 1 || sid > 8) return 0;
+    switch (kernel) {
+    case KT_RESIDUAL: return arith == PPES_ARITH_FAST && tuned_tb[sid] ? tuned_tb[sid] : resid_tb(nn_of(sid));
+    case KT_STAGE: return tile_lanes(nn_of(sid), stage_mult(sid, false));
+    case KT_OEDG: return face_tb(sid);
+    case KT_LIMIT: return tile_lanes(nn_of(sid), limit_mult(sid, false));
+    default: return 0;
+    }
+}
+
+int Engine::set_tile(int kernel, int sid, int lanes)
+{
+    if (sid < 1 || sid > 8) return PPES_BAD_ARG;
+    const int nn = nn_of(sid);
+    if (kernel == KT_RESIDUAL || kernel == KT_OEDG) {
+        if (lanes != 64 && lanes != 128 && lanes != 256) return PPES_BAD_ARG;
+        if (kernel == KT_RESIDUAL) {
+            if (lanes < nn) return PPES_BAD_ARG;
+            tuned_tb[sid] = lanes;
+        } else {
+            tuned_face_tb[sid] = lanes;
+        }
+    } else if (kernel == KT_STAGE || kernel == KT_LIMIT) {
+        int m = 0;
+        for (int c : {1, 2, 4, 8})
+            if (tile_built(sid, c) && tile_lanes(nn, c) == lanes) m = c;
+        if (!m) return PPES_BAD_ARG;
+        (kernel == KT_STAGE ? tuned_stage_m : tuned_limit_m)[sid] = m;
+    } else {
+        return PPES_BAD_ARG;
+    }
+    built_key.clear();
+    plan_ok = false;
+    graph_ok = false;
+    return PPES_OK;
 }
 
 int Engine::autotune(int reps, int *tb_out)
@@ -1022,6 +1126,62 @@ int Engine::autotune(int reps, int *tb_out)
         }
         if (!err) tuned_face_tb[g.sid] = best;
     }
+    // the stage and limiter kernels at each tile: the first stage of the
+    // step to come (u^n and L(u^n) into state slot 1; its flags, counters
+    // and positivity keys go back with the records), and the limiter of the
+    // final stage in place on slot S (the old u^n since the commit) with
+    // the face indicators and OEDG norms the last step left; between the
+    // timed launches the per-step flags are cleared, so that a launch never
+    // finds an instance already marked for a retry
+    for (int kern = 0; kern < 2 && !err; ++kern) {
+        for (const Group &g : groups) {
+            if (err) break;
+            if (g.eo) continue;  // the ent_offset variants keep the default tile
+            if (kern == 1 && !g.oedg_final && g.n_every == 0) continue;  // no limiter launches
+            int best = 0;
+            float best_ms = 0.0f;
+            for (int m : {1, 2, 4, 8}) {
+                if (!tile_built(g.sid, m)) continue;
+                const long long E = m * chunk_of(g.sid);
+                std::vector<Blk> mp;
+                for (int id : g.members) {
+                    const Inst &r = h_recs[id];
+                    const long long n = kern == 0 ? (long long)(r.nx + 2) * (r.ny + 2) : (long long)r.nx * r.ny;
+                    for (long long b2 = 0; b2 < cdiv(n, E); ++b2) mp.push_back({id, (int)b2});
+                }
+                Blk *d = nullptr;
+                if (hipMalloc((void **)&d, std::max<size_t>(mp.size(), 1) * sizeof(Blk)) != hipSuccess) { err = PPES_OOM; break; }
+                Group t = g;
+                if (kern == 0) { t.stage_m = m; t.d_slot = d; t.n_slot = (long long)mp.size(); }
+                else { t.limit_m = m; t.d_el = d; t.n_el = (long long)mp.size(); }
+                const int nm = (int)g.members.size();
+                float ms = 0.0f;
+                auto once = [&]() -> int {
+                    if (kern == 0) STAGE_DISPATCH(t, false, 1, F_COMBINE);
+                    else LIMIT_DISPATCH(t, false, S, F_FORCE_OEDG);
+                    LAUNCH1(k_reset_flags, nm, d_recs, (const int *)g.d_list, nm, 4);
+                    return PPES_OK;
+                };
+                auto run = [&]() -> int {
+                    ENG_TRY(hipMemcpy(d, mp.data(), mp.size() * sizeof(Blk), hipMemcpyHostToDevice));
+                    int rc2 = once();  // warm-up
+                    if (rc2) return rc2;
+                    ENG_TRY(hipEventRecord(e0, stream));
+                    for (int r = 0; r < reps && !rc2; ++r) rc2 = once();
+                    ENG_TRY(hipEventRecord(e1, stream));
+                    ENG_TRY(hipEventSynchronize(e1));
+                    ENG_TRY(hipEventElapsedTime(&ms, e0, e1));
+                    return rc2;
+                };
+                err = run();
+                (void)hipStreamSynchronize(stream);
+                (void)hipFree(d);
+                if (err) break;
+                if (best == 0 || ms < best_ms) { best = m; best_ms = ms; }
+            }
+            if (!err && best) (kern == 0 ? tuned_stage_m : tuned_limit_m)[g.sid] = best;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!err) err = push();  // the records as they were before the timing runs
@@ -1114,15 +1274,13 @@ int Engine::launch_startup(const std::vector<int> &ids)
     for (int sid = 1; sid <= 8 && rc == PPES_OK; ++sid) {
         if (by_sid[sid].empty()) continue;
         Group g;
-        rc = build_group(g, by_sid[sid], sid, &start_pools[sid]);
+        rc = build_group(g, by_sid[sid], sid, &start_pools[sid], false);
         tmp.push_back(g);
     }
     if (rc == PPES_OK) {
         LAUNCH1(k_reset_flags, n, d_recs, (const int *)d_ids, n, 1 | 2 | 4);
         for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
-        for (auto &g : tmp)
-            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, 0,
-                                     (int)(F_GHOST_MEM | F_FORCE_NO_OEDG | F_ENT)));
+        for (auto &g : tmp) STAGE_DISPATCH(g, false, 0, F_GHOST_MEM | F_FORCE_NO_OEDG | F_ENT);
         for (auto &g : tmp) LAUNCH(k_reduce_ent, (long long)g.members.size(), d_recs, (const int *)g.d_list, 0, (int)(F_FORCE_NO_OEDG | F_ENT));
         for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         for (auto &g : tmp) ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
@@ -1158,7 +1316,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
     if (g.d_el && (g.res_tb != residual_tb(hosts[id]->sid, std::vector<int>{id}) || g.face_tb != face_tb(hosts[id]->sid)))
         free_group(g);
     if (!g.d_el) {
-        int rc = build_group(g, std::vector<int>{id}, hosts[id]->sid);
+        int rc = build_group(g, std::vector<int>{id}, hosts[id]->sid, nullptr, false);
         if (rc) { free_group(g); return rc; }
     }
     int rc = PPES_OK;
@@ -1174,7 +1332,7 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         ENG_DISPATCH(g.sid, LAUNCH(k_ghost_copy<NN_>, g.n_bc, d_recs, (const Blk *)g.d_bc, 0));
         break;
     case API_COMBINE:
-        ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, (int)(F_COMBINE | F_COMBINE_ONLY)));
+        STAGE_DISPATCH(g, false, slot, F_COMBINE | F_COMBINE_ONLY);
         break;
     case API_RESID:
         LAUNCH1(k_pre_residual, 1, d_recs, (const int *)d_ids, 1, 0);
@@ -1183,20 +1341,12 @@ int Engine::launch_api(int id, int what, int slot, int flags, int ut_slot)
         break;
     case API_POST: {
         const bool eo = h_recs[id].ctl.ent_offset != 0;
-        if (eo) {
-            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, true>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
-        } else {
-            ENG_DISPATCH(g.sid, LAUNCH((k_stage_pp<PX, PY, false>), stage_blocks(0, g.n_slot), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, slot, flags));
-        }
+        STAGE_DISPATCH(g, eo, slot, flags);
         if (flags & F_FORCE_OEDG) {
             LAUNCH(k_reduce_global, g.n_red1, d_recs, (const Blk *)g.d_red1, slot, flags, 0);
             LAUNCH(k_reduce_global, 1, d_recs, (const Blk *)g.d_red2, slot, flags, 1);
             FACES_DISPATCH(g, slot, flags);
-            if (eo) {
-                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, true>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
-            } else {
-                ENG_DISPATCH(g.sid, LAUNCH((k_oedg_limit<PX, PY, false>), g.n_el, d_recs, (const Blk *)g.d_el, slot, flags));
-            }
+            LIMIT_DISPATCH(g, eo, slot, flags);
         }
         if (flags & F_ENT) LAUNCH(k_reduce_ent, 1, d_recs, (const int *)d_ids, slot, flags);
         break;

@@ -41,6 +41,9 @@ struct Group {
     int *d_list = nullptr;
     int res_tb = 0;  // workgroup size of the residual launches (the residual map's elements per block follow)
     int face_tb = 0;  // workgroup size of the OEDG face launches (the face map's blocks follow)
+    // tiles of the stage and limiter kernels: reduction chunks per
+    // workgroup (the slot and element maps' blocks follow)
+    int stage_m = 0, limit_m = 0;
     long long n_el = 0, n_slot = 0, n_face = 0, n_bc = 0, n_red1 = 0, n_res = 0;
     bool eo = false;  // some member solves with ent_offset: the stage kernels' EO variants
     // OEDG participation: some member applies it on the final stage; the
@@ -96,7 +99,6 @@ public:
     int run_steps(long long nsteps, bool use_graph, long long *done_steps);
     int sync_every = 1;                                          // steps queued per host synchronisation
     int arith = PPES_ARITH_FAST;                                 // residual arithmetic (PPES_ARITH_*)
-    long long stage_blocks(int, long long nmap) const { return nmap; }  // one element group per workgroup
     void set_arith(int a)
     {
         arith = a;
@@ -104,17 +106,32 @@ public:
         built_key.clear();  // the residual maps depend on the mode (tuned sizes are fast-mode only)
         plan_ok = false;
     }
-    // Residual autotuner: for each degree group of the current plan whose
-    // members all use the fast ES-HLL residual, time the residual of the
-    // current states at every candidate workgroup size (64, 128, 256 lanes,
-    // so 64/N^2 to 256/N^2 elements per workgroup) and keep the fastest for
-    // that shape; tb_out[sid - 1] receives it (0: not tuned). Recomputing
-    // L(u^n) into its own slot leaves the states and dt_lim as they were.
+    // Autotuner, for each degree group of the current plan: the residual
+    // (when every member uses the fast ES-HLL residual) at 64, 128 and 256
+    // lanes per workgroup, the OEDG face sums at 64, 128 and 256, and the
+    // stage and limiter kernels at one, two, four and eight reduction chunks
+    // per workgroup (up to TILE_LANES_MAX lanes), each timed on the current
+    // states and kept when fastest for that shape; tb_out[sid - 1] receives
+    // the residual's choice (0: not tuned). The timing runs write only
+    // scratch slots (L(u^n) into residual slot 1, the first stage into state
+    // slot 1, the limiter into slot S, which the commit left holding the old
+    // u^n) and the instance records are put back afterwards.
     int autotune(int reps, int *tb_out);
     int tuned_tb[9] = {};  // by shape id; 0: the default size
     int residual_tb(int sid, const std::vector<int> &members) const;
     int tuned_face_tb[9] = {};  // OEDG face workgroup size by shape id; 0: BLOCK
     int face_tb(int sid) const { return arith == PPES_ARITH_FAST && tuned_face_tb[sid] ? tuned_face_tb[sid] : BLOCK; }
+    // chunks per workgroup of the stage and limiter kernels by shape id (0:
+    // the 256-lane default); the ent_offset variants and the one-instance
+    // maps of the reference-API calls keep the default
+    int tuned_stage_m[9] = {}, tuned_limit_m[9] = {};
+    int stage_mult(int sid, bool eo) const;
+    int limit_mult(int sid, bool eo) const;
+    // the workgroup size (lanes) a kernel class (KT_RESIDUAL, KT_STAGE,
+    // KT_OEDG, KT_LIMIT) runs at for a shape id in the step plan, and a
+    // manual choice of it (PPES_BAD_ARG if no variant of that size is built)
+    int tile_of(int kernel, int sid) const;
+    int set_tile(int kernel, int sid, int lanes);
     int launch_api(int id, int what, int slot, int flags, int ut_slot);  // single reference-API calls
     int set_flux(int id, int kind);                                      // face flux of one instance (FluxKind)
 
@@ -136,7 +153,8 @@ public:
 private:
     int ensure_capacity(int n);
     int build_plan();
-    int build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool = nullptr);
+    // tuned = false: the default tiles of the stage and limiter kernels
+    int build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool = nullptr, bool tuned = true);
     MapPool plan_pools[9], start_pools[9];  // by shape id: the step plan's maps, the start-up's
     MapPool early_pools[9];                 // by shape id: the every-stage OEDG sub-plans' maps
     std::vector<Group> early_groups;        // every-stage OEDG members of a group, when not all of them

@@ -817,13 +817,14 @@ __device__ __forceinline__ void record_positivity(Inst &I, bool on, const double
 }
 
 // Cell entropy of one element summed in the reference's node order, then the
-// workgroup's sum over its counted elements in element order; thread 0
-// receives the workgroup partial (DG_solver.f90:635-649).
-template <int NN>
+// sum over the counted elements of each reduction chunk of the workgroup
+// (CE consecutive elements) in element order (DG_solver.f90:635-649); thread
+// c < EPB / CE receives chunk c's partial, so the partials do not depend on
+// how many chunks a workgroup holds.
+template <int NN, int EPB, int CE>
 __device__ __forceinline__ double entropy_partial(const double q[8], bool valid, bool counted, const Inst &I,
                                                   double (*sred)[2], double *sent, int *scnt, int t, int el, int n)
 {
-    constexpr int EPB = BLOCK / NN;
     const double g = I.g;
     sred[t][0] = valid ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
                        : 0.0;
@@ -837,8 +838,8 @@ __device__ __forceinline__ double entropy_partial(const double q[8], bool valid,
     }
     __syncthreads();
     double s = 0.0;
-    if (t == 0)
-        for (int e = 0; e < EPB; ++e)
+    if (t < EPB / CE)
+        for (int e = t * CE; e < (t + 1) * CE; ++e)
             if (scnt[e]) s = s + sent[e];
     return s;
 }
@@ -847,11 +848,10 @@ __device__ __forceinline__ double entropy_partial(const double q[8], bool valid,
 // before LDF, summed in the reference's node order, NaN unless every node has
 // density and pressure >= PP_eps; with the entropy of the cell mean. Lane
 // n == 0 of each valid element writes them to sce[el]. All lanes call.
-template <int NN>
+template <int NN, int EPB>
 __device__ __forceinline__ void pre_ldf_entropy(const double q[8], const double avg[8], bool valid, const Inst &I,
                                                 double (*sred)[2], double (*sce)[2], int t, int el, int n)
 {
-    constexpr int EPB = BLOCK / NN;
     const double g = I.g;
     double c = 0.0, bad = 0.0;
     if (valid) {
@@ -879,12 +879,11 @@ __device__ __forceinline__ void pre_ldf_entropy(const double q[8], const double 
 // entropy exceeds its value before LDF (ent_old; NaN: skip), scale the cell
 // toward its mean by theta = (ent_old - ent_avg) / (ent_new - ent_avg)
 // clamped to [0, 1]. All lanes call; sth is per-element LDS.
-template <int NN>
+template <int NN, int EPB>
 __device__ __forceinline__ void entropy_offset(double q[8], const double avg[8], bool valid, double ent_old,
                                                double ent_avg, const Inst &I, double (*sred)[2], double *sth, int t,
                                                int el, int n)
 {
-    constexpr int EPB = BLOCK / NN;
     const double g = I.g;
     const bool on = valid && !__builtin_isnan(ent_old);
     sred[t][0] = on ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
@@ -965,16 +964,16 @@ __device__ __forceinline__ double ghost_value(int mode, int v, double x)
 
 // four waves per SIMD asked of the register allocator: the P4 variant's MFMA
 // accumulators then stay in VGPRs (130 registers with AGPRs would allow three)
-template <int PX, int PY, bool EO>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
+template <int PX, int PY, bool EO, int MT_ = tile_mult_default((PX + 1) * (PY + 1))>
+__global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attribute__((amdgpu_waves_per_eu(4))) k_stage_pp(Inst *insts, const Blk *map, int nmap, int st, int flags)
 {
     using S = Shape<PX, PY>;
-    constexpr int NX = S::NX, NY = S::NY, NN = S::NN, EPB = S::EPB;
+    constexpr int NX = S::NX, NY = S::NY, NN = S::NN, CE = S::CE, EPB = MT_ * CE, TB = tile_lanes(NN, MT_);
     constexpr int M = 2 * NN;
     // scons (node values: averages, OEDG extrema), sdb (LDF input) and sred
     // (limiter, entropy) are never live at the same time: one region, which
     // leaves room for a fifth workgroup per CU
-    constexpr int U1 = EPB * 8 * NN, U2 = EPB * M, U3 = BLOCK * 2;
+    constexpr int U1 = EPB * 8 * NN, U2 = EPB * M, U3 = TB * 2;
     constexpr int UN = U1 > U2 ? (U1 > U3 ? U1 : U3) : (U2 > U3 ? U2 : U3);
     __shared__ double sunion[UN];
     double(*scons)[8][NN] = reinterpret_cast<double(*)[8][NN]>(sunion);
@@ -1015,15 +1014,15 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     // the operator's loads are issued here, before the state's, and stored to
     // LDS after them, so that both sets are in flight together (a loop that
     // stored each entry before loading the next would wait on memory
-    // M^2 / BLOCK times in a row)
+    // M^2 / TB times in a row)
     const bool stage_ldf = use_ldf && !(flags & F_COMBINE_ONLY);
-    constexpr int R = (M * M + BLOCK - 1) / BLOCK;
+    constexpr int R = (M * M + TB - 1) / TB;
     double ax[R];
     if (stage_ldf) {
         const gdbl *ldf = gin(I.ldf);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int a = t + r * BLOCK;
+            const int a = t + r * TB;
             if (a < M * M) ax[r] = ldf[a];
         }
     }
@@ -1063,7 +1062,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     if (stage_ldf) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int a = t + r * BLOCK;
+            const int a = t + r * TB;
             if (a < M * M) sldf[a] = ax[r];
         }
     }
@@ -1093,7 +1092,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     // ---- ent_offset: the cell entropy before LDF (workgroup-uniform switch)
     const bool eo = EO && I.ctl.ent_offset != 0;
     if constexpr (EO)
-        if (eo) pre_ldf_entropy<NN>(q, avg, valid && ok, I, sred, sce, t, el, n);
+        if (eo) pre_ldf_entropy<NN, EPB>(q, avg, valid && ok, I, sred, sce, t, el, n);
 
     // ---- LDF projection of (Bx, By) as one dense nodal operator (LDS-staged)
     if (use_ldf) {
@@ -1111,7 +1110,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         // the waves; rows, columns and inputs past M and EPB are zero.
         {
             constexpr int MT = (M + 15) / 16, ET = (EPB + 15) / 16, KT = (M + 3) / 4;
-            constexpr int JOBS = MT * ET, WAVES = BLOCK / 64, JPW = (JOBS + WAVES - 1) / WAVES;
+            constexpr int JOBS = MT * ET, WAVES = TB / 64, JPW = (JOBS + WAVES - 1) / WAVES;
             const int lane = t & 63, wv = t >> 6, lo = lane & 15, kq = lane >> 4;
             f64x4 yacc[JPW];
 #pragma unroll
@@ -1202,16 +1201,19 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
             }
         }
         __syncthreads();
-        if (t < 8) {
-            const int v = t;
+        // one record per reduction chunk of the workgroup: thread 8 c + v
+        // sums variable v over chunk c's elements in element order
+        const long long c = (long long)bm.local * MT_ + t / 8;
+        if (t < 8 * MT_ && c < I.nchk_slot) {
+            const int v = t % 8, e0 = (t / 8) * CE;
             double s = 0.0, mn = DBL_HUGE, mxv = -DBL_HUGE;
-            for (int e = 0; e < EPB; ++e) {
+            for (int e = e0; e < e0 + CE; ++e) {
                 if (!scnt[e]) continue;
                 s = s + sem[0][e][v];
                 mn = dmin(mn, sem[1][e][v]);
                 mxv = dmax(mxv, sem[2][e][v]);
             }
-            gdbl *part = gout(I.part) + (long long)bm.local * 24;
+            gdbl *part = gout(I.part) + c * 24;
             part[v] = s;
             part[8 + v] = mn;
             part[16 + v] = mxv;
@@ -1228,10 +1230,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
     }
     if constexpr (EO)
-        if (eo) entropy_offset<NN>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
+        if (eo) entropy_offset<NN, EPB>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
     if (stage_ent(I, st, flags)) {
-        const double s = entropy_partial<NN>(q, lim, lim, I, sred, sent, scnt, t, el, n);
-        if (t == 0) gout(I.entpart)[bm.local] = s;
+        const double s = entropy_partial<NN, EPB, CE>(q, lim, lim, I, sred, sent, scnt, t, el, n);
+        const long long c = (long long)bm.local * MT_ + t;
+        if (t < MT_ && c < I.nchk_slot) gout(I.entpart)[c] = s;
     }
     if (valid && ok)
 #pragma unroll
@@ -1239,7 +1242,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     record_positivity(I, lim, q);
 }
 
-// Two-level deterministic reduction of the per-workgroup (sum, min, max)
+// Two-level deterministic reduction of the per-chunk (sum, min, max)
 // records into u_avg_global and u_vari_global (DG_solver.f90:228,282-294):
 // level 1 (final = 0): workgroup (inst, chunk) reduces records
 // [256 chunk, 256 chunk + 256) by a fixed tree; level 2 (final = 1): one
@@ -1252,7 +1255,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk 
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const int t = threadIdx.x;
-    const long long nrec1 = (I.nblk_slot + BLOCK - 1) / BLOCK;
+    const long long nrec1 = (I.nchk_slot + BLOCK - 1) / BLOCK;
     double s[8], mn[8], mx[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) { s[v] = 0.0; mn[v] = DBL_HUGE; mx[v] = -DBL_HUGE; }
@@ -1266,7 +1269,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk 
             }
     } else {
         const long long r = (long long)bm.local * BLOCK + t;
-        if (r < I.nblk_slot)
+        if (r < I.nchk_slot)
 #pragma unroll
             for (int v = 0; v < 8; ++v) { s[v] = I.part[r * 24 + v]; mn[v] = I.part[r * 24 + 8 + v]; mx[v] = I.part[r * 24 + 16 + v]; }
     }
@@ -1481,12 +1484,12 @@ __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map,
 
 // OEDG path, limiter loop (DG_solver.f90:428-656): OEDG damping toward the
 // cell mean, the positivity limiter, the cell entropy, the final state.
-template <int PX, int PY, bool EO>
-__global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *map, int st, int flags)
+template <int PX, int PY, bool EO, int MT_ = tile_mult_default((PX + 1) * (PY + 1))>
+__global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_limit(Inst *insts, const Blk *map, int st, int flags)
 {
     using S = Shape<PX, PY>;
-    constexpr int NN = S::NN, EPB = S::EPB;
-    __shared__ double sred[BLOCK][2];
+    constexpr int NN = S::NN, CE = S::CE, EPB = MT_ * CE, TB = tile_lanes(NN, MT_);
+    __shared__ double sred[TB][2];
     __shared__ double sent[EPB];
     __shared__ int scnt[EPB];
     __shared__ double sth[EO ? EPB : 1];
@@ -1575,11 +1578,12 @@ __global__ void __launch_bounds__(BLOCK) k_oedg_limit(Inst *insts, const Blk *ma
     if constexpr (EO)
         if (I.ctl.ent_offset) {
             const gdbl *ce = gin(I.cent) + ge * 2;
-            entropy_offset<NN>(q, avg, valid, valid ? ce[0] : 0.0, valid ? ce[1] : 0.0, I, sred, sth, t, el, n);
+            entropy_offset<NN, EPB>(q, avg, valid, valid ? ce[0] : 0.0, valid ? ce[1] : 0.0, I, sred, sth, t, el, n);
         }
     if (stage_ent(I, st, flags)) {
-        const double s = entropy_partial<NN>(q, valid, valid, I, sred, sent, scnt, t, el, n);
-        if (t == 0) gout(I.entpart)[bm.local] = s;
+        const double s = entropy_partial<NN, EPB, CE>(q, valid, valid, I, sred, sent, scnt, t, el, n);
+        const long long c = (long long)bm.local * MT_ + t;
+        if (t < MT_ && c < I.nchk_el) gout(I.entpart)[c] = s;
     }
     if (valid) {
 #pragma unroll
@@ -1595,7 +1599,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_ent(Inst *insts, const int *li
     __shared__ double ss[BLOCK];
     Inst &I = insts[list[blockIdx.x]];
     if (!inst_live(I) || !stage_ent(I, st, flags)) return;
-    const long long n = stage_oedg(I, st, flags) ? I.nblk_el : I.nblk_slot;
+    const long long n = stage_oedg(I, st, flags) ? I.nchk_el : I.nchk_slot;
     const int t = threadIdx.x;
     const long long per = (n + BLOCK - 1) / BLOCK;
     const long long b0 = t * per, b1 = (b0 + per < n) ? b0 + per : n;

@@ -483,12 +483,12 @@ extern "C" long long ppes_instance_bytes(const ppes_config *cfg)
 {
     if (!cfg || shape_id(cfg->px, cfg->py) < 0 || cfg->nx < 1 || cfg->ny < 1) return -1;
     const long long NN = (long long)(cfg->px + 1) * (cfg->py + 1), nx = cfg->nx, ny = cfg->ny, S = cfg->rk_stages;
-    const long long E = BLOCK / NN;
+    const long long E = chunk_elems((int)NN);
     const long long vs = (nx + 2) * (ny + 2) * NN, vsi = nx * ny * NN;
-    const long long nblk_el = cdiv(nx * ny, E), nblk_slot = cdiv((nx + 2) * (ny + 2), E);
+    const long long nchk_el = cdiv(nx * ny, E), nchk_slot = cdiv((nx + 2) * (ny + 2), E);
     const long long doubles = 8 * vs * (S + 1) + 8 * vsi * S + 8 * (nx + 2) * (ny + 2) + 8 * (nx + 1) * ny +
-                              8 * nx * (ny + 1) + 4 * NN * NN + 24 * nblk_slot + 24 * cdiv(nblk_slot, BLOCK) +
-                              std::max(nblk_el, nblk_slot) + 8 * (2 * nx + 2 * ny) * NN + 2 * (nx + 2) * (ny + 2) +
+                              8 * nx * (ny + 1) + 4 * NN * NN + 24 * nchk_slot + 24 * cdiv(nchk_slot, BLOCK) +
+                              std::max(nchk_el, nchk_slot) + 8 * (2 * nx + 2 * ny) * NN + 2 * (nx + 2) * (ny + 2) +
                               (long long)DT_SLOTS * DT_SLOT_STRIDE;
     return 8 * doubles;
 }
@@ -968,6 +968,39 @@ extern "C" int ppes_face_tb(ppes_solver *s, int *tb_out)
     if (!s || !tb_out) return PPES_BAD_ARG;
     face_sizes(s->eng, tb_out);
     return PPES_OK;
+}
+
+static int engine_tiles(const Engine &e, int kernel, int *lanes_out)
+{
+    const int k = timer_of(kernel);
+    if (!lanes_out || k < 0 || k > KT_LIMIT) return PPES_BAD_ARG;
+    for (int sid = 1; sid <= 8; ++sid) lanes_out[sid - 1] = e.tile_of(k, sid);
+    return PPES_OK;
+}
+
+extern "C" int ppes_batch_tiles(ppes_batch *b, int kernel, int *lanes_out)
+{
+    return b ? engine_tiles(b->eng, kernel, lanes_out) : PPES_BAD_ARG;
+}
+
+extern "C" int ppes_tiles(ppes_solver *s, int kernel, int *lanes_out)
+{
+    return s ? engine_tiles(s->eng, kernel, lanes_out) : PPES_BAD_ARG;
+}
+
+extern "C" int ppes_batch_set_tile(ppes_batch *b, int kernel, int px, int py, int lanes)
+{
+    const int k = timer_of(kernel);
+    const int sid = shape_id(px, py);
+    if (!b || k < 0 || k > KT_LIMIT || sid < 0) return PPES_BAD_ARG;
+    return b->eng.set_tile(k, sid, lanes);
+}
+
+extern "C" int ppes_set_tile(ppes_solver *s, int kernel, int lanes)
+{
+    const int k = timer_of(kernel);
+    if (!s || k < 0 || k > KT_LIMIT) return PPES_BAD_ARG;
+    return s->eng.set_tile(k, s->eng.host(s->id).sid, lanes);
 }
 
 extern "C" int ppes_autotune(ppes_solver *s, int reps, int *tb_out)

@@ -130,7 +130,7 @@ struct Inst {
     double *uavg, *sigx, *sigy, *ldf, *part, *part2, *entpart, *ghost0;
     double *cent;                    // per slot element: cell entropy before LDF (NaN: a node inadmissible), entropy of the mean
     unsigned long long *dtslot;      // DT_SLOTS partial minima of the residual's dt_lim, DT_SLOT_STRIDE apart
-    long long nblk_el, nblk_slot;    // workgroups of the element / slot grids
+    long long nchk_el, nchk_slot;    // reduction chunks of the element / slot grids (chunk_elems(NN) elements each)
     Ctl ctl;
 };
 
@@ -158,13 +158,30 @@ struct Blk {
 constexpr int resid_tb(int nn) { return (nn > 16 && PPES_RESID_TB < 128) ? 128 : PPES_RESID_TB; }
 
 
+// Reduction chunks and tiles of the stage and limiter kernels. The per-chunk
+// partial sums (OEDG means, cell entropy) are summed over a fixed run of
+// chunk_elems(nn) consecutive elements in element order, whatever the
+// workgroup size; a workgroup (a "tile") holds m whole chunks, m = 1, 2, 4
+// or 8, in tile_lanes(nn, m) threads (the chunk's lanes rounded up to whole
+// wavefronts). So the autotuner may pick m per degree and the results stay
+// bit for bit the same. A chunk is the elements of one wavefront (two for
+// elements of more than 16 nodes, which would leave much of one wavefront
+// idle: 25-node elements fill 125 of 128 lanes, 50 of 64).
+constexpr int chunk_elems(int nn) { return nn <= 16 ? 64 / nn : 128 / nn; }
+constexpr int tile_lanes(int nn, int m) { return (m * chunk_elems(nn) * nn + 63) / 64 * 64; }
+// the multiplier of a 256-lane tile (the default of every kernel), and the
+// largest tile the kernels are built for
+constexpr int tile_mult_default(int nn) { return nn <= 16 ? 4 : 2; }
+constexpr int TILE_LANES_MAX = 512;
+
 // Element shape of degrees PX in x and PY in y (PY = PX, or PY = 0: the
 // reference's 1D set-up with one node in y). Node n = k + NX l.
 template <int PX, int PY>
 struct Shape {
     static constexpr int NX = PX + 1, NY = PY + 1;
     static constexpr int NN = NX * NY;
-    static constexpr int EPB = BLOCK / NN;                 // elements per workgroup
+    static constexpr int EPB = BLOCK / NN;                 // elements per 256-lane workgroup
+    static constexpr int CE = chunk_elems(NN);             // elements per reduction chunk
     static constexpr int HX = NX * (NX - 1) / 2;           // pairs per x-line
     static constexpr int HY = NY * (NY - 1) / 2;           // pairs per y-line
     static constexpr int XP = NY * HX;                     // x-pairs per element

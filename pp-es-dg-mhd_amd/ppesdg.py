@@ -114,6 +114,10 @@ def lib():
             "ppes_batch_positivity": [vp, ctypes.c_int, dp, dp],
             "ppes_batch_set_timing": [vp, ctypes.c_int],
             "ppes_batch_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
+            "ppes_tiles": [vp, ctypes.c_int, ip],
+            "ppes_batch_tiles": [vp, ctypes.c_int, ip],
+            "ppes_set_tile": [vp, ctypes.c_int, ctypes.c_int],
+            "ppes_batch_set_tile": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -222,6 +226,7 @@ class Solver:
         if arith is not None:
             self.set_arith(arith)
         self.nx, self.ny, self.p, self.py, self.rk = nx, ny, p, cfg.py, rk
+        self.shape = f"P{p}x{cfg.py}"
         self.N, self.NX, self.NY = p + 1, p + 1, cfg.py + 1
         NX, NY = self.NX, self.NY
         self.shape_u = (8, NX, NY, nx + 2, ny + 2)
@@ -348,6 +353,14 @@ class Solver:
         """OEDG face workgroup size per tuned shape, {"P<px>x<py>": lanes}."""
         return _tuned(lambda out: lib().ppes_face_tb(self.h, out), "ppes_face_tb")
 
+    def tiles(self, kernel):
+        """lanes per workgroup the step launches kernel class `kernel`
+        ("residual", "stage", "oedg", "limit") at, for this solver's degree"""
+        return _tuned(lambda out: lib().ppes_tiles(self.h, KERNELS[kernel], out), "ppes_tiles")[self.shape]
+
+    def set_tile(self, kernel, lanes):
+        _check(lib().ppes_set_tile(self.h, KERNELS[kernel], int(lanes)), "ppes_set_tile")
+
     def set_flux(self, kind):
         """face flux: espp_hll (default), pp_hll, espp_lf or ec_lf (ESPP_HLL_flux_x,
         PP_HLL_flux_x, ESPP_LF_flux_x, EC_LF_flux_x of DG_solver.f90:894-1497)"""
@@ -457,6 +470,14 @@ class Batch:
     def face_tb(self):
         """OEDG face workgroup size per tuned shape, {"P<px>x<py>": lanes}."""
         return _tuned(lambda out: lib().ppes_batch_face_tb(self.h, out), "ppes_batch_face_tb")
+
+    def tiles(self, kernel):
+        """{"P<px>x<py>": lanes per workgroup} at which the step launches
+        kernel class `kernel` ("residual", "stage", "oedg", "limit")"""
+        return _tuned(lambda out: lib().ppes_batch_tiles(self.h, KERNELS[kernel], out), "ppes_batch_tiles")
+
+    def set_tile(self, kernel, px, py, lanes):
+        _check(lib().ppes_batch_set_tile(self.h, KERNELS[kernel], int(px), int(py), int(lanes)), "ppes_batch_set_tile")
 
     def set_flux(self, i, kind):
         """face flux of instance i (see Solver.set_flux)"""
