@@ -10,8 +10,9 @@
  *   residual  (8, N, N, nx, ny)           ut_con(...,s)
  *   averages  (8, 0:nx+1, 0:ny+1)         u_avg
  * with N = px + 1 in x and py + 1 in y (the second N of each layout is
- * py + 1). Degrees: px = py in 1..4, or py = 0 with px in 1..4 (the
- * reference's 1D set-up, main.f90:60-81). On the device the state is kept
+ * py + 1). Degrees: px = py in 1..6, or py = 0 with px in 1..6 (the
+ * reference's 1D set-up, main.f90:60-81); 6 is the largest degree of the
+ * reference's tables (gauss.f90:296). On the device the state is kept
  * as structure-of-arrays planes; the library converts at the boundary.
  *
  * State slots: 0 = u_con (the committed solution), 1..rk_stages = uRK_con(s).
@@ -34,6 +35,11 @@ extern "C" {
 #define PPES_HIP_ERROR 5
 #define PPES_OOM 6
 #define PPES_UNSUPPORTED 7
+
+/* shape ids (entries of the per-shape arrays below): 1-4 = P1-P4 x same
+ * degree in y, 5-8 = P1-P4 x P0, 9-10 = P5-P6 x same degree, 11-12 =
+ * P5-P6 x P0 */
+#define PPES_NSHAPES 12
 
 /* boundary kinds, applied on the device */
 #define PPES_BC_PERIODIC 0  /* BC_periodic      problems.f90:312-348 */
@@ -69,7 +75,7 @@ extern "C" {
 
 typedef struct ppes_config {
     int nx, ny;            /* elements */
-    int px, py;            /* degrees: px = py in 1..4, or py = 0 with px in 1..4 */
+    int px, py;            /* degrees: px = py in 1..6, or py = 0 with px in 1..6 */
     int rk_stages;         /* 1, 2, 3 or 5 (SSP-RK(1,1),(2,2),(3,3),(5,4)) */
     double left, right, bottom, top;
     double gamma;
@@ -246,7 +252,10 @@ int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long lon
  * 64, 128 and 256 lanes (64 / N^2 to 256 / N^2 elements each, N^2 nodes per
  * element) over `reps` launches and keeps the fastest size for that degree
  * in this batch / solver; tb_out[sid - 1] receives it per shape id (1-4:
- * P1-P4 x same degree in y, 5-8: P1-P4 x P0), 0 where nothing was tuned.
+ * P1-P4 x same degree in y, 5-8: P1-P4 x P0, 9-10: P5-P6 x same degree,
+ * 11-12: P5-P6 x P0; PPES_NSHAPES entries), 0 where nothing was tuned. The
+ * elements of degrees 5 and 6 (36 and 49 nodes) are not tuned: their
+ * kernels are built at the default sizes only.
  * Recomputing L(u^n) into its own slot leaves every state as it was, and
  * results do not depend on the size. Call between steps (for a solver: after
  * ppes_solve_begin). */
@@ -275,9 +284,9 @@ int ppes_tiles(ppes_solver *s, int kernel, int *lanes_out);
 /* Sets that size for degree pair (px, py) (a solver: its own pair) by hand:
  * 64, 128 or 256 for the residual (at least the element's node count) and
  * the OEDG face sums; one of the tile sizes above for the stage and limiter
- * kernels (P1, P3, P1-P3 x P0: 64, 128, 256, 512; P2: 64, 128, 256, 512;
- * P4: 128, 256, 512). PPES_BAD_ARG for a size that is not built. Results do
- * not depend on it. */
+ * kernels (P1-P3 and P1-P4 x P0: 64, 128, 256, 512; P4: 128, 256, 512); for
+ * degrees 5 and 6 only the defaults (residual 128, the others 256).
+ * PPES_BAD_ARG for a size that is not built. Results do not depend on it. */
 int ppes_batch_set_tile(ppes_batch *b, int kernel, int px, int py, int lanes);
 int ppes_set_tile(ppes_solver *s, int kernel, int lanes);
 

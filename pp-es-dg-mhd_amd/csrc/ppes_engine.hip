@@ -30,22 +30,43 @@ namespace ppes {
             ENG_TRY(hipGetLastError());                                                        \
         }                                                                                      \
     } while (0)
-// tb: the workgroup size the group's residual map was built for; the fast
-// ES-HLL residual exists for every autotuner candidate, every other variant
-// for the default size only (build_group keeps to that)
-#define RESIDUAL_DISPATCH_F(sid, FK, tb, grid, ...)                                                              \
-    do {                                                                                                         \
-        if (arith == PPES_ARITH_FAST && (FK) == FLUX_ESPP_HLL && (tb) == 64) {                                   \
-            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, 64, FLUX_ESPP_HLL>), grid, 64, __VA_ARGS__));   \
-        } else if (arith == PPES_ARITH_FAST && (FK) == FLUX_ESPP_HLL && (tb) == 128) {                           \
-            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, 128, FLUX_ESPP_HLL>), grid, 128, __VA_ARGS__)); \
-        } else if (arith == PPES_ARITH_FAST && (FK) == FLUX_ESPP_HLL && (tb) == 256) {                           \
-            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, 256, FLUX_ESPP_HLL>), grid, 256, __VA_ARGS__)); \
-        } else if (arith == PPES_ARITH_FAST) {                                                                   \
-            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, true, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); \
-        } else {                                                                                                 \
-            ENG_DISPATCH(sid, LAUNCH_TB((k_residual<PX, PY, false, resid_tb(NN_), FK>), grid, resid_tb(NN_), __VA_ARGS__)); \
-        }                                                                                                        \
+// The residual of flux kind FK over a block map built for workgroups of tb
+// lanes: the fast ES-HLL residual exists for every autotuner candidate (64,
+// 128, 256 lanes) of the shapes that are tuned, every other variant for the
+// default size only (build_group keeps to that).
+template <int PX, int PY, int FK>
+static int launch_residual(hipStream_t stream, int arith, int tb, long long grid, Inst *d_recs, const Blk *map,
+                           int u_slot, int ut_slot, int mode)
+{
+    constexpr int NN = (PX + 1) * (PY + 1), TBD = resid_tb(NN);
+    if (arith == PPES_ARITH_FAST) {
+        if constexpr (tiles_tunable(NN) && FK == FLUX_ESPP_HLL) {
+            if (tb == 64) {
+                LAUNCH_TB((k_residual<PX, PY, true, 64, FK>), grid, 64, d_recs, map, u_slot, ut_slot, mode);
+                return PPES_OK;
+            }
+            if (tb == 128) {
+                LAUNCH_TB((k_residual<PX, PY, true, 128, FK>), grid, 128, d_recs, map, u_slot, ut_slot, mode);
+                return PPES_OK;
+            }
+            if (tb == 256) {
+                LAUNCH_TB((k_residual<PX, PY, true, 256, FK>), grid, 256, d_recs, map, u_slot, ut_slot, mode);
+                return PPES_OK;
+            }
+        }
+        if (tb != TBD) return PPES_BAD_ARG;
+        LAUNCH_TB((k_residual<PX, PY, true, TBD, FK>), grid, TBD, d_recs, map, u_slot, ut_slot, mode);
+        return PPES_OK;
+    }
+    if (tb != TBD) return PPES_BAD_ARG;
+    LAUNCH_TB((k_residual<PX, PY, false, TBD, FK>), grid, TBD, d_recs, map, u_slot, ut_slot, mode);
+    return PPES_OK;
+}
+#define RESIDUAL_DISPATCH_F(sid, FK, tb, grid, ...)                                                            \
+    do {                                                                                                       \
+        int rc_ = PPES_OK;                                                                                     \
+        ENG_DISPATCH(sid, (rc_ = launch_residual<PX, PY, FK>(stream, arith, (tb), (grid), __VA_ARGS__)));       \
+        if (rc_) return rc_;                                                                                   \
     } while (0)
 // the residual over a group: one launch per face flux kind present
 #define RESIDUAL_DISPATCH(g, ...)                                                                                \
@@ -64,18 +85,35 @@ namespace ppes {
     } while (0)
 
 // the OEDG face sums over a group (or sub-plan) at its workgroup size, in the
-// fast form in the fast arithmetic mode (the exact form always at BLOCK)
-#define FACES_DISPATCH(g, ...)                                                                                    \
-    do {                                                                                                          \
-        if (arith != PPES_ARITH_FAST) {                                                                           \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, BLOCK, false>), (g).n_face, BLOCK, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
-        } else if ((g).face_tb == 64) {                                                                           \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 64, true>), (g).n_face, 64, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
-        } else if ((g).face_tb == 128) {                                                                          \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 128, true>), (g).n_face, 128, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
-        } else {                                                                                                  \
-            ENG_DISPATCH((g).sid, LAUNCH_TB((k_oedg_faces<PX, PY, 256, true>), (g).n_face, 256, d_recs, (const Blk *)(g).d_face, __VA_ARGS__)); \
-        }                                                                                                         \
+// fast form in the fast arithmetic mode (the exact form always at BLOCK; the
+// shapes that are not tuned at BLOCK in both forms)
+template <int PX, int PY>
+static int launch_faces(hipStream_t stream, int arith, Inst *d_recs, const Group &g, int st, int flags)
+{
+    constexpr int NN = (PX + 1) * (PY + 1);
+    if (arith != PPES_ARITH_FAST) {
+        LAUNCH_TB((k_oedg_faces<PX, PY, BLOCK, false>), g.n_face, BLOCK, d_recs, (const Blk *)g.d_face, st, flags);
+        return PPES_OK;
+    }
+    if constexpr (tiles_tunable(NN)) {
+        if (g.face_tb == 64) {
+            LAUNCH_TB((k_oedg_faces<PX, PY, 64, true>), g.n_face, 64, d_recs, (const Blk *)g.d_face, st, flags);
+            return PPES_OK;
+        }
+        if (g.face_tb == 128) {
+            LAUNCH_TB((k_oedg_faces<PX, PY, 128, true>), g.n_face, 128, d_recs, (const Blk *)g.d_face, st, flags);
+            return PPES_OK;
+        }
+    }
+    if (g.face_tb != BLOCK) return PPES_BAD_ARG;
+    LAUNCH_TB((k_oedg_faces<PX, PY, BLOCK, true>), g.n_face, BLOCK, d_recs, (const Blk *)g.d_face, st, flags);
+    return PPES_OK;
+}
+#define FACES_DISPATCH(g, st, flags)                                                                          \
+    do {                                                                                                      \
+        int rc_ = PPES_OK;                                                                                    \
+        ENG_DISPATCH((g).sid, (rc_ = launch_faces<PX, PY>(stream, arith, d_recs, (g), (st), (int)(flags)))); \
+        if (rc_) return rc_;                                                                                  \
     } while (0)
 
 // The stage kernel over a group's slot map at the group's tile (stage_m
@@ -86,7 +124,7 @@ template <int PX, int PY, bool EO, int MV>
 static int launch_stage_k(hipStream_t stream, Inst *d_recs, const Group &g, int st, int flags)
 {
     constexpr int NN = (PX + 1) * (PY + 1);
-    if constexpr (tile_lanes(NN, MV) > TILE_LANES_MAX) {
+    if constexpr (tile_lanes(NN, MV) > TILE_LANES_MAX || (!tiles_tunable(NN) && MV != tile_mult_default(NN))) {
         return PPES_BAD_ARG;
     } else {
         LAUNCH_TB((k_stage_pp<PX, PY, EO, MV>), g.n_slot, tile_lanes(NN, MV), d_recs, (const Blk *)g.d_slot, (int)g.n_slot, st, flags);
@@ -110,7 +148,7 @@ template <int PX, int PY, bool EO, int MV>
 static int launch_limit_k(hipStream_t stream, Inst *d_recs, const Group &g, int st, int flags)
 {
     constexpr int NN = (PX + 1) * (PY + 1);
-    if constexpr (tile_lanes(NN, MV) > TILE_LANES_MAX) {
+    if constexpr (tile_lanes(NN, MV) > TILE_LANES_MAX || (!tiles_tunable(NN) && MV != tile_mult_default(NN))) {
         return PPES_BAD_ARG;
     } else {
         LAUNCH_TB((k_oedg_limit<PX, PY, EO, MV>), g.n_el, tile_lanes(NN, MV), d_recs, (const Blk *)g.d_el, st, flags);
@@ -159,6 +197,10 @@ static int launch_limit(hipStream_t stream, Inst *d_recs, const Group &g, bool e
         ENG_CASE_(6) CALL; } break;                  \
         ENG_CASE_(7) CALL; } break;                  \
         ENG_CASE_(8) CALL; } break;                  \
+        ENG_CASE_(9) CALL; } break;                  \
+        ENG_CASE_(10) CALL; } break;                 \
+        ENG_CASE_(11) CALL; } break;                 \
+        ENG_CASE_(12) CALL; } break;                 \
     default: return PPES_UNSUPPORTED;                \
     }
 
@@ -232,7 +274,7 @@ Engine::~Engine()
     for (auto &ge : gevents) (void)hipEventDestroy(ge);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     for (MapPool *ps : {plan_pools, start_pools, early_pools})
-        for (int sid = 0; sid < 9; ++sid) {
+        for (int sid = 0; sid <= NSHAPES; ++sid) {
             for (Blk *p : ps[sid].d) if (p) (void)hipFree(p);
             if (ps[sid].dl) (void)hipFree(ps[sid].dl);
         }
@@ -410,7 +452,7 @@ static void record_base(Inst &r, const ppes_config &cfg, int S)
 int Engine::add(const ppes_config &cfg, int *id)
 {
     if (cfg.nx < 1 || cfg.ny < 1 || cfg.px < 0 || cfg.py < 0) return PPES_BAD_ARG;
-    // degrees: px = py in 1..4, or py = 0 (the reference's 1D set-up) with px in 1..4
+    // degrees: px = py in 1..6, or py = 0 (the reference's 1D set-up) with px in 1..6
     if (shape_id(cfg.px, cfg.py) < 0) return PPES_UNSUPPORTED;
     if (cfg.rk_stages != S) return PPES_BAD_ARG;
     if (!(cfg.right > cfg.left) || !(cfg.top > cfg.bottom) || !(cfg.gamma > 1.0)) return PPES_BAD_ARG;
@@ -777,7 +819,7 @@ int Engine::build_plan()
     early_groups.clear();
     std::vector<int> all;
     for (int s = 0; s <= MAXS; ++s) any_oedg_stage[s] = false;
-    for (int sid = 1; sid <= 8; ++sid) {
+    for (int sid = 1; sid <= NSHAPES; ++sid) {
         std::vector<int> mem;
         for (size_t i = 0; i < hosts.size(); ++i)
             if (hosts[i] && hosts[i]->sid == sid && h_recs[i].ctl.state == ST_ACTIVE) mem.push_back((int)i);
@@ -973,7 +1015,12 @@ int Engine::residual_tb(int sid, const std::vector<int> &members) const
     return tuned_tb[sid];
 }
 
-static bool tile_built(int sid, int m) { return (m == 1 || m == 2 || m == 4 || m == 8) && tile_lanes(nn_of(sid), m) <= TILE_LANES_MAX; }
+static bool tile_built(int sid, int m)
+{
+    const int nn = nn_of(sid);
+    if (!tiles_tunable(nn)) return m == tile_mult_default(nn);
+    return (m == 1 || m == 2 || m == 4 || m == 8) && tile_lanes(nn, m) <= TILE_LANES_MAX;
+}
 
 int Engine::stage_mult(int sid, bool eo) const
 {
@@ -987,7 +1034,7 @@ int Engine::limit_mult(int sid, bool eo) const
 
 int Engine::tile_of(int kernel, int sid) const
 {
-    if (sid < 1 || sid > 8) return 0;
+    if (sid < 1 || sid > NSHAPES) return 0;
     switch (kernel) {
     case KT_RESIDUAL: return arith == PPES_ARITH_FAST && tuned_tb[sid] ? tuned_tb[sid] : resid_tb(nn_of(sid));
     case KT_STAGE: return tile_lanes(nn_of(sid), stage_mult(sid, false));
@@ -999,10 +1046,11 @@ int Engine::tile_of(int kernel, int sid) const
 
 int Engine::set_tile(int kernel, int sid, int lanes)
 {
-    if (sid < 1 || sid > 8) return PPES_BAD_ARG;
+    if (sid < 1 || sid > NSHAPES) return PPES_BAD_ARG;
     const int nn = nn_of(sid);
     if (kernel == KT_RESIDUAL || kernel == KT_OEDG) {
         if (lanes != 64 && lanes != 128 && lanes != 256) return PPES_BAD_ARG;
+        if (!tiles_tunable(nn) && lanes != (kernel == KT_RESIDUAL ? resid_tb(nn) : BLOCK)) return PPES_BAD_ARG;
         if (kernel == KT_RESIDUAL) {
             if (lanes < nn) return PPES_BAD_ARG;
             tuned_tb[sid] = lanes;
@@ -1027,7 +1075,7 @@ int Engine::set_tile(int kernel, int sid, int lanes)
 int Engine::autotune(int reps, int *tb_out)
 {
     if (tb_out)
-        for (int s2 = 0; s2 < 8; ++s2) tb_out[s2] = 0;
+        for (int s2 = 0; s2 < NSHAPES; ++s2) tb_out[s2] = 0;
     int rc = build_plan();
     if (rc) return rc;
     if (arith != PPES_ARITH_FAST || n_all == 0) return PPES_OK;
@@ -1050,6 +1098,7 @@ int Engine::autotune(int reps, int *tb_out)
         for (int id : g.members) fast_hll = fast_hll && hosts[id]->flux == FLUX_ESPP_HLL;
         if (!fast_hll) continue;
         const int NN = (sid_px(g.sid) + 1) * (sid_py(g.sid) + 1);
+        if (!tiles_tunable(NN)) continue;  // built at the default size only
         int best = 0;
         float best_ms = 0.0f;
         for (int tb : {64, 128, 256}) {
@@ -1093,6 +1142,7 @@ int Engine::autotune(int reps, int *tb_out)
     // 64, 128 and 256 threads per workgroup
     for (const Group &g : groups) {
         if (err) break;
+        if (!tiles_tunable(nn_of(g.sid))) continue;
         int best = 0;
         float best_ms = 0.0f;
         for (int tb : {64, 128, 256}) {
@@ -1136,7 +1186,7 @@ int Engine::autotune(int reps, int *tb_out)
     for (int kern = 0; kern < 2 && !err; ++kern) {
         for (const Group &g : groups) {
             if (err) break;
-            if (g.eo) continue;  // the ent_offset variants keep the default tile
+            if (g.eo || !tiles_tunable(nn_of(g.sid))) continue;  // the ent_offset variants and P5, P6 keep the default tile
             if (kern == 1 && !g.oedg_final && g.n_every == 0) continue;  // no limiter launches
             int best = 0;
             float best_ms = 0.0f;
@@ -1263,7 +1313,7 @@ int Engine::run_steps(long long nsteps, bool use_graph, long long *done_steps)
 // BC, saved ghost layer, residual of u^n (drive.f90:66-73)
 int Engine::launch_startup(const std::vector<int> &ids)
 {
-    std::vector<int> by_sid[9];
+    std::vector<int> by_sid[NSHAPES + 1];
     for (int id : ids) by_sid[hosts[id]->sid].push_back(id);
     int *d_ids = nullptr;
     ENG_TRY(hipMalloc((void **)&d_ids, std::max<size_t>(ids.size(), 1) * sizeof(int)));
@@ -1271,7 +1321,7 @@ int Engine::launch_startup(const std::vector<int> &ids)
     const int n = (int)ids.size();
     int rc = PPES_OK;
     std::vector<Group> tmp;
-    for (int sid = 1; sid <= 8 && rc == PPES_OK; ++sid) {
+    for (int sid = 1; sid <= NSHAPES && rc == PPES_OK; ++sid) {
         if (by_sid[sid].empty()) continue;
         Group g;
         rc = build_group(g, by_sid[sid], sid, &start_pools[sid], false);

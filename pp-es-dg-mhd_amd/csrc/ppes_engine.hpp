@@ -117,14 +117,14 @@ public:
     // slot 1, the limiter into slot S, which the commit left holding the old
     // u^n) and the instance records are put back afterwards.
     int autotune(int reps, int *tb_out);
-    int tuned_tb[9] = {};  // by shape id; 0: the default size
+    int tuned_tb[NSHAPES + 1] = {};  // by shape id; 0: the default size
     int residual_tb(int sid, const std::vector<int> &members) const;
-    int tuned_face_tb[9] = {};  // OEDG face workgroup size by shape id; 0: BLOCK
+    int tuned_face_tb[NSHAPES + 1] = {};  // OEDG face workgroup size by shape id; 0: BLOCK
     int face_tb(int sid) const { return arith == PPES_ARITH_FAST && tuned_face_tb[sid] ? tuned_face_tb[sid] : BLOCK; }
     // chunks per workgroup of the stage and limiter kernels by shape id (0:
     // the 256-lane default); the ent_offset variants and the one-instance
     // maps of the reference-API calls keep the default
-    int tuned_stage_m[9] = {}, tuned_limit_m[9] = {};
+    int tuned_stage_m[NSHAPES + 1] = {}, tuned_limit_m[NSHAPES + 1] = {};
     int stage_mult(int sid, bool eo) const;
     int limit_mult(int sid, bool eo) const;
     // the workgroup size (lanes) a kernel class (KT_RESIDUAL, KT_STAGE,
@@ -155,8 +155,8 @@ private:
     int build_plan();
     // tuned = false: the default tiles of the stage and limiter kernels
     int build_group(Group &g, const std::vector<int> &members, int sid, MapPool *pool = nullptr, bool tuned = true);
-    MapPool plan_pools[9], start_pools[9];  // by shape id: the step plan's maps, the start-up's
-    MapPool early_pools[9];                 // by shape id: the every-stage OEDG sub-plans' maps
+    MapPool plan_pools[NSHAPES + 1], start_pools[NSHAPES + 1];  // by shape id: the step plan's maps, the start-up's
+    MapPool early_pools[NSHAPES + 1];                 // by shape id: the every-stage OEDG sub-plans' maps
     std::vector<Group> early_groups;        // every-stage OEDG members of a group, when not all of them
     // the maps that cover exactly the instances of group g that may apply
     // OEDG on stage st (the group itself or its sub-plan), or none

@@ -1016,9 +1016,12 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
     // stored each entry before loading the next would wait on memory
     // M^2 / TB times in a row)
     const bool stage_ldf = use_ldf && !(flags & F_COMBINE_ONLY);
+    // (the operators of degrees 5 and 6, 72^2 and 98^2 entries, would take
+    // 42-76 registers this way: they are copied after the state's loads)
     constexpr int R = (M * M + TB - 1) / TB;
-    double ax[R];
-    if (stage_ldf) {
+    constexpr bool PREF = R <= 12;
+    double ax[PREF ? R : 1];
+    if (PREF && stage_ldf) {
         const gdbl *ldf = gin(I.ldf);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -1060,10 +1063,15 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
     }
     if (flags & F_COMBINE_ONLY) return;
     if (stage_ldf) {
+        if constexpr (PREF) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int a = t + r * TB;
-            if (a < M * M) sldf[a] = ax[r];
+            for (int r = 0; r < R; ++r) {
+                const int a = t + r * TB;
+                if (a < M * M) sldf[a] = ax[r];
+            }
+        } else {
+            const gdbl *ldf = gin(I.ldf);
+            for (int a = t; a < M * M; a += TB) sldf[a] = ldf[a];
         }
     }
     __syncthreads();
@@ -1360,6 +1368,82 @@ __device__ __forceinline__ double oedg_face_sum(const double *A, const double *B
     return total;
 }
 
+// The same sums for degrees 5 and 6, in the reference's own order
+// (DG_solver.f90:314-338): each element's normal derivatives by repeated
+// products with the scaled matrix (2/h) D of the normal direction (dn,
+// column-major), one transverse column of both elements at a time, the
+// jump of order o taken between the derivative columns' end rows. The row
+// form above rounds D^o once, which the reference does not; for these
+// degrees the jumps of the high derivatives cancel by many orders of
+// magnitude, so the two forms differ visibly, and this one matches the
+// reference. Columns are read from memory as needed (both whole elements
+// would take 4 NN registers).
+template <int PN, int PT, int MXP, bool XF, int NX>
+__device__ __forceinline__ double oedg_face_sum_rep(const gdbl *uA, const gdbl *uB, const double *dn, const double *d2t,
+                                                    const double (*coef)[MAXN], const double *qwt)
+{
+    constexpr int NNO = PN + 1, NT = PT + 1;
+    double J[NNO][NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        double ca[NNO], cb[NNO], tmp[NNO];
+#pragma unroll
+        for (int a = 0; a < NNO; ++a) {
+            const int m = XF ? a + NX * t : t + NX * a;
+            ca[a] = uA[m];
+            cb[a] = uB[m];
+        }
+        J[0][t] = cb[0] - ca[NNO - 1];
+#pragma unroll
+        for (int o = 1; o < NNO; ++o) {
+#pragma unroll
+            for (int a = 0; a < NNO; ++a) {
+                double acc = 0.0;
+#pragma unroll
+                for (int b = 0; b < NNO; ++b) acc += dn[a + NNO * b] * ca[b];
+                tmp[a] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < NNO; ++a) ca[a] = tmp[a];
+#pragma unroll
+            for (int a = 0; a < NNO; ++a) {
+                double acc = 0.0;
+#pragma unroll
+                for (int b = 0; b < NNO; ++b) acc += dn[a + NNO * b] * cb[b];
+                tmp[a] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < NNO; ++a) cb[a] = tmp[a];
+            J[o][t] = cb[0] - ca[NNO - 1];
+        }
+    }
+    double total = 0.0;
+#pragma unroll
+    for (int o = 0; o <= PN; ++o) {
+        double *jump = J[o];
+        double jt[NT];
+#pragma unroll
+        for (int q = 0; q <= MXP - o; ++q) {
+            double jl1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) jl1 += (0.5 * qwt[t]) * fabs(jump[t]);
+            total = total + coef[o][q] * jl1;
+            if (q < PT) {
+#pragma unroll
+                for (int a = 0; a < NT; ++a) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) acc += d2t[a + NT * b] * jump[b];
+                    jt[a] = acc;
+                }
+#pragma unroll
+                for (int a = 0; a < NT; ++a) jump[a] = jt[a];
+            }
+        }
+    }
+    return total;
+}
+
 // The same sums in the fast arithmetic mode: every a * b + c a fused
 // multiply-add, the order's scale (2/h)^o folded into the (o, q) weights
 // (|s x| = s |x| for s > 0), the last transverse derivative of each order,
@@ -1370,10 +1454,12 @@ template <int PN, int PT, int MXP, bool XF, int NX>
 __device__ __forceinline__ double oedg_face_sum_fast(const double *A, const double *B, const double *tab, double h)
 {
     constexpr int NNO = PN + 1, NT = PT + 1;
-    // tab: rows of D^o (side 0 then side 1), (2/h_t) D_t, the (o, q) weights,
-    // half the transverse quadrature weights
-    const double *r0 = tab, *r1 = tab + MAXN * MAXN, *d2t = tab + 2 * MAXN * MAXN;
-    const double *coef = d2t + MAXN * MAXN, *hq = coef + MAXN * MAXN;
+    // tab, rows of TS = MXP + 1: rows of D^o (side 0 then side 1), (2/h_t) D_t
+    // (column-major, NT rows), the (o, q) weights, half the transverse
+    // quadrature weights
+    constexpr int TS = MXP + 1;
+    const double *r0 = tab, *r1 = tab + TS * TS, *d2t = tab + 2 * TS * TS;
+    const double *coef = d2t + TS * TS, *hq = coef + TS * TS;
     double total = 0.0, scale = 1.0;
 #pragma unroll
     for (int o = 0; o <= PN; ++o) {
@@ -1382,7 +1468,7 @@ __device__ __forceinline__ double oedg_face_sum_fast(const double *A, const doub
         for (int t = 0; t < NT; ++t) jump[t] = 0.0;
 #pragma unroll
         for (int a = 0; a < NNO; ++a) {
-            const double c1 = r1[o * MAXN + a], c0 = r0[o * MAXN + a];
+            const double c1 = r1[o * TS + a], c0 = r0[o * TS + a];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const int m = XF ? a + NX * t : t + NX * a;
@@ -1394,7 +1480,7 @@ __device__ __forceinline__ double oedg_face_sum_fast(const double *A, const doub
             double jl1 = 0.0;
 #pragma unroll
             for (int t = 0; t < NT; ++t) jl1 = __builtin_fma(hq[t], fabs(jump[t]), jl1);
-            total = __builtin_fma(coef[o * MAXN + q] * scale, jl1, total);
+            total = __builtin_fma(coef[o * TS + q] * scale, jl1, total);
             if (q < PT && q < MXP - o) {
 #pragma unroll
                 for (int a = 0; a < NT; ++a) {
@@ -1421,7 +1507,7 @@ template <int PX, int PY, int TBF = BLOCK, bool FAST = false>
 __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map, int st, int flags)
 {
     constexpr int NX = PX + 1, NY = PY + 1, NN = NX * NY, MXP = PX > PY ? PX : PY;
-    constexpr int TAB = 4 * MAXN * MAXN + MAXN;
+    constexpr int TS = MXP + 1, T2 = TS * TS, TAB = 4 * T2 + TS;
     __shared__ double stab[FAST ? TAB : 1];
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -1449,6 +1535,14 @@ __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map,
     }
     const gdbl *uA = u + v * I.vs + bA;
     const gdbl *uB = u + v * I.vs + bB;
+    if constexpr (MXP >= 5) {
+        // degrees 5 and 6: the reference's repeated differentiation, in both
+        // arithmetic modes
+        if (!act) return;
+        if (xface) gout(I.sigx)[f * 8 + v] = oedg_face_sum_rep<PX, PY, MXP, true, NX>(uA, uB, I.oedg_d2[1], I.oedg_d2[0], I.oedg_coef[0], I.qw_y);
+        else gout(I.sigy)[f * 8 + v] = oedg_face_sum_rep<PY, PX, MXP, false, NX>(uA, uB, I.oedg_d2[0], I.oedg_d2[1], I.oedg_coef[1], I.qw_x);
+        return;
+    }
     double A[NN], B[NN];
 #pragma unroll
     for (int m = 0; m < NN; ++m) { A[m] = uA[m]; B[m] = uB[m]; }
@@ -1457,15 +1551,17 @@ __global__ void __launch_bounds__(TBF) k_oedg_faces(Inst *insts, const Blk *map,
         // workgroup), copied once per workgroup while the element values
         // are in flight: rows of D^o (side 0, then side 1), (2/h_t) D_t, the
         // (o, q) weights, half the transverse quadrature weights
+        // (rows of TS = MXP + 1 entries: the record's rows of MAXN are cut down)
         const int d = xface ? 0 : 1;
         const double *r = &(xface ? I.drow_x : I.drow_y)[0][0][0];
         const double *qw = xface ? I.qw_y : I.qw_x;
         for (int a = threadIdx.x; a < TAB; a += TBF) {
             double x;
-            if (a < 2 * MAXN * MAXN) x = r[a];
-            else if (a < 3 * MAXN * MAXN) x = I.oedg_d2[d][a - 2 * MAXN * MAXN];
-            else if (a < 4 * MAXN * MAXN) x = (&I.oedg_coef[d][0][0])[a - 3 * MAXN * MAXN];
-            else x = 0.5 * qw[a - 4 * MAXN * MAXN];
+            const int blk = a / T2, rem = a - blk * T2, row = rem / TS, col = rem - row * TS;
+            if (blk < 2) x = r[(blk * MAXN + row) * MAXN + col];
+            else if (blk == 2) x = I.oedg_d2[d][rem];
+            else if (blk == 3) x = I.oedg_coef[d][row][col];
+            else x = 0.5 * qw[rem];
             stab[a] = x;
         }
         __syncthreads();

@@ -40,6 +40,10 @@ using namespace ppes;
         LIB_CASE_(6) CALL; } break; \
         LIB_CASE_(7) CALL; } break; \
         LIB_CASE_(8) CALL; } break; \
+        LIB_CASE_(9) CALL; } break; \
+        LIB_CASE_(10) CALL; } break; \
+        LIB_CASE_(11) CALL; } break; \
+        LIB_CASE_(12) CALL; } break; \
     default: return PPES_UNSUPPORTED; \
     }
 
@@ -953,7 +957,7 @@ extern "C" int ppes_batch_autotune(ppes_batch *b, int reps, int *tb_out)
 
 static void face_sizes(const Engine &e, int *tb_out)
 {
-    for (int sid = 1; sid <= 8; ++sid) tb_out[sid - 1] = e.arith == PPES_ARITH_FAST ? e.tuned_face_tb[sid] : 0;
+    for (int sid = 1; sid <= NSHAPES; ++sid) tb_out[sid - 1] = e.arith == PPES_ARITH_FAST ? e.tuned_face_tb[sid] : 0;
 }
 
 extern "C" int ppes_batch_face_tb(ppes_batch *b, int *tb_out)
@@ -974,7 +978,7 @@ static int engine_tiles(const Engine &e, int kernel, int *lanes_out)
 {
     const int k = timer_of(kernel);
     if (!lanes_out || k < 0 || k > KT_LIMIT) return PPES_BAD_ARG;
-    for (int sid = 1; sid <= 8; ++sid) lanes_out[sid - 1] = e.tile_of(k, sid);
+    for (int sid = 1; sid <= NSHAPES; ++sid) lanes_out[sid - 1] = e.tile_of(k, sid);
     return PPES_OK;
 }
 

@@ -46,7 +46,7 @@ constexpr unsigned long long DT_NONE = 0x7FF0000000000000ULL;  // +inf
 typedef __attribute__((address_space(1))) double gdbl;
 __device__ __forceinline__ const gdbl *gin(const double *p) { return (const gdbl *)p; }
 __device__ __forceinline__ gdbl *gout(double *p) { return (gdbl *)p; }
-constexpr int MAXN = 5;  // p <= 4 on the device
+constexpr int MAXN = 7;  // p <= 6 on the device (the reference's tables, gauss.f90:296)
 constexpr int MAXS = 5;  // RK stages
 
 enum InstState { ST_IDLE = 0, ST_ACTIVE = 1, ST_DONE = 2, ST_FAILED = 3 };
@@ -194,14 +194,22 @@ struct Shape {
     static constexpr int BUF_ROUND = (NN * REC > FACE * FREC) ? NN * REC : FACE * FREC;  // one pair round
 };
 
-// Shape ids of the device code: 1..4 = (p, p); 5..8 = (p, 0).
-constexpr int sid_px(int sid) { return sid <= 4 ? sid : sid - 4; }
-constexpr int sid_py(int sid) { return sid <= 4 ? sid : 0; }
+// Shape ids of the device code: 1..4 = (p, p) and 5..8 = (p, 0) for
+// p = 1..4; 9, 10 = (5, 5), (6, 6); 11, 12 = (5, 0), (6, 0).
+constexpr int NSHAPES = 12;
+constexpr int sid_px(int sid) { return sid <= 4 ? sid : (sid <= 10 ? sid - 4 : sid - 6); }
+constexpr int sid_py(int sid) { return sid <= 4 ? sid : (sid <= 8 ? 0 : (sid <= 10 ? sid - 4 : 0)); }
 inline int shape_id(int px, int py)
 {
-    if (px < 1 || px > 4) return -1;
-    return py == px ? px : (py == 0 ? 4 + px : -1);
+    if (px < 1 || px > 6) return -1;
+    if (py == px) return px <= 4 ? px : px + 4;
+    if (py == 0) return px <= 4 ? px + 4 : px + 6;
+    return -1;
 }
+// the shapes whose kernels are built in every workgroup size the autotuner
+// tries (up to 25 nodes); the larger elements of degrees 5 and 6 run at the
+// default sizes only
+constexpr bool tiles_tunable(int nn) { return nn <= 25; }
 
 __device__ __forceinline__ long long gidx(const Inst &I, int ex, int ey) { return (long long)ey * (I.nx + 2) + ex; }
 

@@ -203,14 +203,22 @@ def _stats(st):
     return {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "pad"}
 
 
+NSHAPES = 12  # shape ids 1-4: P1-P4 x same degree, 5-8: P1-P4 x P0, 9-10: P5-P6 x same, 11-12: P5-P6 x P0
+
+
+def shape_of(sid):
+    px = sid if sid <= 4 else (sid - 4 if sid <= 10 else sid - 6)
+    py = px if (sid <= 4 or 9 <= sid <= 10) else 0
+    return px, py
+
+
 def _tuned(call, name):
-    out = (ctypes.c_int * 8)()
+    out = (ctypes.c_int * NSHAPES)()
     _check(call(out), name)
     res = {}
-    for sid in range(1, 9):
+    for sid in range(1, NSHAPES + 1):
         if out[sid - 1]:
-            px = (sid - 1) % 4 + 1
-            py = px if sid <= 4 else 0
+            px, py = shape_of(sid)
             res[f"P{px}x{py}"] = int(out[sid - 1])
     return res
 

@@ -86,6 +86,13 @@ STAGE_CASES = {
     "stages_briowu_p3_py0_ldf0": dict(mode="stages", prob="briowu", nx=12, ny=1, p=3, py=0, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
     "stages_wave1d_p3_py0_ldf0": dict(mode="stages", prob="wave1d", nx=10, ny=1, p=3, py=0, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
     "stages_briowu_p2_py0": dict(mode="stages", prob="briowu", nx=16, ny=1, p=2, py=0, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
+    # degrees 5 and 6, the largest of the reference's tables (gauss.f90:296)
+    "stages_ot2pi_p5_ldf0": dict(mode="stages", prob="ot2pi", nx=5, ny=4, p=5, nsteps=2, oedg=1, ldf=0, cfl=0.2, dumpsteps=1),
+    "stages_rotor_p5": dict(mode="stages", prob="rotor", nx=5, ny=5, p=5, nsteps=2, oedg=1, ldf=1, cfl=0.2, dumpsteps=1),
+    "stages_blast_p6": dict(mode="stages", prob="blast", nx=4, ny=4, p=6, nsteps=2, oedg=2, ldf=1, cfl=0.15, dumpsteps=1),
+    "stages_blast_p6_ldf0": dict(mode="stages", prob="blast", nx=4, ny=3, p=6, nsteps=2, oedg=2, ldf=0, cfl=0.15, dumpsteps=1),
+    "stages_wave1d_p5_py0_ldf0": dict(mode="stages", prob="wave1d", nx=8, ny=1, p=5, py=0, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1),
+    "stages_briowu_p6_py0": dict(mode="stages", prob="briowu", nx=10, ny=1, p=6, py=0, nsteps=3, oedg=1, ldf=1, cfl=0.6, dumpsteps=1),
     # ent_offset on (convex limiting against the cell entropy before LDF, DG_solver.f90:195-225,605-633)
     "stages_blast_p2_entoff_ldf0": dict(mode="stages", prob="blast", nx=8, ny=7, p=2, nsteps=3, oedg=2, ldf=0, cfl=0.15, dumpsteps=1, entoff=1),
     "stages_ot2pi_p3_entoff_ldf0": dict(mode="stages", prob="ot2pi", nx=6, ny=6, p=3, nsteps=3, oedg=1, ldf=0, cfl=0.6, dumpsteps=1, entoff=1),
@@ -102,6 +109,8 @@ END_CASES = {
     # (also keeps the reference's Tecplot output of this run: solution_0.plt, entropy.plt)
     "end_ot2pi_p1_32": dict(mode="solve", prob="ot2pi", nx=32, ny=32, p=1, tfinal=0.1, oedg=1, ldf=1, cfl=0.2, plt=1),
     "end_ot2pi_p4_12": dict(mode="solve", prob="ot2pi", nx=12, ny=12, p=4, tfinal=0.1, oedg=1, ldf=1, cfl=0.2),
+    "end_ot2pi_p6_8": dict(mode="solve", prob="ot2pi", nx=8, ny=8, p=6, tfinal=0.05, oedg=1, ldf=1, cfl=0.2),
+    "end_rotor_p5_10": dict(mode="solve", prob="rotor", nx=10, ny=10, p=5, tfinal=0.005, oedg=1, ldf=1, cfl=0.2),
     # the other RK methods of solve_execute (drive.f90:132-174,213-272): RK1, RK2, RK5 (CFL_RK 1.508)
     "end_ot2pi_p2_16_rk1": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.1, rk=1),
     "end_ot2pi_p2_16_rk2": dict(mode="solve", prob="ot2pi", nx=16, ny=16, p=2, tfinal=0.05, oedg=1, ldf=1, cfl=0.2, rk=2),

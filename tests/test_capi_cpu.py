@@ -59,12 +59,13 @@ def test_ldf_nodal_operator_matches_modal_projector(p, hx, hy):
 
 
 def test_element_shapes_accepted():
-    """The device takes py = px in 1..4 and the reference's 1D set-ups
-    py = 0 (px in 1..4); other pairs are rejected before any allocation."""
-    for p in (1, 2, 3, 4):
+    """The device takes py = px in 1..6 (every degree of the reference's
+    tables) and the reference's 1D set-ups py = 0 (px in 1..6); other pairs
+    are rejected before any allocation."""
+    for p in (1, 2, 3, 4, 5, 6):
         assert ppesdg.instance_bytes(64, 64, p) > 0
         assert ppesdg.instance_bytes(480, 1, p, rk=5, py=0) > 0
-    for p, py in ((3, 1), (2, 3), (0, 0), (5, 5), (5, 0)):
+    for p, py in ((3, 1), (2, 3), (0, 0), (7, 7), (7, 0), (6, 5)):
         assert ppesdg.instance_bytes(16, 16, p, py=py) == -1
     # a P3 x P0 element holds a quarter of a P3 element's nodes
     a, b = ppesdg.instance_bytes(480, 1, 3, rk=5, py=0), ppesdg.instance_bytes(480, 1, 3, rk=5)

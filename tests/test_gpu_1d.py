@@ -40,7 +40,7 @@ def advanced(prob, nx, p, nsteps):
     return u
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("prob", ["briowu", "wave1d"])
 def test_init_matches_oracle(prob, p):
     o, g = make_pair(prob, 24, p)
@@ -50,7 +50,7 @@ def test_init_matches_oracle(prob, p):
 
 
 @pytest.mark.parametrize("arith", ["exact", "fast"])
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("prob", ["briowu", "wave1d"])
 def test_residual_matches_oracle(prob, p, arith):
     nx = 20
@@ -79,7 +79,7 @@ def test_residual_matches_oracle(prob, p, arith):
     assert abs(dtl - dtl_ref) <= 1e-14 * dtl_ref
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("oedg", [0, 1])
 def test_post_process_matches_oracle(p, oedg):
     nx = 20
@@ -96,7 +96,7 @@ def test_post_process_matches_oracle(p, oedg):
 
 
 @pytest.mark.parametrize("rk", [3, 5])
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("prob,ldf", [("briowu", True), ("wave1d", False)])
 def test_solve_matches_oracle(prob, ldf, p, rk):
     o, g = make_pair(prob, 40, p, rk=rk)

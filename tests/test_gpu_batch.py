@@ -7,7 +7,7 @@ import pytest
 
 import oracle as O
 import ppesdg
-from helpers import interior, normwise, setup
+from helpers import interior, normwise, rel_l1, setup
 
 pytestmark = pytest.mark.gpu
 
@@ -139,3 +139,39 @@ def test_ent_offset_instance_alone_equals_in_mix():
     mix.start(j, -1.0, 5, 0.6, 1, True, ent_offset=True)
     mix.step(0)
     assert np.array_equal(mix.download_state(j, 0), ref)
+
+
+def test_degree_5_and_6_instances_alone_equal_in_mix():
+    """P5 and P6 instances (36 and 49 nodes per element, the default
+    workgroup sizes only) give the same bits alone and in a batch with every
+    other degree, and match the oracle after the same steps."""
+    import oracle as O
+    s = setup("ot2pi")
+    refs = {}
+    for p in (5, 6):
+        alone = ppesdg.Batch(3)
+        i = alone.add(9, 8, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+        alone.init_problem(i, "ot2pi")
+        alone.start(i, -1.0, 4, 0.2, 1, True)
+        alone.step(0)
+        refs[p] = alone.download_state(i, 0)
+        alone.close()
+        o = O.Oracle(9, 8, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+        o.init_problem("ot2pi")
+        o.run(-1.0, 4, 0.2, 1, True)
+        # P6 with OEDG: the conditioning of the order-6 jumps (test_oedg_conditioning_cpu.py)
+        assert normwise(interior(refs[p]), interior(o.u)) <= (1e-10 if p == 5 else 1e-7)
+        assert rel_l1(interior(refs[p])[0], interior(o.u)[0]) <= 1e-10
+    mix = ppesdg.Batch(3)
+    ids = {}
+    for p in (1, 2, 3, 4, 5, 6):
+        k = mix.add(9, 8, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+        mix.init_problem(k, "ot2pi")
+        mix.start(k, -1.0, 4, 0.2, 1, True)
+        ids[p] = k
+    mix.step(1)
+    mix.autotune(2)
+    mix.step(0)
+    for p in (5, 6):
+        assert np.array_equal(mix.download_state(ids[p], 0), refs[p])
+    mix.close()

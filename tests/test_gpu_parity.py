@@ -17,6 +17,19 @@ from helpers import interior, normwise, perturbation_table, rel_l1, setup, strip
 pytestmark = pytest.mark.gpu
 
 PROBS = ["ot2pi", "rotor", "blast"]
+# every degree of the reference's tables (gauss.f90:296)
+DEGREES = [1, 2, 3, 4, 5, 6]
+
+
+def oedg_tol(p, base):
+    """At degree 6 the OEDG indicators' order-6 jumps cancel by about five
+    orders of magnitude, so a last-bit difference of the state (the device's
+    LDF product sums in another order) comes out ~1e5 larger in the damping
+    factor. The oracle itself, given a one-ulp change of B_x, moves by 1.6e-8
+    normwise over four P6 steps (1.9e-14 at P4): test_oedg_conditioning_cpu.py.
+    The tolerance for P6 with OEDG follows that measured conditioning; the
+    L1 density checks keep 1e-10."""
+    return base if p <= 5 else max(base, 1e-7)
 
 
 def make_pair(prob, nx, ny, p, rk=3):
@@ -38,7 +51,7 @@ def advanced_state(prob, nx, ny, p, nsteps, oedg_mode=1):
     return u
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("prob", PROBS)
 def test_init_problem_matches_oracle(prob, p):
     o, g = make_pair(prob, 7, 5, p)
@@ -56,7 +69,7 @@ def test_init_perturbed_ot_matches_oracle():
 
 
 @pytest.mark.parametrize("arith", ["exact", "fast"])
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("prob", PROBS)
 def test_residual_matches_oracle(prob, p, arith):
     nx, ny = 9, 7
@@ -70,11 +83,13 @@ def test_residual_matches_oracle(prob, p, arith):
     ut = g.download_residual(1)
     err = normwise(ut, ut_ref)
     print(f"residual {prob} P{p} {arith}: normwise {err:.3e}, dt_lim {dtl} vs {dtl_ref}")
-    assert err <= 1e-12
+    # the fast form's few-ulp flux differences enter the volume sums through
+    # D, whose entries grow like N^2: 1.1e-12 at P6 on the Orszag-Tang state
+    assert err <= (1e-12 if arith == "exact" or p <= 4 else 2e-12)
     assert abs(dtl - dtl_ref) <= 1e-14 * dtl_ref
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("prob", ["ot2pi", "blast"])
 def test_residual_exact_mode_is_bitwise(prob, p):
     """In the exact mode the residual is the oracle's (and so the
@@ -93,7 +108,7 @@ def test_residual_exact_mode_is_bitwise(prob, p):
     assert np.array_equal(ut, ut_ref) and dtl == dtl_ref
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("prob", PROBS)
 @pytest.mark.parametrize("oedg", [0, 1])
 def test_post_process_matches_oracle(prob, p, oedg):
@@ -109,7 +124,7 @@ def test_post_process_matches_oracle(prob, p, oedg):
     err = normwise(strips(got), strips(ref))
     print(f"post_process {prob} P{p} oedg={oedg}: {err:.3e}, ent {r['tot_ent']} vs {r_ref['tot_ent']}")
     assert not r["neg_avg"] and not r_ref["neg_avg"]
-    assert err <= 1e-12
+    assert err <= (oedg_tol(p, 1e-12) if oedg else 1e-12)
     assert r["num_negcell"] == r_ref["num_negcell"]
     assert abs(r["tot_ent"] - r_ref["tot_ent"]) <= 1e-12 * abs(r_ref["tot_ent"])
     avg = g.download_avg()
@@ -146,7 +161,7 @@ def test_positivity_limiter_matches_oracle(p):
     assert q[0].min() > 0 and prs.min() > 0
 
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("prob,oedg_mode,cfl", [("ot2pi", 1, 0.2), ("rotor", 1, 0.2), ("blast", 2, 0.15)])
 def test_solve_matches_oracle(prob, oedg_mode, cfl, p):
     nx, ny = 10, 9
@@ -163,9 +178,40 @@ def test_solve_matches_oracle(prob, oedg_mode, cfl, p):
     assert s["status"] == 0 and s_ref["status"] == 0
     assert s["steps"] == s_ref["steps"] and s["retries"] == s_ref["retries"]
     assert s["negcells"] == s_ref["negcells"]
-    assert err <= 1e-10
+    assert err <= oedg_tol(p, 1e-10)
     assert rel_l1(got[0], ref[0]) <= 1e-10
-    assert abs(s["t"] - s_ref["t"]) <= 1e-12 * s_ref["t"]  # dt follows dt_lim, which carries the LDF round-off
+    # dt follows dt_lim, which carries the LDF round-off (at P6 lifted by the OEDG conditioning)
+    assert abs(s["t"] - s_ref["t"]) <= (1e-12 if p <= 5 else 1e-10) * s_ref["t"]
+
+
+@pytest.mark.parametrize("p", [5, 6])
+@pytest.mark.parametrize("prob", ["ot2pi", "blast"])
+def test_high_degree_oedg_without_ldf_matches_oracle(prob, p):
+    """Degrees 5 and 6 with OEDG on and LDF off (the state then carries no
+    last-bit difference from the LDF product's order): the exact device path
+    follows the oracle to round-off, so the P6 tolerance above is the
+    algorithm's conditioning, not a device error."""
+    o, g = make_pair(prob, 9, 8, p)
+    g.set_arith("exact")
+    o.init_problem(prob)
+    g.init_problem(prob)
+    cfl, mode = (0.15, 2) if prob == "blast" else (0.2, 1)
+    s_ref = o.run(-1.0, 3, cfl, mode, False)
+    s = g.solve(-1.0, 3, cfl, mode, False)
+    got, ref = interior(g.download_state(0)), interior(o.u)
+    err = normwise(got, ref)
+    # the oracle's own response to one ulp added to B_x, over the same steps:
+    # the global OEDG means are summed per chunk here and serially there, and
+    # the blast's pressure ratio of 1e4 lifts such a last-bit difference (the
+    # oracle's spread: 3.7e-12 at P4, 7.7e-11 at P6)
+    o2 = make_pair(prob, 9, 8, p)[0]
+    o2.init_problem(prob)
+    o2.u[4] = np.nextafter(o2.u[4], np.inf)
+    o2.run(-1.0, 3, cfl, mode, False)
+    spread = normwise(interior(o2.u), ref)
+    print(f"{prob} P{p} OEDG without LDF: normwise {err:.3e} (oracle's one-ulp spread {spread:.3e})")
+    assert s["steps"] == s_ref["steps"] and s["negcells"] == s_ref["negcells"]
+    assert err <= max(1e-12, 10.0 * spread)
 
 
 def test_solve_to_final_time_matches_oracle():
@@ -211,7 +257,8 @@ def test_solve_other_rk_methods_match_oracle(prob, rk, cfl):
 
 
 GOLDEN_END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12",
-              "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5", "end_ot2pi_p2_128"]
+              "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5", "end_ot2pi_p2_128",
+              "end_ot2pi_p6_8", "end_rotor_p5_10"]
 
 
 @pytest.mark.parametrize("name", GOLDEN_END)
@@ -238,7 +285,7 @@ def test_solve_matches_reference_fixture(name):
 
 # ------------------------------------------------------------------ ent_offset
 
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("oedg", [0, 1])
 def test_post_process_ent_offset_matches_oracle(p, oedg):
     """ent_offset (DG_solver.f90:195-225,605-633) in one post_process call,
@@ -294,7 +341,7 @@ def test_solve_ent_offset_matches_reference_fixture():
 # --------------------------------------------------------- alternative fluxes
 
 @pytest.mark.parametrize("arith", ["exact", "fast"])
-@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("p", DEGREES)
 @pytest.mark.parametrize("flux", ["pp_hll", "espp_lf", "ec_lf"])
 def test_residual_alternative_flux_matches_oracle(flux, p, arith):
     """PP_HLL_flux_x / ESPP_LF_flux_x / EC_LF_flux_x at the face nodes

@@ -173,10 +173,11 @@ LDF0 = ["stages_ot2pi_p2_ldf0", "stages_rotor_p3_ldf0", "stages_blast_p2_ldf0",
         "stages_briowu_p3_ldf0", "stages_cloud_p2_ldf0", "stages_jet_p2_ldf0", "stages_yee_p2_ldf0",
         "stages_alfven_p2_ldf0", "stages_wave1d_p3_ldf0",
         "stages_blast_p2_entoff_ldf0", "stages_ot2pi_p3_entoff_ldf0",
-        "stages_briowu_p3_py0_ldf0", "stages_wave1d_p3_py0_ldf0"]
+        "stages_briowu_p3_py0_ldf0", "stages_wave1d_p3_py0_ldf0",
+        "stages_ot2pi_p5_ldf0", "stages_blast_p6_ldf0", "stages_wave1d_p5_py0_ldf0"]
 LDF1 = ["stages_ot2pi_p2", "stages_rotor_p3", "stages_blast_p1", "stages_blast_p4", "stages_otunit_p3",
         "stages_khi_p2", "stages_jet_p3", "stages_cloud_p1", "stages_rotor_p2_entoff", "stages_khi_p3_entoff",
-        "stages_briowu_p2_py0"]
+        "stages_briowu_p2_py0", "stages_rotor_p5", "stages_blast_p6", "stages_briowu_p6_py0"]
 
 
 @pytest.mark.parametrize("name", LDF0)
@@ -200,8 +201,10 @@ def test_stages_bit_exact_without_ldf(name):
 
 # With LDF on, the projector's factorisation differs from the reference's in
 # the last bits (3e-15 on B); the blast problem's pressure ratio of 1e4 lifts
-# that to ~1e-12 relative in the momentum after one step, hence 1e-11 there.
-LDF_TOL = {"stages_blast_p1": 1e-11, "stages_blast_p4": 1e-11}
+# that to ~1e-12 relative in the momentum after one step, hence 1e-11 there;
+# the P6 projector (98 x 98, entries up to ~10 times the P4 one's) leaves
+# 3e-11 after the first step's limited final stage, hence 1e-10.
+LDF_TOL = {"stages_blast_p1": 1e-11, "stages_blast_p4": 1e-11, "stages_blast_p6": 1e-10}
 
 
 @pytest.mark.parametrize("name", LDF1)
@@ -240,7 +243,7 @@ def test_multistep_final_state(name):
 END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12",
        "end_ot2pi_p2_16_rk1", "end_ot2pi_p2_16_rk2", "end_rotor_p3_16_rk5",
        "end_alfven_p2_8", "end_wave1d_p3_16_rk5", "end_cloud_p2_16", "end_jet_p2_20", "end_briowu_p2_32",
-       "end_ot2pi_p2_16_entoff", "end_briowu_main0", "end_wave1d_main1"]
+       "end_ot2pi_p2_16_entoff", "end_briowu_main0", "end_wave1d_main1", "end_ot2pi_p6_8", "end_rotor_p5_10"]
 
 
 # The Brio-Wu strip (480 x 1 elements, P3 x P0) carries round-off noise on its
