@@ -575,7 +575,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 
     // acc waits in LDS (in the place of the primitives, which are no longer
     // read) while the face fluxes take the registers
-    double(*sacc)[8] = reinterpret_cast<double(*)[8]>(&sprim[0][0].rho);
+    // (rows of 9 doubles where the primitives leave room: a 64-byte row
+    // stride puts every fourth lane on the same LDS bank)
+    constexpr int SA = sizeof(PT) >= 9 * sizeof(double) ? 9 : 8;
+    double(*sacc)[SA] = reinterpret_cast<double(*)[SA]>(&sprim[0][0].rho);
     if (valid)
 #pragma unroll
         for (int v = 0; v < 8; ++v) sacc[t][v] = acc[v];
@@ -979,14 +982,16 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
     double(*scons)[8][NN] = reinterpret_cast<double(*)[8][NN]>(sunion);
     double(*sdb)[M] = reinterpret_cast<double(*)[M]>(sunion);
     double(*sred)[2] = reinterpret_cast<double(*)[2]>(sunion);
-    __shared__ double savg[EPB][8];
+    // (rows padded to 9 and 17 doubles: 64- and 128-byte row strides put the
+    // rows of every fourth / second element on the same LDS bank)
+    __shared__ double savg[EPB][9];
     // the LDF operator as stored: sldf[c * M + o] takes input c to output o
     // (o < NN: Bx of node o; otherwise By of node o - NN)
     __shared__ double sldf[M * M];
     __shared__ int sok[EPB];
     __shared__ int scnt[EPB];
     __shared__ double sent[EPB];
-    __shared__ double sem[3][EPB][8];
+    __shared__ double sem[3][EPB][9];
     // ent_offset records (the EO variant only: the default variant keeps its registers and LDS)
     __shared__ double sce[EO ? EPB : 1][2];
     __shared__ double sth[EO ? EPB : 1];
@@ -1258,7 +1263,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
 // max|u - c| = max(max u - c, c - min u) exactly.
 __global__ void __launch_bounds__(BLOCK) k_reduce_global(Inst *insts, const Blk *map, int st, int flags, int final)
 {
-    __shared__ double ss[BLOCK][8], smn[BLOCK][8], smx[BLOCK][8];
+    __shared__ double ss[BLOCK][9], smn[BLOCK][9], smx[BLOCK][9];  // (9: see k_stage_pp's savg)
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
@@ -1589,7 +1594,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
     __shared__ double sent[EPB];
     __shared__ int scnt[EPB];
     __shared__ double sth[EO ? EPB : 1];
-    __shared__ double ssig[EPB][16];
+    __shared__ double ssig[EPB][17];  // (17: a 128-byte row stride puts every second element on one LDS bank)
     __shared__ double smax[64 % NN == 0 ? 1 : EPB][2];  // element's largest signal speeds (LDS path)
 
     const Blk bm = map[blockIdx.x];
