@@ -194,6 +194,15 @@ int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *co
 #define PPES_ARITH_EXACT 0
 #define PPES_ARITH_FAST 1
 int ppes_set_arith(ppes_solver *s, int mode);
+/* Stage combine in the residual launch (no reference counterpart; on by
+ * default): in SSP-RK2 and RK3 the step's launch that computes L(u_s) for a
+ * later stage s + 1 also forms that stage's combined state from u^n, L(u_s)
+ * and u_s (drive.f90:148-211), the operations the stage kernel would do, and
+ * stores it in place of L(u_s); the stage kernel then reads one array
+ * instead of three. Results are bit for bit those with it off (off is for
+ * comparison). The residual of u^n, whose dt is not known until it ends,
+ * and ppes_semidiscretize keep storing L(u). */
+int ppes_set_precombine(ppes_solver *s, int on);
 
 /* ---- Batched instances (no reference counterpart: the reference runs one
  * problem per process, drive.f90:32-343). A batch holds independent
@@ -244,6 +253,7 @@ int ppes_batch_memory(ppes_batch *b, int id, long long *bytes);
 long long ppes_instance_bytes(const ppes_config *cfg);
 int ppes_batch_set_timing(ppes_batch *b, int on);
 int ppes_batch_set_arith(ppes_batch *b, int mode);
+int ppes_batch_set_precombine(ppes_batch *b, int on);
 int ppes_batch_kernel_time(ppes_batch *b, int kernel, double *total_ms, long long *count);
 
 /* Residual autotuner (no reference counterpart). For each degree among the

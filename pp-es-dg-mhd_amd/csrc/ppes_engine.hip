@@ -923,7 +923,7 @@ int Engine::launch_step(bool all_kernels)
     for (int st = 1; st <= S; ++st) {
         for (auto &g : groups) {
             Timed tm(this, KT_STAGE);
-            STAGE_DISPATCH(g, g.eo, st, F_COMBINE);
+            STAGE_DISPATCH(g, g.eo, st, F_COMBINE | (st >= 2 && precomb ? F_PRECOMB : 0));
         }
         if (any_oedg_stage[st]) {
             for (auto &g0 : groups) {
@@ -954,7 +954,7 @@ int Engine::launch_step(bool all_kernels)
         LAUNCH1(k_pre_residual, n_all, d_recs, (const int *)d_all, n_all, 0);
         for (auto &g : groups) {
             Timed tm(this, KT_RESIDUAL);
-            RESIDUAL_DISPATCH(g, st, (st % S) + 1, 0);
+            RESIDUAL_DISPATCH(g, st, (st % S) + 1, st < S && precomb ? 2 : 0);
         }
     }
     LAUNCH1(k_step_end, n_all, d_recs, (const int *)d_all, n_all);
@@ -989,7 +989,7 @@ int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
         RESIDUAL_DISPATCH(g, 0, 1, 1);
     }
     for (int st = 1; st <= S; ++st) {
-        STAGE_DISPATCH(g, g.eo, st, F_COMBINE);
+        STAGE_DISPATCH(g, g.eo, st, F_COMBINE | (st >= 2 && precomb ? F_PRECOMB : 0));
         if (const Group *og = oedg_group(g, st)) {
             const Group &o = *og;
             LAUNCH(k_reduce_global, o.n_red1, d_recs, (const Blk *)o.d_red1, st, 0, 0);
@@ -1000,7 +1000,7 @@ int Engine::launch_group_step(Group &g, bool any_retry, hipStream_t stream)
         if (st == S) LAUNCH(k_reduce_ent, (long long)nm, d_recs, dl, st, 0);
         ENG_DISPATCH(g.sid, LAUNCH((k_bc<PX, PY>), g.n_bc, d_recs, (const Blk *)g.d_bc, st));
         LAUNCH1(k_pre_residual, nm, d_recs, dl, nm, 0);
-        RESIDUAL_DISPATCH(g, st, (st % S) + 1, 0);
+        RESIDUAL_DISPATCH(g, st, (st % S) + 1, st < S && precomb ? 2 : 0);
     }
     LAUNCH1(k_step_end, nm, d_recs, dl, nm);
     return PPES_OK;

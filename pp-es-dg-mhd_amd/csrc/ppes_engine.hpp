@@ -99,6 +99,14 @@ public:
     int run_steps(long long nsteps, bool use_graph, long long *done_steps);
     int sync_every = 1;                                          // steps queued per host synchronisation
     int arith = PPES_ARITH_FAST;                                 // residual arithmetic (PPES_ARITH_*)
+    // the step's residual launches of L(u_st), st < S, store stage st + 1's
+    // combined state for the stage kernel (precombine_ok); same bits either way
+    bool precomb = true;
+    void set_precombine(bool on)
+    {
+        precomb = on;
+        graph_ok = false;
+    }
     void set_arith(int a)
     {
         arith = a;

@@ -109,7 +109,12 @@ __device__ __forceinline__ void rk_combine_node(const StageSrc &c, long long os,
 // DG_semidiscretize (DG_solver.f90:663-892): EC flux differencing with the
 // Godunov-Powell source, ES-HLL face fluxes, surface terms and dt_lim.
 // mode 0: every live instance, ut[ut_slot] = L(u[u_slot]);
-// mode 1: only instances retrying a step (ut[1] = L(u^n), drive.f90:104).
+// mode 1: only instances retrying a step (ut[1] = L(u^n), drive.f90:104);
+// mode 2: as mode 0, from a step's launch of L(u_st) with st < S: where
+// precombine_ok, ut[ut_slot] takes the combined state of stage st + 1
+// instead, combine3 of u^n, L(u_st) and u_st as the stage kernel would form
+// it (the same operations on the same values), which that kernel then reads
+// alone (F_PRECOMB).
 // FAST: the shared-reciprocal forms of the EC and ES-HLL fluxes (a few ulp
 // from the reference's rounding); otherwise the reference's written forms.
 // Build-time shape of the fast residual: waves per SIMD asked of the register
@@ -684,6 +689,18 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     if (valid) {
         const unsigned ob = eg * NN + n;
         const long long vsi = I.vsi;
+        if (mode == 2 && precombine_ok(I) && u_slot >= 1 && u_slot < I.S) {
+            const gdbl *un = gin(I.u[0]);
+            double a[8], b[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                a[v] = un[v * vs + base];
+                b[v] = up[v][base];
+            }
+            const double dt = I.ctl.dt;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) acc[v] = combine3(I.S, u_slot + 1, dt, a[v], acc[v], b[v]);
+        }
 #pragma unroll
         for (int v = 0; v < 8; ++v) (ut + v * vsi)[ob] = acc[v];
     }
@@ -1035,15 +1052,25 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
         }
     }
 
-    // ---- the stage state at this node
+    // ---- the stage state at this node (combined here, or by the residual
+    // launch that formed the stage's residual slot)
+    const bool pre = (flags & F_PRECOMB) && precombine_ok(I) && st >= 2;
     double q[8];
+    auto combined = [&](long long os, long long ot) {
+        if (pre) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) q[v] = src.t[ot + v * I.vsi];
+        } else {
+            rk_combine_node(src, os, ot, vs, I.vsi, q);
+        }
+    };
     if (valid) {
         if (!(flags & F_COMBINE) || (!interior && (flags & F_GHOST_MEM))) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) q[v] = out[v * vs + base];
         } else if (interior) {
             const long long gt = ((long long)(ey - 1) * I.nx + (ex - 1)) * NN + n;
-            rk_combine_node(src, base, gt, vs, I.vsi, q);
+            combined(base, gt);
         } else {
             const GhostSrc gsrc = ghost_src<NX, NY>(I, ex, ey, k, l);
             if (gsrc.mode == 1) {
@@ -1052,7 +1079,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
             } else {
                 const long long gs = gidx(I, gsrc.sx, gsrc.sy) * NN + gsrc.sn;
                 const long long gt = ((long long)(gsrc.sy - 1) * I.nx + (gsrc.sx - 1)) * NN + gsrc.sn;
-                rk_combine_node(src, gs, gt, vs, I.vsi, q);
+                combined(gs, gt);
 #pragma unroll
                 for (int v = 0; v < 8; ++v) q[v] = ghost_value(gsrc.mode, v, q[v]);
             }

@@ -798,6 +798,13 @@ extern "C" int ppes_set_arith(ppes_solver *s, int mode)
     return PPES_OK;
 }
 
+extern "C" int ppes_set_precombine(ppes_solver *s, int on)
+{
+    if (!s) return PPES_BAD_ARG;
+    s->eng.set_precombine(on != 0);
+    return PPES_OK;
+}
+
 extern "C" int ppes_kernel_time(ppes_solver *s, int kernel, double *total_ms, long long *count)
 {
     return s ? engine_kernel_time(s->eng, kernel, total_ms, count) : PPES_BAD_ARG;
@@ -941,6 +948,13 @@ extern "C" int ppes_batch_set_arith(ppes_batch *b, int mode)
 {
     if (!b || (mode != PPES_ARITH_EXACT && mode != PPES_ARITH_FAST)) return PPES_BAD_ARG;
     b->eng.set_arith(mode);
+    return PPES_OK;
+}
+
+extern "C" int ppes_batch_set_precombine(ppes_batch *b, int on)
+{
+    if (!b) return PPES_BAD_ARG;
+    b->eng.set_precombine(on != 0);
     return PPES_OK;
 }
 

@@ -143,6 +143,7 @@ enum PPFlags {
     F_COMBINE_ONLY = 16,  // write the combined interior and stop
     F_ENT = 32,           // compute the entropy (else: on the final stage)
     F_NO_ENT = 64,
+    F_PRECOMB = 128,      // the residual slot of this stage holds the combined state (precombine_ok)
 };
 
 struct Blk {
@@ -229,6 +230,12 @@ __device__ __forceinline__ bool stage_ent(const Inst &I, int st, int flags)
     if (flags & F_NO_ENT) return false;
     return st == I.S;
 }
+
+// May the step's residual launch of L(u_st), st < S, store the combined
+// state of stage st + 1 in place of L(u_st)? In SSP-RK2 and RK3 that stage
+// reads u^n, L(u_st) and u_st only (drive.f90:148-211), which that launch
+// holds or reads once; RK5 reads L(u3) in two stages (drive.f90:213-272).
+__device__ __forceinline__ bool precombine_ok(const Inst &I) { return I.S == 2 || I.S == 3; }
 
 // OEDG dt of stage st: dt on the final stage (drive.f90:280) or beta_s dt on
 // every stage (drive.f90:283)

@@ -114,6 +114,8 @@ def lib():
             "ppes_batch_positivity": [vp, ctypes.c_int, dp, dp],
             "ppes_batch_set_timing": [vp, ctypes.c_int],
             "ppes_batch_kernel_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
+            "ppes_set_precombine": [vp, ctypes.c_int],
+            "ppes_batch_set_precombine": [vp, ctypes.c_int],
             "ppes_tiles": [vp, ctypes.c_int, ip],
             "ppes_batch_tiles": [vp, ctypes.c_int, ip],
             "ppes_set_tile": [vp, ctypes.c_int, ctypes.c_int],
@@ -352,6 +354,10 @@ class Solver:
         """"exact" (the reference's rounding order) or "fast" (default)"""
         _check(lib().ppes_set_arith(self.h, ARITH[mode]), "ppes_set_arith")
 
+    def set_precombine(self, on):
+        """stage combine in the residual launches of later stages (default on; same bits)"""
+        _check(lib().ppes_set_precombine(self.h, int(bool(on))), "ppes_set_precombine")
+
     def autotune(self, reps=5):
         """Residual workgroup size by timing (after solve_begin); returns
         {"P<px>x<py>": lanes} for the tuned shape."""
@@ -469,6 +475,9 @@ class Batch:
 
     def set_arith(self, mode):
         _check(lib().ppes_batch_set_arith(self.h, ARITH[mode]), "ppes_batch_set_arith")
+
+    def set_precombine(self, on):
+        _check(lib().ppes_batch_set_precombine(self.h, int(bool(on))), "ppes_batch_set_precombine")
 
     def autotune(self, reps=5):
         """Residual workgroup size per degree among the active instances, by
