@@ -43,14 +43,18 @@ METRIC = "DOF-stage updates/s, batched vs plain at matched L1 err; p50/p99 solve
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory) HBM bytes per GLL node for one launch of each
-# kernel class: 8 FP64 variables = 64 B per node per state stream.
-#   residual: read u, write ut (neighbour traces and ghosts not counted)
-#   stage:    read u^n, ut (+ u^(s-1) in stages 2, 3), write u^(s): mean over
-#             the three SSP-RK3 stages (cell averages, LDF, limiter fused)
+# kernel class: 8 FP64 variables = 64 B per node per state stream; means over
+# the three launches of an SSP-RK3 step.
+#   residual: read u, write ut (neighbour traces and ghosts not counted); the
+#             launches of L(u1) and L(u2) also read u^n and store the next
+#             stage's combined state in place of ut: 128, 192, 192
+#   stage:    stage 1 reads u^n and ut, stages 2 and 3 the combined state;
+#             each writes its stage (cell averages, LDF, limiter fused): 192,
+#             128, 128
 #   oedg:     read u once (face indicators)
 #   limit:    read u, write u (OEDG damping + limiter)
-ALG_BYTES_PER_NODE = {"residual": 128.0, "stage": (192.0 + 256.0 + 256.0) / 3.0, "oedg": 64.0,
-                      "limit": 128.0, "reduce": 0.0, "bc": 0.0}
+ALG_BYTES_PER_NODE = {"residual": (128.0 + 192.0 + 192.0) / 3.0, "stage": (192.0 + 128.0 + 128.0) / 3.0,
+                      "oedg": 64.0, "limit": 128.0, "reduce": 0.0, "bc": 0.0}
 # whole path, SSP-RK3 with OEDG on the final stage: 640 B per node-step
 STEP_BYTES_PER_NODE_STAGE = 640.0 / 3.0
 
