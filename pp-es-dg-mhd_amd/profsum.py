@@ -40,6 +40,11 @@ CLASS_OF = {"k_residual": "residual", "k_stage_pp": "stage", "k_oedg_faces": "oe
 ALG_BYTES_PER_NODE = {"residual": (128.0 + 192.0 + 192.0) / 3.0, "stage": (192.0 + 128.0 + 128.0) / 3.0, "oedg": 64.0,
                       "limit": 128.0}
 LAUNCHES_PER_STEP = {"residual": 3, "stage": 3}
+# launches of a step that read about 64 and about 128 B per node: (launches
+# of each per SSP-RK3 step, algorithmic bytes per node of each); the residual
+# of u^n stores L(u^n), those of u1 and u2 the combined state; stage 1 reads
+# u^n and L(u^n), stages 2 and 3 the combined state
+STEP_KINDS = {"residual": ((1, 2), (128.0, 192.0)), "stage": ((2, 1), (128.0, 192.0))}
 
 
 def concurrent_launches(tr):
@@ -158,18 +163,37 @@ def traffic(a):
         keys = [k for k in F if k[0] == f"ppes::{kname}<{p}>" or k[0].startswith(f"ppes::{kname}<{p},")]
         if not keys:
             continue
-        # the variant the steps run: the most dispatches (the others are autotuner trials)
-        name = max({k[0] for k in keys}, key=lambda n: sum(len(F[k]) for k in keys if k[0] == n))
-        g = max(k[1] for k in keys if k[0] == name)
-        f, w = F[(name, g)], Wr[(name, g)]
-        fmax, wmax = max(f), max(w)
-        f = [x for x in f if x > 0.02 * fmax]
-        w = [x for x in w if x > 0.02 * wmax]
-        rb = sum(f) / len(f) * 1024.0 * ff
-        wb = sum(w) / len(w) * 1024.0 * fw
-        alg = ALG_BYTES_PER_NODE[a.kernel] * nodes_p[p]
-        per_degree[p] = {"read_bytes": rb, "write_bytes": wb, "alg_bytes": alg, "traffic_over_alg": (rb + wb) / alg,
-                         "launches_counted": len(f)}
+        # the variant the steps run: the most dispatches (the others are
+        # autotuner trials), in each run on its own (their autotuners time
+        # separately); its full launches (largest grid, counter over 2% of
+        # the largest)
+        def full(C):
+            keys = [k for k in C if k[0] == f"ppes::{kname}<{p}>" or k[0].startswith(f"ppes::{kname}<{p},")]
+            name = max({k[0] for k in keys}, key=lambda n: sum(len(C[k]) for k in keys if k[0] == n))
+            v = C[(name, max(k[1] for k in keys if k[0] == name))]
+            return [x * 1024.0 for x in v if x > 0.02 * max(v)]
+        f = [x * ff for x in full(F)]
+        w = [x * fw for x in full(Wr)]
+        # every launch writes one state stream (64 B per node)
+        wb = sum(w) / len(w)
+        rec_p = {"launches_counted": len(f)}
+        if a.kernel in STEP_KINDS:
+            # launches of the two kinds of a step, told apart by the bytes they
+            # read per node (about 64 or 128), weighted as a step runs them
+            (w_lo, w_hi), (alg_lo, alg_hi) = STEP_KINDS[a.kernel]
+            lo = [x for x in f if x < 96.0 * nodes_p[p]]
+            hi = [x for x in f if x >= 96.0 * nodes_p[p]]
+            kinds = [(k, v, wt, al) for k, v, wt, al in (("lo", lo, w_lo, alg_lo), ("hi", hi, w_hi, alg_hi)) if v]
+            tw = sum(wt for _, _, wt, _ in kinds)
+            rb = sum(wt * sum(v) / len(v) for _, v, wt, _ in kinds) / tw
+            alg = sum(wt * al for _, _, wt, al in kinds) / tw * nodes_p[p]
+            for k, v, _, al in kinds:
+                rec_p[k] = {"launches": len(v), "read_bytes": sum(v) / len(v), "alg_bytes": al * nodes_p[p]}
+        else:
+            rb = sum(f) / len(f)
+            alg = ALG_BYTES_PER_NODE[a.kernel] * nodes_p[p]
+        per_degree[p] = dict({"read_bytes": rb, "write_bytes": wb, "alg_bytes": alg,
+                              "traffic_over_alg": (rb + wb) / alg}, **rec_p)
         step_read += rb
         step_write += wb
     groups = len(per_degree)
