@@ -647,42 +647,49 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         const double p_tot = p_fld + p_mag;
         const double b_dot_u = bx * vx + by * vy + bz * vz;
         const double GP[8] = {0.0, bx, by, bz, vx, vy, vz, b_dot_u};
-        // left then right, bottom then top; with one node in y (PY = 0) the
-        // bottom and top terms both land on the node, in that order
+        // the x-face term (left or right), then the y-face term (bottom or
+        // top); with one node in y (PY = 0) the bottom and top terms both
+        // land on the node, in that order. A node lies on one x-face at most
+        // (N >= 2), so the lanes of the left and right faces share one body:
+        // the face's record, its factor with the side's sign (acc - c X and
+        // acc + (-c) X are the same bits) and its signal speed (sr at the left
+        // face, -sl at the right) are selected per lane; likewise bottom and
+        // top.
         if (k == 0 || k == NX - 1) {
             const double F[8] = {mx, vx * mx - bx * bx + p_tot, vx * my - by * bx + 0.0, vx * mz - bz * bx + 0.0,
                                  0.0, -ez, ey, vx * (e_tot + p_tot) - bx * b_dot_u};
-            if (k == 0) {
-                const double *o = &sbuf[el][l * S::FREC];
-                const double c = I.cface[0];
+            const bool lf = k == 0;
+            const double *o = &sbuf[el][(lf ? l : NY + l) * S::FREC];
+            const double c = lf ? -I.cface[0] : I.cface[1];
 #pragma unroll
-                for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (F[v] - o[v] + GP[v] * (bx - o[8]));
-                ax = dmax(ax, o[10]);  // sr at the left face
-            }
-            if (k == NX - 1) {
-                const double *o = &sbuf[el][(NY + l) * S::FREC];
-                const double c = I.cface[1];
-#pragma unroll
-                for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (F[v] - o[v] + GP[v] * (bx - o[8]));
-                ax = dmax(ax, -o[9]);  // -sl at the right face
-            }
+            for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (F[v] - o[v] + GP[v] * (bx - o[8]));
+            ax = dmax(ax, lf ? o[10] : -o[9]);
         }
         if (l == 0 || l == NY - 1) {
             const double G[8] = {my, vy * mx - bx * by + 0.0, vy * my - by * by + p_tot, vy * mz - bz * by + 0.0,
                                  ez, 0.0, -ex, vy * (e_tot + p_tot) - by * b_dot_u};
-            if (l == 0) {
-                const double *o = &sbuf[el][(2 * NY + k) * S::FREC];
-                const double c = I.cface[2];
-#pragma unroll
-                for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (G[v] - o[v] + GP[v] * (by - o[8]));
-                ay = dmax(ay, o[10]);
-            }
-            if (l == NY - 1) {
-                const double *o = &sbuf[el][(2 * NY + NX + k) * S::FREC];
-                const double c = I.cface[3];
+            if constexpr (NY > 1) {
+                const bool bf = l == 0;
+                const double *o = &sbuf[el][(2 * NY + (bf ? 0 : NX) + k) * S::FREC];
+                const double c = bf ? -I.cface[2] : I.cface[3];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (G[v] - o[v] + GP[v] * (by - o[8]));
-                ay = dmax(ay, -o[9]);
+                ay = dmax(ay, bf ? o[10] : -o[9]);
+            } else {
+                {
+                    const double *o = &sbuf[el][(2 * NY + k) * S::FREC];
+                    const double c = I.cface[2];
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) acc[v] = acc[v] - c * (G[v] - o[v] + GP[v] * (by - o[8]));
+                    ay = dmax(ay, o[10]);
+                }
+                {
+                    const double *o = &sbuf[el][(2 * NY + NX + k) * S::FREC];
+                    const double c = I.cface[3];
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) acc[v] = acc[v] + c * (G[v] - o[v] + GP[v] * (by - o[8]));
+                    ay = dmax(ay, -o[9]);
+                }
             }
         }
     }
