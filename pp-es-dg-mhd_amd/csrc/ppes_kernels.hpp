@@ -759,10 +759,14 @@ __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, i
 
 // Zhang-Shu positivity scaling of one element toward its mean
 // (DG_solver.f90:528-603). Called by every thread of the workgroup (it may
-// synchronise); lanes with valid = false pass neutral values.
+// synchronise); lanes with valid = false pass neutral values. prs_q: the
+// node's pressure prs_macro(q) of the returned q where the limiter's first
+// pressure check formed it and the node was not scaled after it, else NaN
+// (prs_known then forms it again).
 template <int NN>
 __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8], bool valid, double g,
-                                                 double (*sred)[2], int t, int el, bool &neg_node, bool &fail)
+                                                 double (*sred)[2], int t, int el, bool &neg_node, bool &fail,
+                                                 double &prs_q)
 {
     // density, once
     const double den_min = elem_min<NN>(valid ? q[0] : DBL_HUGE, sred, t, el, valid);
@@ -782,11 +786,13 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
     bool active = false;
     for (int round = 0; round < 4; ++round) {
         if (round > 0 && !__syncthreads_or(active ? 1 : 0)) break;
-        const double prs_min =
-            elem_min<NN>(valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE, sred, t, el, valid);
+        const double p_node = valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE;
+        if (round == 0) prs_q = p_node;
+        const double prs_min = elem_min<NN>(p_node, sred, t, el, valid);
         if (!valid) continue;
         if (round == 0) {
             if (prs_min < PP_EPS) {
+                prs_q = __builtin_nan("");
                 neg_node = true;
                 active = true;
                 theta_p = dmax(dmin((prs_avg - PP_EPS) / (prs_avg - prs_min), 1.0), 0.0);
@@ -798,6 +804,7 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
                 if (round == 3) {
                     fail = true;
                 } else {
+                    prs_q = __builtin_nan("");
                     theta_p = 0.6 * theta_p;
 #pragma unroll
                     for (int v = 0; v < 8; ++v) q[v] = avg[v] + theta_p * (q[v] - avg[v]);
@@ -807,6 +814,13 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
             }
         }
     }
+}
+
+// prs_macro(q), or the pressure the limiter left in known (not NaN) for
+// this same q: the same bits
+__device__ __forceinline__ double prs_known(const double q[8], double known, double g)
+{
+    return known == known ? known : prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g);
 }
 
 // The smallest density and pressure over the nodes of a workgroup that
@@ -819,12 +833,12 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
 // one (every workgroup of an instance updating one address serialises at
 // the L2 otherwise; a stale read only costs an extra atomic). A NaN enters as
 // -inf, below every number. Called by every thread of the workgroup.
-__device__ __forceinline__ void record_positivity(Inst &I, bool on, const double q[8])
+__device__ __forceinline__ void record_positivity(Inst &I, bool on, const double q[8], double prs_q)
 {
     double r = DBL_HUGE, p = DBL_HUGE;
     if (on) {
         r = q[0];
-        p = prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], I.g);
+        p = prs_known(q, prs_q, I.g);
         r = r == r ? r : -__builtin_inf();
         p = p == p ? p : -__builtin_inf();
     }
@@ -850,11 +864,11 @@ __device__ __forceinline__ void record_positivity(Inst &I, bool on, const double
 // how many chunks a workgroup holds.
 template <int NN, int EPB, int CE>
 __device__ __forceinline__ double entropy_partial(const double q[8], bool valid, bool counted, const Inst &I,
-                                                  double (*sred)[2], double *sent, int *scnt, int t, int el, int n)
+                                                  double (*sred)[2], double *sent, int *scnt, int t, int el, int n,
+                                                  double prs_q)
 {
     const double g = I.g;
-    sred[t][0] = valid ? I.went[n] * ent_macro(q[0], dmax(prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g), PP_EPS), g)
-                       : 0.0;
+    sred[t][0] = valid ? I.went[n] * ent_macro(q[0], dmax(prs_known(q, prs_q, g), PP_EPS), g) : 0.0;
     __syncthreads();
     if (el < EPB && n == 0) {
         double ce = 0.0;
@@ -1271,22 +1285,26 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
     // ---- PP path: positivity limiter of interior elements, entropy, final state
     bool neg_node = false, fail = false;
     const bool lim = interior && ok;
-    pp_limit_element<NN>(q, avg, lim, I.g, sred, t, el, neg_node, fail);
+    double prs_q;
+    pp_limit_element<NN>(q, avg, lim, I.g, sred, t, el, neg_node, fail, prs_q);
     if (lim && n == 0) {
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
     }
     if constexpr (EO)
-        if (eo) entropy_offset<NN, EPB>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
+        if (eo) {
+            entropy_offset<NN, EPB>(q, avg, lim, lim ? sce[el][0] : 0.0, lim ? sce[el][1] : 0.0, I, sred, sth, t, el, n);
+            prs_q = __builtin_nan("");
+        }
     if (stage_ent(I, st, flags)) {
-        const double s = entropy_partial<NN, EPB, CE>(q, lim, lim, I, sred, sent, scnt, t, el, n);
+        const double s = entropy_partial<NN, EPB, CE>(q, lim, lim, I, sred, sent, scnt, t, el, n, prs_q);
         const long long c = (long long)bm.local * MT_ + t;
         if (t < MT_ && c < I.nchk_slot) gout(I.entpart)[c] = s;
     }
     if (valid && ok)
 #pragma unroll
         for (int v = 0; v < 8; ++v) out[v * vs + base] = q[v];
-    record_positivity(I, lim, q);
+    record_positivity(I, lim, q, prs_q);
 }
 
 // Two-level deterministic reduction of the per-chunk (sum, min, max)
@@ -1630,6 +1648,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
     __shared__ double sth[EO ? EPB : 1];
     __shared__ double ssig[EPB][17];  // (17: a 128-byte row stride puts every second element on one LDS bank)
     __shared__ double smax[64 % NN == 0 ? 1 : EPB][2];  // element's largest signal speeds (LDS path)
+    __shared__ double sdel[64 % NN == 0 ? 1 : EPB][8];  // the element's lanes' largest deltas (LDS path)
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
@@ -1663,22 +1682,25 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
     }
     {
         // the element's damping factor (DG_solver.f90:448-524) from its
-        // largest signal speeds: by lane exchanges when the element is an
-        // aligned lane group of one wavefront (P1, P3), else by its first lane
-        // from LDS, which also forms theta once for the element
-        auto theta_of = [&](double mex, double mey) {
-            double dm = 0.0;
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
+        // largest signal speeds (by lane exchanges when the element is an
+        // aligned lane group of one wavefront, P1 and P3, else by LDS
+        // atomics) and the largest of its eight variables' deltas. Lane n < 8
+        // forms the delta of variable n (and n + N, ... for elements of fewer
+        // than 8 nodes), and the maximum over the lanes (max is exact in any
+        // order, and every delta is >= 0) goes by lane exchanges or through
+        // LDS: one division per lane instead of eight.
+        auto delta_max = [&](double mex, double mey) {
+            double dv = -DBL_HUGE;
+            for (int v = n; v < 8; v += NN) {
                 const double var = I.ctl.glob[8 + v];
                 double delta;
                 if (var > OEDG_EPS)
                     delta = ((mex / I.hx) * ssig[el][v] + (mey / I.hy) * ssig[el][8 + v]) / var;
                 else
                     delta = 0.0;
-                dm = (v == 0) ? delta : dmax(dm, delta);
+                dv = dmax(dv, delta);
             }
-            return exp(-stage_oedg_dt(I, st) * dm);
+            return dv;
         };
         double sx = 0.0, sy = 0.0;
         if (valid) node_speeds(dmax(q[0], PP_EPS), q[1], q[2], q[3], q[4], q[5], q[6], q[7], g, sx, sy);
@@ -1686,7 +1708,10 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
         if constexpr (64 % NN == 0) {
             const double mex = seg_max<NN>(sx, t & 63, n), mey = seg_max<NN>(sy, t & 63, n);
             __syncthreads();  // the face indicator sums are in LDS
-            if (valid) theta = theta_of(mex, mey);
+            double dm = delta_max(mex, mey);
+#pragma unroll
+            for (int m = 1; m < NN; m <<= 1) dm = dmax(dm, __shfl_xor(dm, m));
+            if (valid) theta = exp(-stage_oedg_dt(I, st) * dm);
         } else {
             // one LDS atomic maximum per lane into the element's cells
             // (zeroed with the face sums' load; the speeds are >= 0)
@@ -1696,7 +1721,14 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
                 lds_max(&smax[el][1], sy);
             }
             __syncthreads();
-            if (valid) theta = theta_of(smax[el][0], smax[el][1]);
+            if (valid && n < 8) sdel[el][n] = delta_max(smax[el][0], smax[el][1]);
+            __syncthreads();
+            if (valid) {
+                double dm = sdel[el][0];
+#pragma unroll
+                for (int m = 1; m < (NN < 8 ? NN : 8); ++m) dm = dmax(dm, sdel[el][m]);
+                theta = exp(-stage_oedg_dt(I, st) * dm);
+            }
         }
         if (valid) {
 #pragma unroll
@@ -1705,7 +1737,8 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
         __syncthreads();
     }
     bool neg_node = false, fail = false;
-    pp_limit_element<NN>(q, avg, valid, g, sred, t, el, neg_node, fail);
+    double prs_q;
+    pp_limit_element<NN>(q, avg, valid, g, sred, t, el, neg_node, fail, prs_q);
     if (valid && n == 0) {
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
@@ -1714,9 +1747,10 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
         if (I.ctl.ent_offset) {
             const gdbl *ce = gin(I.cent) + ge * 2;
             entropy_offset<NN, EPB>(q, avg, valid, valid ? ce[0] : 0.0, valid ? ce[1] : 0.0, I, sred, sth, t, el, n);
+            prs_q = __builtin_nan("");
         }
     if (stage_ent(I, st, flags)) {
-        const double s = entropy_partial<NN, EPB, CE>(q, valid, valid, I, sred, sent, scnt, t, el, n);
+        const double s = entropy_partial<NN, EPB, CE>(q, valid, valid, I, sred, sent, scnt, t, el, n, prs_q);
         const long long c = (long long)bm.local * MT_ + t;
         if (t < MT_ && c < I.nchk_el) gout(I.entpart)[c] = s;
     }
@@ -1724,7 +1758,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
 #pragma unroll
         for (int v = 0; v < 8; ++v) u[v * vs + base] = q[v];
     }
-    record_positivity(I, valid, q);
+    record_positivity(I, valid, q, prs_q);
 }
 
 // total entropy of each instance: one workgroup per instance, contiguous
