@@ -589,7 +589,13 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         for (int v = 0; v < 8; ++v) sacc[t][v] = acc[v];
 
     // ---- face fluxes of the element's four faces; both neighbours compute a
-    // shared face from identical inputs, so they agree bit for bit
+    // shared face from identical inputs, so they agree bit for bit. The fast
+    // ES-HLL form also forms the own node's surface term there (TERMS): its
+    // record is that term (in the original component order) and the
+    // element's signal speed at the face; otherwise the record is the face
+    // flux, B_n* and the two wave speeds, and the surface phase forms the
+    // term.
+    constexpr bool TERMS = FAST && FLUX == FLUX_ESPP_HLL;
 #pragma unroll
     for (int r = 0; r < FR; ++r) {
         // the fast form loads each round's traces just before its face
@@ -609,17 +615,26 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             else if constexpr (FLUX == FLUX_EC_LF) ec_lf_flux_x(wl, wr, g, tf, nB, sl, sr);
             else if constexpr (FAST) {
                 // the element's own trace node: the right state of its left
-                // and bottom faces, the left state of its right and top faces
+                // and bottom faces, the left state of its right and top faces;
+                // the term's factor is negative on the left and bottom faces
                 const int side = f < 2 * NY ? f / NY : 2 + (f - 2 * NY) / NX;
                 const int a = f < 2 * NY ? f - side * NY : f - 2 * NY - (side - 2) * NX;
                 const int no = side == 0 ? NX * a : side == 1 ? (NX - 1) + NX * a : side == 2 ? a : a + NX * (NY - 1);
-                espp_hll_flux_x_fast(wl, wr, g, inv_gm1, side == 1 || side == 3, slog[el][no][0], slog[el][no][1], tf,
-                                     nB, sl, sr);
+                const bool own_l = side == 1 || side == 3;
+                const double ct = side == 0 ? -I.cface[0] : side == 1 ? I.cface[1] : side == 2 ? -I.cface[2] : I.cface[3];
+                double term[8];
+                espp_hll_flux_x_fast<true>(wl, wr, g, inv_gm1, own_l, slog[el][no][0], slog[el][no][1], tf, nB, sl, sr,
+                                           ct, term);
+                double *o = &sbuf[el][f * S::FREC];
+                swap_xy_if(yf, term, o);
+                o[8] = own_l ? -sl : sr;
             }
             else espp_hll_flux_x(wl, wr, g, tf, nB, sl, sr);
-            double *o = &sbuf[el][f * S::FREC];
-            swap_xy_if(yf, tf, o);
-            o[8] = nB; o[9] = sl; o[10] = sr;
+            if constexpr (!TERMS) {
+                double *o = &sbuf[el][f * S::FREC];
+                swap_xy_if(yf, tf, o);
+                o[8] = nB; o[9] = sl; o[10] = sr;
+            }
         }
     }
     __syncthreads();
@@ -633,7 +648,25 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 #pragma unroll
         for (int v = 0; v < 8; ++v) acc[v] = sacc[t][v];
     }
-    if (valid && (k == 0 || k == NX - 1 || l == 0 || l == NY - 1)) {
+    if constexpr (TERMS) {
+        // the terms the face phase formed: the x-face's, then the y-face's
+        // (with one node in y, bottom then top)
+        auto add = [&](int f, double &s) {
+            const double *o = &sbuf[el][f * S::FREC];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) acc[v] = acc[v] + o[v];
+            s = dmax(s, o[8]);
+        };
+        if (valid && (k == 0 || k == NX - 1)) add(k == 0 ? l : NY + l, ax);
+        if (valid && (l == 0 || l == NY - 1)) {
+            if constexpr (NY > 1) {
+                add(2 * NY + (l == 0 ? 0 : NX) + k, ay);
+            } else {
+                add(2 * NY + k, ay);
+                add(2 * NY + NX + k, ay);
+            }
+        }
+    } else if (valid && (k == 0 || k == NX - 1 || l == 0 || l == NY - 1)) {
 #pragma unroll
         for (int v = 0; v < 8; ++v) q[v] = up[v][base];
         const double mx = q[1], my = q[2], mz = q[3], bx = q[4], by = q[5], bz = q[6], e_tot = q[7];

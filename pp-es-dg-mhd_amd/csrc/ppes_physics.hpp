@@ -462,10 +462,15 @@ __device__ __forceinline__ void ec_flux_fast(const PrimL &L, const PrimL &R, dou
 // lnp_o = ln p, on the left when own_l) come from the node primitives; the
 // other element's are formed here, from pfld_sym, as that element forms its
 // own (and swap_xy leaves them unchanged): both elements of a face still
-// compute the same bits.
+// compute the same bits. TERM: also the own node's surface term
+// sterm = ct ((F_own - F*) + GP_own (B_n,own - B_n*)) (DG_solver.f90:872-882),
+// from the physical flux and the Godunov-Powell source vector of the own
+// state that the face flux forms anyway (ct carries the face's sign).
+template <bool TERM = false>
 __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const double wr[8], double g, double inv_gm1,
                                                      bool own_l, double lnr_o, double lnp_o, double nflux[8],
-                                                     double &nBn, double &sl_out, double &sr_out)
+                                                     double &nBn, double &sl_out, double &sr_out, double ct = 0.0,
+                                                     double *sterm = nullptr)
 {
 #pragma clang fp contract(on)
     const double rho_l = wl[0], mx_l = wl[1], my_l = wl[2], mz_l = wl[3], bx_l = wl[4], by_l = wl[5], bz_l = wl[6], e_l = wl[7];
@@ -544,6 +549,14 @@ __device__ __forceinline__ void espp_hll_flux_x_fast(const double wl[8], const d
     nBn = wei_l * bx_l + wei_r * bx_r;
     sl_out = sl;
     sr_out = sr;
+    if constexpr (TERM) {
+        const double bxo = own_l ? bx_l : bx_r;
+        const double GP[8] = {0.0, bxo, own_l ? by_l : by_r, own_l ? bz_l : bz_r, own_l ? vx_l : vx_r,
+                              own_l ? vy_l : vy_r, own_l ? vz_l : vz_r, own_l ? b_dot_u_l : b_dot_u_r};
+        const double db = bxo - nBn;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sterm[i] = ct * (((own_l ? fl[i] : fr[i]) - nflux[i]) + GP[i] * db);
+    }
 }
 
 // x <-> y component swap (swap_xy, DG_solver.f90:1556-1568)
