@@ -639,10 +639,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
     }
     __syncthreads();
 
-    // ---- surface terms, x-faces then y-faces (DG_solver.f90:872-882). The
-    // node's state is read again (a cache hit) rather than kept in registers
-    // through the flux phases, its velocity recomputed as the primitives
-    // were, and acc comes back from LDS.
+    // ---- surface terms, x-faces then y-faces (DG_solver.f90:872-882); acc
+    // comes back from LDS. Without TERMS the node's state is read again (a
+    // cache hit) rather than kept in registers through the flux phases, and
+    // its velocity recomputed as the primitives were.
     double ax = msx, ay = msy;
     if (valid) {
 #pragma unroll
