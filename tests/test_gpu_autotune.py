@@ -113,3 +113,41 @@ def test_exact_mode_is_not_tuned():
     g.advance(-1.0, 1, 0.2, 1)
     assert g.autotune(2) == {}
     g.close()
+
+
+# odd grids (face rows that a workgroup's faces straddle, a one-element-wide
+# grid) and every-stage OEDG, whose earlier stages run the face sums over a
+# sub-plan of the group
+ODD = [("ot2pi", 7, 5, 2, 0.2, 1), ("blast", 13, 3, 4, 0.15, 2), ("rotor", 1, 9, 3, 0.2, 1),
+       ("ot2pi", 31, 2, 1, 0.2, 1), ("blast", 9, 11, 2, 0.15, 2)]
+
+
+def run_odd(tiles):
+    b = ppesdg.Batch(3)
+    ids = []
+    for prob, nx, ny, p, cfl, oedg in ODD:
+        s = setup(prob)
+        i = b.add(nx, ny, p, s["domain"], s["gamma"], s["bc"], s["bc_state"])
+        b.init_problem(i, prob)
+        b.start(i, -1.0, 6, cfl, oedg, True)
+        ids.append(i)
+    for kernel, lanes in tiles.items():
+        for p in (1, 2, 3, 4):
+            if kernel != "residual" or lanes >= (p + 1) ** 2:
+                b.set_tile(kernel, p, p, lanes)
+    b.step(0, True)
+    out = [(b.download_state(i, 0), b.status(i)) for i in ids]
+    b.close()
+    return out
+
+
+@pytest.mark.parametrize("lanes", [64, 128, 256])
+def test_every_face_and_residual_workgroup_size_keeps_results(lanes):
+    """the OEDG face sums and the residual at 64, 128 and 256 lanes (their
+    block maps in XCD order: row-interleaved face blocks, column-strip
+    residual blocks) give the bits of the default sizes on odd grids"""
+    ref = run_odd({})
+    got = run_odd({"oedg": lanes, "residual": lanes})
+    for (u0, s0), (u1, s1), m in zip(ref, got, ODD):
+        assert np.array_equal(u0, u1), (m, lanes)
+        assert s0 == s1, (m, lanes)
