@@ -769,25 +769,33 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
 // element. When NN divides the wavefront (P1, P3) the element's lanes are one
 // aligned group of a wavefront and a butterfly of lane exchanges does it with
 // no barrier; otherwise (elements straddle wavefronts) through LDS: one
-// atomic minimum per lane into the element's cell, which the element's
-// first lane has set to DBL_HUGE (called by every thread of the workgroup).
-// min is exact, so both give the value of the reference's loop.
+// atomic minimum per lane into the element's cell scell[el][c], then one
+// barrier. Every minimum a kernel takes has a cell of its own, set to
+// DBL_HUGE before the kernel's first barrier (limiter_cells_init), so that no
+// barrier guards a cell's reset or reuse. Called by every thread of the
+// workgroup. min is exact, so both give the value of the reference's loop.
+constexpr int LIM_CELLS = 5;  // the density, then the pressure of up to four rounds
 template <int NN>
-__device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, int el, bool valid)
+__device__ __forceinline__ double elem_min(double x, double (*scell)[LIM_CELLS], int el, int c, bool valid)
 {
     if constexpr (64 % NN == 0) {
 #pragma unroll
         for (int m = 1; m < NN; m <<= 1) x = dmin(x, __shfl_xor(x, m));
         return x;
     } else {
-        if (valid && t == el * NN) sred[t][1] = DBL_HUGE;
+        if (valid) lds_min(&scell[el][c], x);
         __syncthreads();
-        if (valid) lds_min(&sred[el * NN][1], x);
-        __syncthreads();
-        const double r = valid ? sred[el * NN][1] : DBL_HUGE;
-        __syncthreads();
-        return r;
+        return valid ? scell[el][c] : DBL_HUGE;
     }
+}
+
+// the limiter's cells of a workgroup of TB threads and EPB elements, before
+// the kernel's first barrier (elements straddling wavefronts only)
+template <int NN, int EPB, int TB>
+__device__ __forceinline__ void limiter_cells_init(double (*scell)[LIM_CELLS], int t)
+{
+    if constexpr (64 % NN != 0)
+        for (int a = t; a < EPB * LIM_CELLS; a += TB) scell[a / LIM_CELLS][a % LIM_CELLS] = DBL_HUGE;
 }
 
 // Zhang-Shu positivity scaling of one element toward its mean
@@ -798,11 +806,11 @@ __device__ __forceinline__ double elem_min(double x, double (*sred)[2], int t, i
 // (prs_known then forms it again).
 template <int NN>
 __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8], bool valid, double g,
-                                                 double (*sred)[2], int t, int el, bool &neg_node, bool &fail,
+                                                 double (*scell)[LIM_CELLS], int el, bool &neg_node, bool &fail,
                                                  double &prs_q)
 {
     // density, once
-    const double den_min = elem_min<NN>(valid ? q[0] : DBL_HUGE, sred, t, el, valid);
+    const double den_min = elem_min<NN>(valid ? q[0] : DBL_HUGE, scell, el, 0, valid);
     neg_node = false;
     fail = false;
     if (valid && den_min < PP_EPS) {
@@ -821,7 +829,7 @@ __device__ __forceinline__ void pp_limit_element(double q[8], const double avg[8
         if (round > 0 && !__syncthreads_or(active ? 1 : 0)) break;
         const double p_node = valid ? prs_macro(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], g) : DBL_HUGE;
         if (round == 0) prs_q = p_node;
-        const double prs_min = elem_min<NN>(p_node, sred, t, el, valid);
+        const double prs_min = elem_min<NN>(p_node, scell, el, 1 + round, valid);
         if (!valid) continue;
         if (round == 0) {
             if (prs_min < PP_EPS) {
@@ -1066,12 +1074,14 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
     // ent_offset records (the EO variant only: the default variant keeps its registers and LDS)
     __shared__ double sce[EO ? EPB : 1][2];
     __shared__ double sth[EO ? EPB : 1];
+    __shared__ double scell[64 % NN == 0 ? 1 : EPB][LIM_CELLS];
 
     if ((int)blockIdx.x >= nmap) return;
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I)) return;
     const int t = threadIdx.x;
+    limiter_cells_init<NN, EPB, TB>(scell, t);
     const int el = t / NN, n = t - el * NN;
     const int k = n % NX, l = n / NX;
     const long long nslots = (long long)(I.nx + 2) * (I.ny + 2);
@@ -1319,7 +1329,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) __attrib
     bool neg_node = false, fail = false;
     const bool lim = interior && ok;
     double prs_q;
-    pp_limit_element<NN>(q, avg, lim, I.g, sred, t, el, neg_node, fail, prs_q);
+    pp_limit_element<NN>(q, avg, lim, I.g, scell, el, neg_node, fail, prs_q);
     if (lim && n == 0) {
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
@@ -1682,11 +1692,13 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
     __shared__ double ssig[EPB][17];  // (17: a 128-byte row stride puts every second element on one LDS bank)
     __shared__ double smax[64 % NN == 0 ? 1 : EPB][2];  // element's largest signal speeds (LDS path)
     __shared__ double sdel[64 % NN == 0 ? 1 : EPB][8];  // the element's lanes' largest deltas (LDS path)
+    __shared__ double scell[64 % NN == 0 ? 1 : EPB][LIM_CELLS];
 
     const Blk bm = map[blockIdx.x];
     Inst &I = insts[bm.inst];
     if (!inst_live(I) || !stage_oedg(I, st, flags)) return;
     const int t = threadIdx.x;
+    limiter_cells_init<NN, EPB, TB>(scell, t);
     const int el = t / NN, n = t - el * NN;
     const long long eg = (long long)bm.local * EPB + el;
     const bool valid = (el < EPB) && (eg < (long long)I.nx * I.ny);
@@ -1771,7 +1783,7 @@ __global__ void __launch_bounds__(tile_lanes((PX + 1) * (PY + 1), MT_)) k_oedg_l
     }
     bool neg_node = false, fail = false;
     double prs_q;
-    pp_limit_element<NN>(q, avg, valid, g, sred, t, el, neg_node, fail, prs_q);
+    pp_limit_element<NN>(q, avg, valid, g, scell, el, neg_node, fail, prs_q);
     if (valid && n == 0) {
         if (neg_node) atomicAdd(&I.ctl.num_negcell, 1);
         if (fail) atomicOr(&I.ctl.pp_fail, 1);
