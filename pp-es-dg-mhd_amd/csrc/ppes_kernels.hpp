@@ -512,6 +512,39 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             o[5] = ec[6];
             o[6] = ec[7];
         };
+        // the same record from the lane's own node (held in registers across
+        // the full rounds, elements of 16 nodes or more: the others' lanes
+        // have no registers to spare) and its partner: the fast EC flux is
+        // symmetric in its two states bit for bit (sums commute, the
+        // logarithmic means' differences change sign in numerator and
+        // denominator alike, rcp_fast(-x) = -rcp_fast(x)), so the order of
+        // the pair does not matter
+        constexpr bool OWN = NN >= 16;
+        PT me;
+        if constexpr (OWN) me = sprim[el][valid ? n : 0];
+        auto pair_rec_own = [&](int nB, bool ydir, double *o) {
+            if constexpr (!OWN) {
+                pair_rec(n > nB ? n : nB, n > nB ? nB : n, ydir, o);
+            } else {
+                PT A = me, B = sprim[el][nB];
+                if (ydir) {
+                    double x;
+                    x = A.vx; A.vx = A.vy; A.vy = x;
+                    x = A.bx; A.bx = A.by; A.by = x;
+                    x = B.vx; B.vx = B.vy; B.vy = x;
+                    x = B.bx; B.bx = B.by; B.by = x;
+                }
+                double ec[8];
+                ec_flux_fast<0>(A, B, inv_gm1, ec);
+                o[0] = ec[0];
+                o[1] = ydir ? ec[2] : ec[1];
+                o[2] = ydir ? ec[1] : ec[2];
+                o[3] = ec[3];
+                o[4] = ec[5];
+                o[5] = ec[6];
+                o[6] = ec[7];
+            }
+        };
         auto fsub_x = [&](double a, const double *o) {
             acc[0] = __builtin_fma(-a, o[0], acc[0]); acc[1] = __builtin_fma(-a, o[1], acc[1]);
             acc[2] = __builtin_fma(-a, o[2], acc[2]); acc[3] = __builtin_fma(-a, o[3], acc[3]);
@@ -531,7 +564,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         for (int d = 1; d <= XD; ++d) {
             if (nround++) __syncthreads();
             const int kp = k + d < NX ? k + d : k + d - NX, km = k >= d ? k - d : k - d + NX;
-            if (valid) pair_rec((kp > k ? kp : k) + NX * l, (kp > k ? k : kp) + NX * l, false, &sbuf[el][n * REC]);
+            if (valid) pair_rec_own(kp + NX * l, false, &sbuf[el][n * REC]);
             __syncthreads();
             if (valid) {
                 fsub_x(cx * dx_at(kp), &sbuf[el][n * REC]);
@@ -543,7 +576,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         for (int d = 1; d <= YD; ++d) {
             if (nround++) __syncthreads();
             const int lp = l + d < NY ? l + d : l + d - NY, lm = l >= d ? l - d : l - d + NY;
-            if (valid) pair_rec(k + NX * (lp > l ? lp : l), k + NX * (lp > l ? l : lp), true, &sbuf[el][n * REC]);
+            if (valid) pair_rec_own(k + NX * lp, true, &sbuf[el][n * REC]);
             __syncthreads();
             if (valid) {
                 fsub_y(cy * dy_at(lp), &sbuf[el][n * REC]);

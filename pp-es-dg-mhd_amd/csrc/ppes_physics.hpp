@@ -423,9 +423,14 @@ __device__ __forceinline__ void ec_flux_fast(const PrimL &L, const PrimL &R, dou
     const double beta_bar = 0.5 * (L.beta + R.beta);
     const double v_sq_bar = 0.5 * ((L.vx * L.vx + L.vy * L.vy + L.vz * L.vz) + (R.vx * R.vx + R.vy * R.vy + R.vz * R.vz));
     const double b_sq_bar = 0.5 * ((L.bx * L.bx + L.by * L.by + L.bz * L.bz) + (R.bx * R.bx + R.by * R.by + R.bz * R.bz));
-    const double betavx_bar = 0.5 * (L.beta * L.vx + R.beta * R.vx);
-    const double betavy_bar = 0.5 * (L.beta * L.vy + R.beta * R.vy);
-    const double betavz_bar = 0.5 * (L.beta * L.vz + R.beta * R.vz);
+    // (each side's product rounded on its own, so that no fused multiply-add
+    // takes one side's product unrounded: the flux is then symmetric in L and
+    // R bit for bit, as every other mean here is)
+    const double bvx_l = L.beta * L.vx, bvy_l = L.beta * L.vy, bvz_l = L.beta * L.vz;
+    const double bvx_r = R.beta * R.vx, bvy_r = R.beta * R.vy, bvz_r = R.beta * R.vz;
+    const double betavx_bar = 0.5 * (bvx_l + bvx_r);
+    const double betavy_bar = 0.5 * (bvy_l + bvy_r);
+    const double betavz_bar = 0.5 * (bvz_l + bvz_r);
     const double inv_bb = rcp_fast(beta_bar);
     // rho_hat, the logarithmic mean of rho, and 1 / ((g - 1) beta_hat): one
     // division each, of the operands the mean's branch selects
