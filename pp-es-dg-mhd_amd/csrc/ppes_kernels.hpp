@@ -514,11 +514,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         };
         // the same record from the lane's own node (held in registers across
         // the full rounds, elements of 16 nodes or more: the others' lanes
-        // have no registers to spare) and its partner: the fast EC flux is
-        // symmetric in its two states bit for bit (sums commute, the
+        // have no registers to spare) and its partner, the own node first.
+        // The fast EC flux is symmetric in its two states (sums commute, the
         // logarithmic means' differences change sign in numerator and
-        // denominator alike, rcp_fast(-x) = -rcp_fast(x)), so the order of
-        // the pair does not matter
+        // denominator alike, each side's products are rounded on their own),
+        // and each record is formed once and read by both of its nodes
         constexpr bool OWN = NN >= 16;
         PT me;
         if constexpr (OWN) me = sprim[el][valid ? n : 0];
