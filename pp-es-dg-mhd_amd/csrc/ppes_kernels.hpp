@@ -522,9 +522,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         constexpr bool OWN = NN >= 16;
         PT me;
         if constexpr (OWN) me = sprim[el][valid ? n : 0];
-        auto pair_rec_own = [&](int nB, bool ydir, double *o) {
+        auto pair_rec_own = [&](int nB, bool ydir, double *o, double (&reg)[7]) {
             if constexpr (!OWN) {
                 pair_rec(n > nB ? n : nB, n > nB ? nB : n, ydir, o);
+#pragma unroll
+                for (int c = 0; c < 7; ++c) reg[c] = o[c];
             } else {
                 PT A = me, B = sprim[el][nB];
                 if (ydir) {
@@ -536,13 +538,15 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
                 }
                 double ec[8];
                 ec_flux_fast<0>(A, B, inv_gm1, ec);
-                o[0] = ec[0];
-                o[1] = ydir ? ec[2] : ec[1];
-                o[2] = ydir ? ec[1] : ec[2];
-                o[3] = ec[3];
-                o[4] = ec[5];
-                o[5] = ec[6];
-                o[6] = ec[7];
+                reg[0] = ec[0];
+                reg[1] = ydir ? ec[2] : ec[1];
+                reg[2] = ydir ? ec[1] : ec[2];
+                reg[3] = ec[3];
+                reg[4] = ec[5];
+                reg[5] = ec[6];
+                reg[6] = ec[7];
+#pragma unroll
+                for (int c = 0; c < 7; ++c) o[c] = reg[c];
             }
         };
         auto fsub_x = [&](double a, const double *o) {
@@ -564,10 +568,12 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         for (int d = 1; d <= XD; ++d) {
             if (nround++) __syncthreads();
             const int kp = k + d < NX ? k + d : k + d - NX, km = k >= d ? k - d : k - d + NX;
-            if (valid) pair_rec_own(kp + NX * l, false, &sbuf[el][n * REC]);
+            // (the lane's own record is summed from its registers)
+            double reg[7];
+            if (valid) pair_rec_own(kp + NX * l, false, &sbuf[el][n * REC], reg);
             __syncthreads();
             if (valid) {
-                fsub_x(cx * dx_at(kp), &sbuf[el][n * REC]);
+                fsub_x(cx * dx_at(kp), reg);
                 fsub_x(cx * dx_at(km), &sbuf[el][(km + NX * l) * REC]);
             }
         }
@@ -576,10 +582,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
         for (int d = 1; d <= YD; ++d) {
             if (nround++) __syncthreads();
             const int lp = l + d < NY ? l + d : l + d - NY, lm = l >= d ? l - d : l - d + NY;
-            if (valid) pair_rec_own(k + NX * lp, true, &sbuf[el][n * REC]);
+            double reg[7];
+            if (valid) pair_rec_own(k + NX * lp, true, &sbuf[el][n * REC], reg);
             __syncthreads();
             if (valid) {
-                fsub_y(cy * dy_at(lp), &sbuf[el][n * REC]);
+                fsub_y(cy * dy_at(lp), reg);
                 fsub_y(cy * dy_at(lm), &sbuf[el][(k + NX * lm) * REC]);
             }
         }
