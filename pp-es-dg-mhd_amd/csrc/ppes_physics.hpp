@@ -391,7 +391,7 @@ __device__ __forceinline__ double recip(double b)
 
 struct LogMean {
     bool small;
-    double F, d, dl;
+    double F, d, dl, rs;  // rs = 1 / (a + b)
 };
 
 __device__ __forceinline__ LogMean log_mean_parts(double a, double b, double la, double lb)
@@ -401,7 +401,8 @@ __device__ __forceinline__ LogMean log_mean_parts(double a, double b, double la,
     const double s = a + b;
     m.d = a - b;
     m.dl = la - lb;
-    const double f = m.d * rcp_fast(s);
+    m.rs = rcp_fast(s);
+    const double f = m.d * m.rs;
     const double u = f * f;
     m.small = u < 1e-2;
     m.F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0 + u * (1.0 / 11.0 +
@@ -431,11 +432,13 @@ __device__ __forceinline__ void ec_flux_fast(const PrimL &L, const PrimL &R, dou
     const double betavx_bar = 0.5 * (bvx_l + bvx_r);
     const double betavy_bar = 0.5 * (bvy_l + bvy_r);
     const double betavz_bar = 0.5 * (bvz_l + bvz_r);
-    const double inv_bb = rcp_fast(beta_bar);
     // rho_hat, the logarithmic mean of rho, and 1 / ((g - 1) beta_hat): one
     // division each, of the operands the mean's branch selects
     const LogMean mr = log_mean_parts(L.rho, R.rho, L.lnr, R.lnr);
     const LogMean mb = log_mean_parts(L.beta, R.beta, L.lnb, R.lnb);
+    // 1 / beta_bar = 2 / (beta_l + beta_r), the reciprocal the beta mean
+    // forms (the scaling by 2 is exact)
+    const double inv_bb = 2.0 * mb.rs;
     const double rho_hat = (mr.small ? rho_bar : mr.d) * rcp_fast(mr.small ? mr.F : mr.dl);
     const double e_coef = ((mb.small ? mb.F : mb.dl) * rcp_fast(mb.small ? beta_bar : mb.d)) * inv_gm1;
     if (DIR == 0) {
