@@ -426,8 +426,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(resid_w
             ec_flux<0>(sprim[el][kk + NX * lr], sprim[el][rr + NX * lr], g, ec);
             o[0] = ec[0]; o[1] = ec[1]; o[2] = ec[2]; o[3] = ec[3]; o[4] = ec[5]; o[5] = ec[6]; o[6] = ec[7];
         } else {
+            // (one row, HY = 0, has no y-pairs and never takes this branch)
+            constexpr int HYD = HY > 0 ? HY : 1;
             const int qq = qp - XP;
-            const int kc = qq / HY, a = qq - kc * HY;
+            const int kc = qq / HYD, a = qq - kc * HYD;
             int ll = 1;
             while ((ll + 1) * ll / 2 <= a) ++ll;
             const int ss = a - ll * (ll - 1) / 2;
