@@ -254,7 +254,6 @@ END = ["end_blast_p2_40", "end_rotor_p2_32", "end_ot2pi_p1_32", "end_ot2pi_p4_12
 END_TOL = {"end_briowu_main0": 5e-8}
 
 
-@pytest.mark.long
 @pytest.mark.parametrize("name", ["end_briowu_main0", "end_wave1d_main1"])
 def test_main_cases_bit_exact_with_reference_tables(name):
     """main.f90 cases 0 and 1 (degree 0 in y, SSP-RK(5,4)) to t_final, with
@@ -269,12 +268,7 @@ def test_main_cases_bit_exact_with_reference_tables(name):
     assert np.array_equal(interior(o.u)[0], g["rho_final"])
 
 
-# the runs of several minutes and more on the CPU (the 480-element strips take
-# 1409 RK5 steps on one thread)
-LONG = {"end_blast_p2_40", "end_briowu_main0", "end_wave1d_main1"}
-
-
-@pytest.mark.parametrize("name", [pytest.param(n, marks=pytest.mark.long) if n in LONG else n for n in END])
+@pytest.mark.parametrize("name", END)
 def test_end_to_end(name):
     """The reference's solve_execute to t_final (RK3, and RK1 / RK2 / RK5 in
     the *_rk* cases)."""
@@ -327,7 +321,6 @@ def test_error_functionals_closed_form():
     assert abs(l2 - 2e-3 * math.sqrt(8 * area)) <= 1e-13
 
 
-@pytest.mark.long
 def test_long_orszag_tang_fixture():
     """P2, 128 x 128, t = 0.5 (the CPU-runnable configuration), when present."""
     path = os.path.join(GOLD, "end_ot2pi_p2_128.npz")
